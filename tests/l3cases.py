@@ -1,0 +1,311 @@
+"""Shared helpers for the L3 parity tests.
+
+`Cblas` wraps any library with the CBLAS L3 signatures (the product's
+``cblas_*``, the reference build's ``cblas_*`` or the oracle's ``orc_*``),
+taking numpy arrays.  The case generators follow the reference's own
+correctness harness (test/correctness/check_cblas.py:157-473): seed 314159,
+5x4x3-sized problems, NaN in every element the routine must not read, and
+the untouched triangle checked bit for bit.
+"""
+import ctypes
+import itertools
+import os
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_LIB = os.path.join(ROOT, "oracle", "liboracle.so")
+REF_AVX2_LIB = os.path.join(ROOT, "oracle", "_ref", "libref_avx2.so")
+REF_REF_LIB = os.path.join(ROOT, "oracle", "_ref", "libref_ref.so")
+
+ROW, COL = 101, 102
+NOTR, TR = 111, 112
+UP, LO = 121, 122
+NONUNIT, UNIT = 131, 132
+LEFT, RIGHT = 141, 142
+
+# check_cblas.py:24-37
+TOL = {np.dtype(np.float32): (5e-4, 5e-5), np.dtype(np.float64): (2e-11, 1e-12)}
+SEED = 314159
+DTYPES = [np.dtype(np.float32), np.dtype(np.float64)]
+LAYOUTS = [("C", ROW), ("F", COL)]
+
+_VP = ctypes.c_void_p
+_I = ctypes.c_int
+
+
+def ld_of(x, order):
+    axis = 0 if order == "C" else 1
+    return x.strides[axis] // x.itemsize
+
+
+def ptr(x):
+    if x is None:
+        return None
+    if hasattr(x, "data_ptr"):
+        return _VP(x.data_ptr())
+    return _VP(x.ctypes.data)
+
+
+class Cblas:
+    """Call CBLAS-shaped routines of a ctypes library by prefix."""
+
+    def __init__(self, path_or_lib, prefix="cblas_", returns_status=False):
+        self.lib = (ctypes.CDLL(path_or_lib) if isinstance(path_or_lib, str)
+                    else path_or_lib)
+        self.prefix = prefix
+        self.returns_status = returns_status
+        for p, ct in (("s", ctypes.c_float), ("d", ctypes.c_double)):
+            sig = {
+                "gemm": [_I, _I, _I, _I, _I, _I, ct, _VP, _I, _VP, _I, ct, _VP, _I],
+                "symm": [_I, _I, _I, _I, _I, ct, _VP, _I, _VP, _I, ct, _VP, _I],
+                "syrk": [_I, _I, _I, _I, _I, ct, _VP, _I, ct, _VP, _I],
+                "syr2k": [_I, _I, _I, _I, _I, ct, _VP, _I, _VP, _I, ct, _VP, _I],
+                "trmm": [_I, _I, _I, _I, _I, _I, _I, ct, _VP, _I, _VP, _I],
+                "trsm": [_I, _I, _I, _I, _I, _I, _I, ct, _VP, _I, _VP, _I],
+            }
+            for r, args in sig.items():
+                f = getattr(self.lib, f"{prefix}{p}{r}")
+                f.argtypes = args
+                f.restype = _I if returns_status else None
+
+    def fn(self, dtype, routine):
+        p = "s" if np.dtype(dtype) == np.float32 else "d"
+        return getattr(self.lib, f"{self.prefix}{p}{routine}")
+
+    def call(self, dtype, routine, *args):
+        conv = [ptr(a) if (isinstance(a, np.ndarray) or hasattr(a, "data_ptr"))
+                else a for a in args]
+        return self.fn(dtype, routine)(*conv)
+
+
+def make(values, dtype, order):
+    return np.array(values, dtype=dtype, order=order, copy=True)
+
+
+def rand(rng, shape, dtype, order):
+    return make(rng.standard_normal(shape), dtype, order)
+
+
+def symmetric_storage(rng, size, dtype, order, lower):
+    """check_cblas.py:192-200: the unstored triangle is NaN."""
+    v = rng.standard_normal((size, size))
+    if lower:
+        v[np.triu_indices(size, 1)] = np.nan
+    else:
+        v[np.tril_indices(size, -1)] = np.nan
+    return make(v, dtype, order)
+
+
+def triangular_storage(rng, size, dtype, order, lower, unit):
+    """check_cblas.py:202-222: NaN outside the triangle and on a unit
+    diagonal; off-diagonal U(-0.25, 0.25), diagonal U(1.5, 2.5)."""
+    v = rng.uniform(-0.25, 0.25, size=(size, size))
+    if lower:
+        v[np.triu_indices(size, 1)] = np.nan
+    else:
+        v[np.tril_indices(size, -1)] = np.nan
+    if unit:
+        np.fill_diagonal(v, np.nan)
+    else:
+        np.fill_diagonal(v, rng.uniform(1.5, 2.5, size=size))
+    return make(v, dtype, order)
+
+
+def conditioned_triangular(rng, size, dtype, order, lower, unit):
+    """Larger triangular operands: off-diagonal N(0,1)/sqrt(size), diagonal
+    1 + |N(0,1)| (the well-conditioned DTRSM workload's distribution), NaN
+    outside the triangle and on a unit diagonal."""
+    v = rng.standard_normal((size, size)) / np.sqrt(size)
+    if lower:
+        v[np.triu_indices(size, 1)] = np.nan
+    else:
+        v[np.tril_indices(size, -1)] = np.nan
+    if unit:
+        np.fill_diagonal(v, np.nan)
+    else:
+        np.fill_diagonal(v, 1.0 + np.abs(rng.standard_normal(size)))
+    return make(v, dtype, order)
+
+
+def tri_dense(a, lower, unit):
+    """Dense float64 triangular matrix from poisoned storage."""
+    t = np.tril(np.nan_to_num(a.astype(np.float64), nan=0.0)) if lower else \
+        np.triu(np.nan_to_num(a.astype(np.float64), nan=0.0))
+    if unit:
+        np.fill_diagonal(t, 1.0)
+    return t
+
+
+def sym_dense(a, lower):
+    x = np.nan_to_num(a.astype(np.float64), nan=0.0)
+    t = np.tril(x) if lower else np.triu(x)
+    return t + t.T - np.diag(np.diag(t))
+
+
+# ---------------------------------------------------------------- cases ----
+def gemm_cases():
+    for dt, (order, co), (ta, tb) in itertools.product(
+            DTYPES, LAYOUTS, itertools.product((False, True), repeat=2)):
+        yield dict(routine="gemm", dtype=dt, order=order, corder=co, ta=ta, tb=tb)
+
+
+def symm_cases():
+    for dt, (order, co), right, lower in itertools.product(
+            DTYPES, LAYOUTS, (False, True), (False, True)):
+        yield dict(routine="symm", dtype=dt, order=order, corder=co,
+                   right=right, lower=lower)
+
+
+def rk_cases(routine):
+    for dt, (order, co), trans, lower in itertools.product(
+            DTYPES, LAYOUTS, (False, True), (False, True)):
+        yield dict(routine=routine, dtype=dt, order=order, corder=co,
+                   trans=trans, lower=lower)
+
+
+def tr_cases(routine):
+    for dt, (order, co), (right, lower, trans, unit) in itertools.product(
+            DTYPES, LAYOUTS, itertools.product((False, True), repeat=4)):
+        yield dict(routine=routine, dtype=dt, order=order, corder=co,
+                   right=right, lower=lower, trans=trans, unit=unit)
+
+
+def all_cases():
+    """The reference harness's 192 cases, in a fixed order."""
+    out = list(gemm_cases()) + list(symm_cases()) + list(rk_cases("syrk")) + \
+        list(rk_cases("syr2k")) + list(tr_cases("trmm")) + list(tr_cases("trsm"))
+    return out
+
+
+def case_id(c):
+    keys = [k for k in c if k not in ("routine", "dtype", "order", "corder")]
+    flags = "-".join(f"{k}{int(c[k])}" for k in keys)
+    return f"{c['routine']}-{c['dtype'].name}-{c['order']}-{flags}"
+
+
+def build_problem(c, rng):
+    """Inputs for one case, drawn in the reference harness's order.  Returns
+    (args-builder, outputs): `args(out)` gives the CBLAS argument list with
+    `out` as the output array."""
+    dt, order, co = c["dtype"], c["order"], c["corder"]
+    r = c["routine"]
+    m, n, k = c.get("m", 5), c.get("n", 4), c.get("k", 3)
+    if r == "gemm":
+        a = rand(rng, (k, m) if c["ta"] else (m, k), dt, order)
+        b = rand(rng, (n, k) if c["tb"] else (k, n), dt, order)
+        c0 = rand(rng, (m, n), dt, order)
+        alpha, beta = 1.25, -0.5
+
+        def args(out):
+            return (co, TR if c["ta"] else NOTR, TR if c["tb"] else NOTR, m, n, k,
+                    alpha, a, ld_of(a, order), b, ld_of(b, order), beta, out,
+                    ld_of(out, order))
+        ref = alpha * (a.astype(np.float64).T if c["ta"] else a.astype(np.float64)) @ \
+            (b.astype(np.float64).T if c["tb"] else b.astype(np.float64)) + beta * c0.astype(np.float64)
+        return dict(inputs=dict(a=a, b=b), c0=c0, args=args, exact=ref, mask=None)
+    if r == "symm":
+        mm, nn = c.get("m", 5), c.get("n", 3)
+        size = nn if c["right"] else mm
+        a = symmetric_storage(rng, size, dt, order, c["lower"])
+        b = rand(rng, (mm, nn), dt, order)
+        c0 = rand(rng, (mm, nn), dt, order)
+        alpha, beta = 0.75, -0.25
+        s = sym_dense(a, c["lower"])
+        ref = alpha * (b.astype(np.float64) @ s if c["right"] else s @ b.astype(np.float64)) \
+            + beta * c0.astype(np.float64)
+
+        def args(out):
+            return (co, RIGHT if c["right"] else LEFT, LO if c["lower"] else UP, mm, nn,
+                    alpha, a, ld_of(a, order), b, ld_of(b, order), beta, out,
+                    ld_of(out, order))
+        return dict(inputs=dict(a=a, b=b), c0=c0, args=args, exact=ref, mask=None)
+    if r in ("syrk", "syr2k"):
+        nn, kk = c.get("n", 5), c.get("k", 3)
+        shape = (kk, nn) if c["trans"] else (nn, kk)
+        a = rand(rng, shape, dt, order)
+        b = rand(rng, shape, dt, order) if r == "syr2k" else None
+        c0 = rand(rng, (nn, nn), dt, order)
+        alpha, beta = (1.25, -0.5) if r == "syrk" else (0.75, -0.25)
+        A = a.astype(np.float64)
+        A = A.T if c["trans"] else A
+        if r == "syrk":
+            ref = alpha * A @ A.T + beta * c0.astype(np.float64)
+        else:
+            B = b.astype(np.float64)
+            B = B.T if c["trans"] else B
+            ref = alpha * (A @ B.T + B @ A.T) + beta * c0.astype(np.float64)
+        mask = np.tril(np.ones((nn, nn), bool)) if c["lower"] else np.triu(np.ones((nn, nn), bool))
+
+        def args(out):
+            lst = [co, LO if c["lower"] else UP, TR if c["trans"] else NOTR, nn, kk,
+                   alpha, a, ld_of(a, order)]
+            if r == "syr2k":
+                lst += [b, ld_of(b, order)]
+            lst += [beta, out, ld_of(out, order)]
+            return tuple(lst)
+        return dict(inputs=dict(a=a, b=b), c0=c0, args=args, exact=ref, mask=mask)
+    if r in ("trmm", "trsm"):
+        mm, nn = c.get("m", 5), c.get("n", 3)
+        size = nn if c["right"] else mm
+        if size <= 8:
+            a = triangular_storage(rng, size, dt, order, c["lower"], c["unit"])
+        else:
+            a = conditioned_triangular(rng, size, dt, order, c["lower"], c["unit"])
+        b0 = rand(rng, (mm, nn), dt, order)
+        alpha = 0.75
+        t = tri_dense(a, c["lower"], c["unit"])
+        op = t.T if c["trans"] else t
+        B = b0.astype(np.float64)
+        if r == "trmm":
+            ref = alpha * (B @ op if c["right"] else op @ B)
+        else:
+            ref = alpha * (np.linalg.solve(op.T, B.T).T if c["right"] else np.linalg.solve(op, B))
+
+        def args(out):
+            return (co, RIGHT if c["right"] else LEFT, LO if c["lower"] else UP,
+                    TR if c["trans"] else NOTR, UNIT if c["unit"] else NONUNIT, mm, nn,
+                    alpha, a, ld_of(a, order), out, ld_of(out, order))
+        return dict(inputs=dict(a=a), c0=b0, args=args, exact=ref, mask=None)
+    raise ValueError(r)
+
+
+def assert_ref_close(actual, expected, dtype, mask=None, initial=None):
+    """Reference tolerance (check_cblas.py:182-190) on the written part and
+    bit equality on the untouched triangle (check_cblas.py:355-363)."""
+    rtol, atol = TOL[np.dtype(dtype)]
+    if mask is None:
+        np.testing.assert_allclose(actual, expected, rtol=rtol, atol=atol)
+    else:
+        np.testing.assert_allclose(actual[mask], expected[mask], rtol=rtol, atol=atol)
+        np.testing.assert_array_equal(actual[~mask], initial[~mask])
+
+
+def ragged_cases():
+    """Larger, ragged problems: several micro-tiles of the reference's avx2
+    target (S 8x8, D 12x4; blocks D 72/252/512), edge tiles in every
+    dimension, and for TRSM/TRMM more than one kc block, so the full-tile
+    path and the trailing update (frame/l3/trsm.c:201-274) are exercised."""
+    out = []
+    for dt in DTYPES:
+        for (ta, tb) in itertools.product((False, True), repeat=2):
+            out.append(dict(routine="gemm", dtype=dt, order="F", corder=COL, ta=ta,
+                            tb=tb, m=67, n=45, k=33))
+        out.append(dict(routine="gemm", dtype=dt, order="C", corder=ROW, ta=False,
+                        tb=True, m=100, n=60, k=80))
+        for right, lower in itertools.product((False, True), repeat=2):
+            out.append(dict(routine="symm", dtype=dt, order="F", corder=COL,
+                            right=right, lower=lower, m=70, n=50))
+        for r in ("syrk", "syr2k"):
+            for trans, lower in itertools.product((False, True), repeat=2):
+                out.append(dict(routine=r, dtype=dt, order="F", corder=COL,
+                                trans=trans, lower=lower, n=70, k=45))
+        for r in ("trmm", "trsm"):
+            for right, lower, trans, unit in itertools.product((False, True), repeat=4):
+                mm, nn = (76, 17) if not right else (17, 40)
+                if dt == np.float32:
+                    mm, nn = (48, 19) if not right else (19, 48)
+                out.append(dict(routine=r, dtype=dt, order="F", corder=COL,
+                                right=right, lower=lower, trans=trans, unit=unit,
+                                m=mm, n=nn))
+    return out

@@ -1,0 +1,383 @@
+"""Pins the CPU oracle (oracle/oracle.c) before anything is checked against
+it.  CPU only.
+
+  * every golden vector written from the reference's own C build (both the
+    linux_avx2 and the reference config) -- tests/golden/*.npz;
+  * scipy's BLAS, the reference harness's own expected values
+    (test/correctness/check_cblas.py:179-190), on the 192 harness cases;
+  * exact closed forms: small-integer operands (every product and sum is an
+    exactly representable integer, so a dropped term, a wrong sign, index or
+    transposed operand changes the bits), identity/ones inputs of the
+    reference benchmark (test/performance/benchmark_cblas.py:124-141), and
+    TRSM on integer systems with power-of-two diagonals;
+  * the argument-check parameter numbers against the reference build's
+    cblas_xerbla messages (frame/compat/cblas/cblas.c:114-516);
+  * the special cases of the driver contract: alpha == 0, beta == 0 with NaN
+    in C, k == 0, m == 0 / n == 0.
+"""
+import ctypes
+import itertools
+import json
+import os
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+from scipy.linalg import blas as sblas
+
+import l3cases as L
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+@pytest.fixture(scope="module")
+def orc():
+    if not os.path.exists(L.ORACLE_LIB):
+        subprocess.check_call(["make", "-C", os.path.join(L.ROOT, "oracle"), "oracle"])
+    return L.Cblas(L.ORACLE_LIB, prefix="orc_", returns_status=True)
+
+
+def load_golden(name):
+    z = np.load(os.path.join(GOLDEN, f"{name}.npz"))
+    meta = json.loads(str(z["meta"]))
+    cases = []
+    for i, m in enumerate(meta):
+        c = dict(m)
+        c["dtype"] = np.dtype(c["dtype"])
+        arr = {k: z[f"{i}_{k}"] for k in ("a", "b", "c0", "avx2", "ref") if f"{i}_{k}" in z.files}
+        cases.append((c, arr))
+    return cases
+
+
+def args_from_golden(c, arr, out):
+    """Rebuild the CBLAS argument list for a golden case from its arrays."""
+    r, co, order = c["routine"], c["corder"], c["order"]
+    a, b = arr["a"], arr.get("b")
+    lda = L.ld_of(a, order)
+    if r == "gemm":
+        al, be = (1.25, -0.5)
+        m, n = out.shape
+        k = a.shape[0] if c["ta"] else a.shape[1]
+        return (co, L.TR if c["ta"] else L.NOTR, L.TR if c["tb"] else L.NOTR, m, n, k,
+                al, a, lda, b, L.ld_of(b, order), be, out, L.ld_of(out, order))
+    if r == "symm":
+        m, n = out.shape
+        return (co, L.RIGHT if c["right"] else L.LEFT, L.LO if c["lower"] else L.UP, m, n,
+                0.75, a, lda, b, L.ld_of(b, order), -0.25, out, L.ld_of(out, order))
+    if r in ("syrk", "syr2k"):
+        n = out.shape[0]
+        k = a.shape[0] if c["trans"] else a.shape[1]
+        al, be = (1.25, -0.5) if r == "syrk" else (0.75, -0.25)
+        lst = [co, L.LO if c["lower"] else L.UP, L.TR if c["trans"] else L.NOTR, n, k, al,
+               a, lda]
+        if r == "syr2k":
+            lst += [b, L.ld_of(b, order)]
+        return tuple(lst + [be, out, L.ld_of(out, order)])
+    m, n = out.shape
+    return (co, L.RIGHT if c["right"] else L.LEFT, L.LO if c["lower"] else L.UP,
+            L.TR if c["trans"] else L.NOTR, L.UNIT if c["unit"] else L.NONUNIT, m, n,
+            0.75, a, lda, out, L.ld_of(out, order))
+
+
+def golden_mask(c, n):
+    if c["routine"] in ("syrk", "syr2k"):
+        return np.tril(np.ones((n, n), bool)) if c["lower"] else np.triu(np.ones((n, n), bool))
+    return None
+
+
+def rel_err(x, y):
+    x = np.asarray(x, np.float64)
+    y = np.asarray(y, np.float64)
+    return np.linalg.norm(x - y) / max(np.linalg.norm(y), 1e-300)
+
+
+@pytest.mark.parametrize("name", ["l3_harness", "l3_ragged"])
+def test_oracle_matches_reference_build(orc, name):
+    """Oracle vs the reference's compiled outputs, both configs."""
+    for c, arr in load_golden(name):
+        out = np.array(arr["c0"], copy=True, order=c["order"])
+        st = orc.call(c["dtype"], c["routine"], *args_from_golden(c, arr, out))
+        assert st == 0, L.case_id(c)
+        mask = golden_mask(c, out.shape[0])
+        for key in ("avx2", "ref"):
+            try:
+                L.assert_ref_close(out, arr[key], c["dtype"], mask, arr["c0"])
+            except AssertionError as e:
+                raise AssertionError(f"{name} {L.case_id(c)} vs {key}: {e}")
+
+
+@pytest.mark.parametrize("case", L.all_cases(), ids=L.case_id)
+def test_oracle_matches_scipy_and_closed_form(orc, case):
+    """The reference harness's own check (scipy BLAS), plus float64 numpy."""
+    c = case
+    prob = L.build_problem(c, np.random.default_rng(L.SEED))
+    out = np.array(prob["c0"], copy=True, order=c["order"])
+    assert orc.call(c["dtype"], c["routine"], *prob["args"](out)) == 0
+    L.assert_ref_close(out, prob["exact"], c["dtype"], prob["mask"], prob["c0"])
+    # scipy BLAS, as check_cblas.py:179-190 computes the expected value
+    f = sblas.get_blas_funcs(c["routine"], dtype=c["dtype"])
+    F = lambda x: np.array(x, dtype=c["dtype"], order="F", copy=True)  # noqa: E731
+    a, b = prob["inputs"]["a"], prob["inputs"].get("b")
+    r = c["routine"]
+    if r == "gemm":
+        exp = f(1.25, F(a), F(b), beta=-0.5, c=F(prob["c0"]), trans_a=int(c["ta"]),
+                trans_b=int(c["tb"]))
+    elif r == "symm":
+        exp = f(0.75, F(a), F(b), beta=-0.25, c=F(prob["c0"]), side=int(c["right"]),
+                lower=int(c["lower"]))
+    elif r == "syrk":
+        exp = f(1.25, F(a), beta=-0.5, c=F(prob["c0"]), trans=int(c["trans"]),
+                lower=int(c["lower"]))
+    elif r == "syr2k":
+        exp = f(0.75, F(a), F(b), beta=-0.25, c=F(prob["c0"]), trans=int(c["trans"]),
+                lower=int(c["lower"]))
+    else:
+        exp = f(0.75, F(a), F(prob["c0"]), side=int(c["right"]), lower=int(c["lower"]),
+                trans_a=int(c["trans"]), diag=int(c["unit"]))
+    L.assert_ref_close(out, exp, c["dtype"], prob["mask"], prob["c0"])
+
+
+# ------------------------------------------------------ exact closed forms --
+def ints(rng, shape, dt, order="F", lo=-4, hi=5):
+    return np.array(rng.integers(lo, hi, size=shape), dtype=dt, order=order)
+
+
+@pytest.mark.parametrize("dt", L.DTYPES)
+@pytest.mark.parametrize("order,co", L.LAYOUTS)
+def test_exact_integer_products(orc, dt, order, co):
+    rng = np.random.default_rng(7)
+    m, n, k = 9, 7, 6
+    for ta, tb in itertools.product((0, 1), repeat=2):
+        a = ints(rng, (k, m) if ta else (m, k), dt, order)
+        b = ints(rng, (n, k) if tb else (k, n), dt, order)
+        c0 = ints(rng, (m, n), dt, order)
+        out = c0.copy(order=order)
+        assert orc.call(dt, "gemm", co, L.TR if ta else L.NOTR, L.TR if tb else L.NOTR, m, n,
+                        k, 2.0, a, L.ld_of(a, order), b, L.ld_of(b, order), -1.0, out,
+                        L.ld_of(out, order)) == 0
+        A = a.astype(np.int64).T if ta else a.astype(np.int64)
+        B = b.astype(np.int64).T if tb else b.astype(np.int64)
+        np.testing.assert_array_equal(out, (2 * A @ B - c0.astype(np.int64)).astype(dt))
+    # SYMM: only the stored triangle is read; the rest is NaN
+    for right, lower in itertools.product((0, 1), repeat=2):
+        s = n if right else m
+        full = ints(rng, (s, s), np.int64)
+        full = np.tril(full) + np.tril(full, -1).T
+        store = full.astype(dt)
+        store[np.triu_indices(s, 1) if lower else np.tril_indices(s, -1)] = np.nan
+        store = np.array(store, order=order)
+        b = ints(rng, (m, n), dt, order)
+        c0 = ints(rng, (m, n), dt, order)
+        out = c0.copy(order=order)
+        assert orc.call(dt, "symm", co, L.RIGHT if right else L.LEFT, L.LO if lower else L.UP,
+                        m, n, 3.0, store, L.ld_of(store, order), b, L.ld_of(b, order), 0.0,
+                        out, L.ld_of(out, order)) == 0
+        B = b.astype(np.int64)
+        np.testing.assert_array_equal(out, (3 * (B @ full if right else full @ B)).astype(dt))
+    # SYRK / SYR2K: untouched triangle keeps its bits
+    for trans, lower in itertools.product((0, 1), repeat=2):
+        a = ints(rng, (k, n) if trans else (n, k), dt, order)
+        b = ints(rng, (k, n) if trans else (n, k), dt, order)
+        c0 = ints(rng, (n, n), dt, order)
+        A = a.astype(np.int64).T if trans else a.astype(np.int64)
+        B = b.astype(np.int64).T if trans else b.astype(np.int64)
+        mask = np.tril(np.ones((n, n), bool)) if lower else np.triu(np.ones((n, n), bool))
+        out = c0.copy(order=order)
+        assert orc.call(dt, "syrk", co, L.LO if lower else L.UP, L.TR if trans else L.NOTR, n,
+                        k, 1.0, a, L.ld_of(a, order), 1.0, out, L.ld_of(out, order)) == 0
+        exp = A @ A.T + c0.astype(np.int64)
+        np.testing.assert_array_equal(out[mask], exp[mask].astype(dt))
+        np.testing.assert_array_equal(out[~mask], c0[~mask])
+        out = c0.copy(order=order)
+        assert orc.call(dt, "syr2k", co, L.LO if lower else L.UP, L.TR if trans else L.NOTR, n,
+                        k, 1.0, a, L.ld_of(a, order), b, L.ld_of(b, order), 2.0, out,
+                        L.ld_of(out, order)) == 0
+        exp = A @ B.T + B @ A.T + 2 * c0.astype(np.int64)
+        np.testing.assert_array_equal(out[mask], exp[mask].astype(dt))
+        np.testing.assert_array_equal(out[~mask], c0[~mask])
+
+
+@pytest.mark.parametrize("dt", L.DTYPES)
+@pytest.mark.parametrize("order,co", L.LAYOUTS)
+def test_exact_triangular(orc, dt, order, co):
+    """TRMM on integers and TRSM on integer systems whose diagonal is a power
+    of two, so substitution with reciprocal diagonals is exact."""
+    rng = np.random.default_rng(11)
+    m, n = 8, 5
+    for right, lower, trans, unit in itertools.product((0, 1), repeat=4):
+        s = n if right else m
+        t = ints(rng, (s, s), np.int64, lo=-2, hi=3)
+        t = np.tril(t, -1) if lower else np.triu(t, 1)
+        d = np.ones(s) if unit else 2.0 ** rng.integers(-1, 2, size=s)
+        dense = t + np.diag(d)
+        store = dense.astype(dt)
+        store[np.triu_indices(s, 1) if lower else np.tril_indices(s, -1)] = np.nan
+        if unit:
+            np.fill_diagonal(store, np.nan)
+        store = np.array(store, order=order)
+        op = dense.T if trans else dense
+        x = ints(rng, (m, n), np.float64, lo=-3, hi=4)
+        flags = (co, L.RIGHT if right else L.LEFT, L.LO if lower else L.UP,
+                 L.TR if trans else L.NOTR, L.UNIT if unit else L.NONUNIT, m, n)
+        # TRMM: B := 2 op(A) X (or 2 X op(A)), exact in both precisions
+        b = np.array(x, dtype=dt, order=order)
+        assert orc.call(dt, "trmm", *flags, 2.0, store, L.ld_of(store, order), b,
+                        L.ld_of(b, order)) == 0
+        np.testing.assert_array_equal(b, (2 * (x @ op if right else op @ x)).astype(dt))
+        # TRSM: B = op(A) X exactly; solving with alpha 1 recovers X bit for bit
+        rhs = x @ op if right else op @ x
+        if np.abs(rhs).max() > 2 ** 20:
+            continue
+        b = np.array(rhs, dtype=dt, order=order)
+        assert orc.call(dt, "trsm", *flags, 1.0, store, L.ld_of(store, order), b,
+                        L.ld_of(b, order)) == 0
+        np.testing.assert_array_equal(b, x.astype(dt))
+
+
+@pytest.mark.parametrize("dt", L.DTYPES)
+def test_benchmark_known_answers(orc, dt):
+    """benchmark_cblas.py:124-141: ones give k in every entry; identity
+    leaves B unchanged under TRMM and TRSM."""
+    m = n = k = 33
+    a = np.ones((m, k), dt, order="F")
+    b = np.ones((k, n), dt, order="F")
+    c = np.zeros((m, n), dt, order="F")
+    orc.call(dt, "gemm", L.COL, L.NOTR, L.NOTR, m, n, k, 1.0, a, m, b, k, 0.0, c, m)
+    assert np.all(c == k)
+    eye = np.eye(m, dtype=dt, order="F")
+    x = np.array(np.random.default_rng(3).standard_normal((m, n)), dt, order="F")
+    for r in ("trmm", "trsm"):
+        y = x.copy(order="F")
+        orc.call(dt, r, L.COL, L.LEFT, L.LO, L.NOTR, L.NONUNIT, m, n, 1.0, eye, m, y, m)
+        np.testing.assert_array_equal(y, x)
+
+
+@pytest.mark.parametrize("dt", L.DTYPES)
+def test_driver_special_cases(orc, dt):
+    """alpha == 0 reads neither A nor B; beta == 0 never reads C
+    (include/mublis_instantiate.h:51, frame/l3/l3_driver.c:104-124);
+    k == 0 gives beta*C; m == 0 or n == 0 touches nothing."""
+    m, n, k = 6, 5, 4
+    nan_a = np.full((m, k), np.nan, dt, order="F")
+    nan_b = np.full((k, n), np.nan, dt, order="F")
+    c0 = np.array(np.arange(m * n).reshape(m, n), dt, order="F")
+    c = c0.copy(order="F")
+    orc.call(dt, "gemm", L.COL, L.NOTR, L.NOTR, m, n, k, 0.0, nan_a, m, nan_b, k, 2.0, c, m)
+    np.testing.assert_array_equal(c, 2 * c0)
+    a = np.ones((m, k), dt, order="F")
+    b = np.ones((k, n), dt, order="F")
+    c = np.full((m, n), np.nan, dt, order="F")
+    orc.call(dt, "gemm", L.COL, L.NOTR, L.NOTR, m, n, k, 1.0, a, m, b, k, 0.0, c, m)
+    assert np.all(c == k)
+    c = c0.copy(order="F")
+    orc.call(dt, "gemm", L.COL, L.NOTR, L.NOTR, m, n, 0, 1.0, a, m, b, 1, 0.5, c, m)
+    np.testing.assert_array_equal(c, 0.5 * c0)
+    c = c0.copy(order="F")
+    orc.call(dt, "gemm", L.COL, L.NOTR, L.NOTR, 0, n, k, 1.0, a, 1, b, k, 0.5, c, 1)
+    orc.call(dt, "gemm", L.COL, L.NOTR, L.NOTR, m, 0, k, 1.0, a, m, b, k, 0.5, c, m)
+    np.testing.assert_array_equal(c, c0)
+    for r in ("trmm", "trsm"):
+        y = c0.copy(order="F")
+        tri = np.full((m, m), np.nan, dt, order="F")
+        orc.call(dt, r, L.COL, L.LEFT, L.LO, L.NOTR, L.NONUNIT, m, n, 0.0, tri, m, y, m)
+        assert np.all(y == 0)
+
+
+# -------------------------------------------------- argument-check numbers --
+BAD_CALLS = [
+    ("gemm", (0, 111, 111, 2, 2, 2, 1.0, "A", 2, "B", 2, 0.0, "C", 2)),
+    ("gemm", (102, 0, 111, 2, 2, 2, 1.0, "A", 2, "B", 2, 0.0, "C", 2)),
+    ("gemm", (102, 111, 0, 2, 2, 2, 1.0, "A", 2, "B", 2, 0.0, "C", 2)),
+    ("gemm", (102, 111, 111, -1, 2, 2, 1.0, "A", 2, "B", 2, 0.0, "C", 2)),
+    ("gemm", (102, 111, 111, 2, -1, 2, 1.0, "A", 2, "B", 2, 0.0, "C", 2)),
+    ("gemm", (102, 111, 111, 2, 2, -1, 1.0, "A", 2, "B", 2, 0.0, "C", 2)),
+    ("gemm", (102, 111, 111, 3, 2, 2, 1.0, "A", 2, "B", 2, 0.0, "C", 3)),
+    ("gemm", (101, 111, 112, 2, 3, 2, 1.0, "A", 2, "B", 1, 0.0, "C", 3)),
+    ("gemm", (102, 111, 111, 3, 2, 2, 1.0, "A", 3, "B", 2, 0.0, "C", 2)),
+    ("symm", (102, 0, 122, 2, 2, 1.0, "A", 2, "B", 2, 0.0, "C", 2)),
+    ("symm", (102, 141, 0, 2, 2, 1.0, "A", 2, "B", 2, 0.0, "C", 2)),
+    ("symm", (102, 142, 122, 2, 3, 1.0, "A", 2, "B", 2, 0.0, "C", 2)),
+    ("symm", (101, 141, 122, 2, 3, 1.0, "A", 2, "B", 2, 0.0, "C", 3)),
+    ("symm", (101, 141, 122, 2, 3, 1.0, "A", 2, "B", 3, 0.0, "C", 2)),
+    ("syrk", (102, 0, 111, 2, 2, 1.0, "A", 2, 0.0, "C", 2)),
+    ("syrk", (102, 122, 0, 2, 2, 1.0, "A", 2, 0.0, "C", 2)),
+    ("syrk", (102, 122, 111, -2, 2, 1.0, "A", 2, 0.0, "C", 2)),
+    ("syrk", (102, 122, 111, 2, -2, 1.0, "A", 2, 0.0, "C", 2)),
+    ("syrk", (101, 122, 111, 2, 3, 1.0, "A", 2, 0.0, "C", 2)),
+    ("syrk", (102, 122, 111, 3, 2, 1.0, "A", 3, 0.0, "C", 2)),
+    ("syr2k", (102, 122, 112, 3, 2, 1.0, "A", 2, "B", 1, 0.0, "C", 3)),
+    ("syr2k", (102, 122, 111, 3, 2, 1.0, "A", 3, "B", 3, 0.0, "C", 2)),
+    ("trsm", (102, 0, 122, 111, 131, 2, 2, 1.0, "A", 2, "B", 2)),
+    ("trsm", (102, 141, 0, 111, 131, 2, 2, 1.0, "A", 2, "B", 2)),
+    ("trsm", (102, 141, 122, 0, 131, 2, 2, 1.0, "A", 2, "B", 2)),
+    ("trsm", (102, 141, 122, 111, 0, 2, 2, 1.0, "A", 2, "B", 2)),
+    ("trsm", (102, 141, 122, 111, 131, -1, 2, 1.0, "A", 2, "B", 2)),
+    ("trsm", (102, 141, 122, 111, 131, 2, -1, 1.0, "A", 2, "B", 2)),
+    ("trmm", (102, 142, 122, 111, 131, 2, 3, 1.0, "A", 2, "B", 2)),
+    ("trmm", (101, 141, 122, 111, 131, 2, 3, 1.0, "A", 2, "B", 2)),
+]
+
+_XERBLA_PROBE = r"""
+import ctypes, sys, numpy as np
+sys.path.insert(0, %r)
+import l3cases as L
+lib = L.Cblas(%r)
+routine = %r
+args = %r
+buf = {k: np.zeros(64, np.float64) for k in "ABC"}
+conv = [buf[a] if isinstance(a, str) else a for a in args]
+lib.call(np.float64, routine, *conv)
+print("RETURNED")
+"""
+
+
+@pytest.mark.parametrize("routine,args", BAD_CALLS, ids=[f"{r}{i}" for i, (r, _) in
+                                                         enumerate(BAD_CALLS)])
+def test_argument_numbers_match_reference_xerbla(orc, routine, args):
+    """The oracle reports the parameter number that the reference build's
+    cblas_xerbla prints ("Parameter %d to routine %s was incorrect") before
+    it exits (frame/compat/cblas/cblas.c:14-32)."""
+    if not os.path.exists(L.REF_AVX2_LIB):
+        pytest.skip("reference build absent")
+    code = _XERBLA_PROBE % (os.path.dirname(os.path.abspath(__file__)), L.REF_AVX2_LIB,
+                            routine, args)
+    p = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True,
+                       timeout=60)
+    assert p.returncode != 0 and "RETURNED" not in p.stdout
+    line = [ln for ln in p.stderr.splitlines() if ln.startswith("Parameter")][0]
+    ref_num = int(line.split()[1])
+    buf = {k: np.zeros(64, np.float64) for k in "ABC"}
+    conv = [buf[a] if isinstance(a, str) else a for a in args]
+    assert orc.call(np.float64, routine, *conv) == ref_num
+
+
+def test_large_problem_probes(orc):
+    """The sampled-entry and matrix-vector helpers used for n = 8192 checks
+    agree with numpy on a moderate problem."""
+    lib = orc.lib
+    rng = np.random.default_rng(5)
+    m = n = k = 96
+    A = np.asfortranarray(rng.standard_normal((k, m)))
+    B = np.asfortranarray(rng.standard_normal((k, n)))
+    C0 = np.asfortranarray(rng.standard_normal((m, n)))
+    I = np.array(rng.integers(0, m, 50), np.int32)
+    J = np.array(rng.integers(0, n, 50), np.int32)
+    out = np.zeros(50)
+    f = lib.orc_dgemm_entries
+    f.argtypes = [ctypes.c_int] * 3 + [ctypes.c_double, ctypes.c_void_p, ctypes.c_int,
+                                        ctypes.c_void_p, ctypes.c_int, ctypes.c_double,
+                                        ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
+                                        ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+    f(L.TR, L.NOTR, k, 1.5, A.ctypes.data, k, B.ctypes.data, k, 0.5, C0.ctypes.data, m, 50,
+      I.ctypes.data, J.ctypes.data, out.ctypes.data)
+    full = 1.5 * A.T @ B + 0.5 * C0
+    np.testing.assert_allclose(out, full[I, J], rtol=1e-13, atol=1e-13)
+    T = np.asfortranarray(np.tril(rng.standard_normal((m, m))))
+    x = rng.standard_normal(m)
+    y = np.zeros(m)
+    g = lib.orc_dtrmv_ld
+    g.argtypes = [ctypes.c_int] * 4 + [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
+                                        ctypes.c_void_p]
+    g(L.LO, L.TR, L.NONUNIT, m, T.ctypes.data, m, x.ctypes.data, y.ctypes.data)
+    np.testing.assert_allclose(y, T.T @ x, rtol=1e-12, atol=1e-12)
