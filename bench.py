@@ -1,0 +1,206 @@
+"""Benchmark of the gfx950 L3 engine on one MI355X.
+
+Workload (one step): DGEMM M=N=K=8192 with op(A)=A^T, alpha=1, beta=0,
+entries N(0,1), operands resident in HBM, called through the unchanged
+cblas_dgemm C ABI with device pointers.  The line also carries DTRSM
+(Left/Lower/NoTrans/NonUnit, m=8192, nrhs=4096) and DSYRK (lower, n=8192,
+k=2048) from the same workload family, and native FP32 SGEMM at 8192.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+
+Prints ONE JSON line.  `roofline` is for the dominant kernel (the FP64 MFMA
+GEMM kernel): achieved = 2*M*N*K per launch / average launch time measured
+with HIP events on the library's own stream; peak = MI355X dense FP64 MFMA
+(78.6 TFLOP/s).  `cpu_baseline` times the reference's own linux_avx2 build
+(oracle/_ref, compiled from the reference sources) on one host core over a
+bounded slice of the same DGEMM.
+"""
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK = {"f64": 78.6, "f32": 157.3}  # dense MFMA TFLOP/s, MI355X
+METRIC = ("TFLOP/s & %MFMA peak, D/SGEMM+DTRSM n=8192; p50/p99 mixed-L3 request "
+          "latency")
+COL, ROW, NOTR, TR, UP, LO, NONUNIT, LEFT = 102, 101, 111, 112, 121, 122, 131, 141
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=10)
+    p.add_argument("--warmup", type=int, default=3)
+    p.add_argument("--n", type=int, default=8192)
+    p.add_argument("--no-cpu", action="store_true")
+    p.add_argument("--no-extra", action="store_true")
+    p.add_argument("--cpu-slice", type=int, default=2048,
+                   help="M = N of the CPU-baseline DGEMM slice (K stays 8192)")
+    return p.parse_args()
+
+
+def timed_calls(torch, lib, fn, steps, warmup):
+    """Run fn() warmup+steps times; return (wall ms per call, event ms per
+    call on the library stream)."""
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    stream = torch.cuda.ExternalStream(lib.mublis_rt_stream())
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    e0.record(stream)
+    for _ in range(steps):
+        fn()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    wall = (time.perf_counter() - t0) * 1e3 / steps
+    return wall, e0.elapsed_time(e1) / steps
+
+
+def cpu_baseline(n_k, m_slice):
+    """Reference linux_avx2 build, one thread, DGEMM op(A)=A^T on an
+    m_slice x m_slice x n_k slice of the workload."""
+    ref = os.path.join(ROOT, "oracle", "_ref", "libref_avx2.so")
+    kind = "reference"
+    if not os.path.exists(ref):
+        return None
+    lib = ctypes.CDLL(ref)
+    f = lib.cblas_dgemm
+    f.argtypes = [ctypes.c_int] * 6 + [ctypes.c_double, ctypes.c_void_p, ctypes.c_int,
+                                       ctypes.c_void_p, ctypes.c_int, ctypes.c_double,
+                                       ctypes.c_void_p, ctypes.c_int]
+    rng = np.random.default_rng(314159)
+    m = m_slice
+    a = np.asfortranarray(rng.standard_normal((n_k, m)))
+    b = np.asfortranarray(rng.standard_normal((n_k, m)))
+    c = np.zeros((m, m), order="F")
+    t0 = time.perf_counter()
+    f(COL, TR, NOTR, m, m, n_k, 1.0, a.ctypes.data, n_k, b.ctypes.data, n_k, 0.0,
+      c.ctypes.data, m)
+    dt = time.perf_counter() - t0
+    flops = 2.0 * m * m * n_k
+    return {"value": round(flops / dt / 1e12, 5), "unit": "TFLOP/s", "cores": 1,
+            "kind": kind,
+            "sample": f"DGEMM op(A)=A^T M=N={m} K={n_k} N(0,1), reference linux_avx2 "
+                      f"build (-O3 -mavx2 -mfma -ffast-math), one thread, "
+                      f"{dt:.2f} s"}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only (asked for {args.gpus}); "
+              "multi-GPU sharding is described in DESIGN.md but not run", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    import mublis_amd as M
+
+    lib = M.load()
+    M.check(lib.mublis_rt_init(), "mublis_rt_init")
+    dev = torch.device("cuda:0")
+    n = args.n
+    g = torch.Generator(device=dev).manual_seed(314159)
+    # Column-major K x M A is the row-major (M, K) tensor; op(A) = A^T.
+    A = torch.randn((n, n), dtype=torch.float64, device=dev, generator=g)
+    B = torch.randn((n, n), dtype=torch.float64, device=dev, generator=g)
+    C = torch.zeros((n, n), dtype=torch.float64, device=dev)
+    pa, pb, pc = M.ptr(A), M.ptr(B), M.ptr(C)
+
+    def dgemm():
+        lib.cblas_dgemm(COL, TR, NOTR, n, n, n, 1.0, pa, n, pb, n, 0.0, pc, n)
+
+    wall, ev = timed_calls(torch, lib, dgemm, args.steps, args.warmup)
+    flops = 2.0 * n * n * n
+    tflops = flops / (wall * 1e-3) / 1e12
+    achieved = flops / (ev * 1e-3) / 1e12
+    assert torch.isfinite(C[:64, :64]).all().item()
+
+    extra = {}
+    if not args.no_extra:
+        # DTRSM L/L/N/NonUnit m=8192 nrhs=4096 (off-diag N(0,1)/sqrt(m), diag 1+|N|)
+        m, nrhs = n, n // 2
+        At = torch.randn((m, m), dtype=torch.float64, device=dev, generator=g) / m ** 0.5
+        d = 1.0 + torch.randn(m, dtype=torch.float64, device=dev, generator=g).abs()
+        At = torch.triu(At, 1) + torch.diag(d)
+        B0 = torch.randn((nrhs, m), dtype=torch.float64, device=dev, generator=g)
+        Bt = B0.clone()
+
+        def dtrsm():
+            Bt.copy_(B0)
+            lib.cblas_dtrsm(COL, LEFT, LO, NOTR, NONUNIT, m, nrhs, 1.0, M.ptr(At), m,
+                            M.ptr(Bt), m)
+        # time the copy separately and subtract it
+        def copy_only():
+            Bt.copy_(B0)
+        steps = max(3, args.steps // 2)
+        w_t, _ = timed_calls(torch, lib, dtrsm, steps, 1)
+        w_c, _ = timed_calls(torch, lib, copy_only, steps, 1)
+        t_trsm = max(w_t - w_c, 1e-6)
+        extra["dtrsm_8192x4096_tflops"] = round(m * m * nrhs / (t_trsm * 1e-3) / 1e12, 3)
+        extra["dtrsm_pct_fp64_peak"] = round(100 * extra["dtrsm_8192x4096_tflops"] /
+                                             PEAK["f64"], 2)
+        del At, B0, Bt
+        # DSYRK lower n=8192 k=2048
+        k = n // 4
+        As = torch.randn((k, n), dtype=torch.float64, device=dev, generator=g)
+
+        def dsyrk():
+            lib.cblas_dsyrk(COL, LO, NOTR, n, k, 1.0, M.ptr(As), n, 0.0, pc, n)
+        w_s, _ = timed_calls(torch, lib, dsyrk, steps, 1)
+        extra["dsyrk_8192_2048_tflops"] = round(n * (n + 1) * k / (w_s * 1e-3) / 1e12, 3)
+        extra["dsyrk_pct_fp64_peak"] = round(100 * extra["dsyrk_8192_2048_tflops"] /
+                                             PEAK["f64"], 2)
+        del As
+        # native FP32 SGEMM 8192 (op(A)=A^T, same shape)
+        Af, Bf = A.float(), B.float()
+        Cf = torch.zeros((n, n), dtype=torch.float32, device=dev)
+
+        def sgemm():
+            lib.cblas_sgemm(COL, TR, NOTR, n, n, n, 1.0, M.ptr(Af), n, M.ptr(Bf), n, 0.0,
+                            M.ptr(Cf), n)
+        w_f, _ = timed_calls(torch, lib, sgemm, steps, 1)
+        extra["sgemm_8192_tflops"] = round(flops / (w_f * 1e-3) / 1e12, 3)
+        extra["sgemm_pct_fp32_peak"] = round(100 * extra["sgemm_8192_tflops"] /
+                                             PEAK["f32"], 2)
+        del Af, Bf, Cf
+
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", "dgemm_8192_transA_traffic.json")
+    if os.path.exists(tfile):
+        traffic = json.load(open(tfile)).get("hbm_bytes_per_launch")
+
+    out = {
+        "metric": METRIC,
+        "value": round(tflops, 3),
+        "unit": "TFLOP/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic N(0,1) generated on device (seed 314159)",
+        "config": {"workload": f"DGEMM M=N=K={n} op(A)=A^T alpha=1 beta=0, cblas_dgemm "
+                               "with device-resident operands",
+                   "pct_fp64_mfma_peak": round(100 * tflops / PEAK["f64"], 2),
+                   **extra},
+        "roofline": {"bound": "mfma", "achieved": round(achieved, 3), "peak": PEAK["f64"],
+                     "unit": "TFLOP/s", "frac": round(achieved / PEAK["f64"], 4),
+                     "traffic": traffic},
+        "cpu_baseline": None if args.no_cpu else cpu_baseline(n, args.cpu_slice),
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,138 @@
+/*
+ * mublis_gfx950.h -- thin C ABI between the host C frame and the gfx950
+ * (MI355X, CDNA4) HIP layer.
+ *
+ * The frame (cblas.h / mublis.h entry points) stages operands and calls the
+ * device-level routines below.  Every pointer passed to a *_dev routine is a
+ * device pointer; sizes are element counts, strides are element strides
+ * (64-bit).  Routines enqueue work on `stream` (NULL: the calling thread's
+ * library stream) and return without synchronising; the return value is
+ * MUBLIS_OK or one of the MUBLIS_E* codes.
+ *
+ * Each routine names the reference interface whose work it replaces.
+ */
+#ifndef MUBLIS_GFX950_H
+#define MUBLIS_GFX950_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  MUBLIS_OK = 0,
+  MUBLIS_ECONTEXT = 1, /* no usable gfx950 device, or context invalid     */
+  MUBLIS_ENOMEM = 2,   /* device or pinned allocation failed              */
+  MUBLIS_EINVAL = 3,   /* argument the device path does not support       */
+  MUBLIS_EHIP = 4      /* a HIP runtime call failed                       */
+};
+
+/* ------------------------------------------------------------- runtime -- */
+/* Lazily selects the device and checks that it is gfx950.  Replaces the
+ * first-call work of mublis_get_safe_context + mublis_pool_init
+ * (frame/base/safe_dispatch.c:45-71, frame/base/pool.c:182-244). */
+int mublis_rt_init(void);
+/* Architecture name of the current device ("gfx950"), or "" before init. */
+const char *mublis_rt_arch(void);
+/* 1 if p is device (or managed) memory usable by kernels in place. */
+int mublis_rt_is_device_ptr(const void *p);
+/* The calling thread's stream (created on first use). */
+void *mublis_rt_stream(void);
+/* Device workspace from the thread-safe caching arena (replaces the
+ * reference's mutex-guarded pack-buffer pool, frame/base/pool.c:266-318). */
+void *mublis_rt_alloc(size_t bytes);
+void mublis_rt_free(void *p);
+int mublis_rt_h2d(void *dst, const void *src, size_t bytes, void *stream);
+int mublis_rt_d2h(void *dst, const void *src, size_t bytes, void *stream);
+int mublis_rt_sync(void *stream);
+/* Last HIP error text seen by this thread (for diagnostics). */
+const char *mublis_rt_last_error(void);
+
+/* ------------------------------------------------------- generic L3 ------ */
+/*
+ * C := alpha * Aview * Bview + beta * C on the elements (i, j) allowed by ji
+ * (0 all, 1 lower: j <= i, 2 upper: j >= i).  A is m x k, B is k x n.
+ * When a2/b2 are non-NULL a second product alpha * A2view * B2view is added
+ * in the same launch (SYR2K).  Any (rs, cs) pair is accepted; operands with
+ * no unit stride are first packed to a dense buffer.
+ * Replaces the generic driver's loop nest + packm + gemm micro-kernel
+ * (frame/l3/l3_driver.c:126-239, frame/l1m/packm.c:18-84,
+ * include/mublis_instantiate.h:53-66).
+ */
+int mublis_gfx950_dgemm_dev(int m, int n, int k, double alpha,
+                            const double *a, int64_t rs_a, int64_t cs_a,
+                            const double *b, int64_t rs_b, int64_t cs_b,
+                            const double *a2, int64_t rs_a2, int64_t cs_a2,
+                            const double *b2, int64_t rs_b2, int64_t cs_b2,
+                            double beta, double *c, int64_t rs_c, int64_t cs_c,
+                            int ji, void *stream);
+int mublis_gfx950_sgemm_dev(int m, int n, int k, float alpha,
+                            const float *a, int64_t rs_a, int64_t cs_a,
+                            const float *b, int64_t rs_b, int64_t cs_b,
+                            const float *a2, int64_t rs_a2, int64_t cs_a2,
+                            const float *b2, int64_t rs_b2, int64_t cs_b2,
+                            float beta, float *c, int64_t rs_c, int64_t cs_c,
+                            int ji, void *stream);
+
+/*
+ * Materialise the view of an m x n operand into dense column-major `dst`
+ * (leading dimension ld): struc 0 GENERAL, 1 SYMMETRIC (read the reflected
+ * element outside `uplo`), 2 TRIANGULAR (zero outside `uplo`; diag 1 UNIT
+ * writes 1 without reading, 2 INVERT writes 1/a).  diagoff = row - col offset
+ * of the view's (0,0) from the diagonal.  Unreferenced storage is never
+ * read, so NaN there cannot leak.  Replaces packm at whole-operand
+ * granularity (frame/l1m/packm.c:18-84).
+ */
+int mublis_gfx950_dpack_dev(int m, int n, const double *src, int64_t rs,
+                            int64_t cs, int struc, int uplo, int diag,
+                            int diagoff, double *dst, int64_t ld, void *stream);
+int mublis_gfx950_spack_dev(int m, int n, const float *src, int64_t rs,
+                            int64_t cs, int struc, int uplo, int diag,
+                            int diagoff, float *dst, int64_t ld, void *stream);
+
+/*
+ * In-place triangular solve, Left side only: B := alpha * A^{-1} * B, A m x m
+ * lower (lower = 1) or upper triangular as viewed through (rs_a, cs_a); unit
+ * = 1 means the diagonal is 1 and is never read.  Right-side and transposed
+ * problems are mapped onto this form by the caller exactly as the reference
+ * does (frame/l3/trsm.c:35-54).  Diagonal blocks are solved in FP64 (also for
+ * float data) with reciprocal diagonals; the rest is MFMA GEMM.  Replaces the
+ * TRSM loop nest and the GEMMTRSM micro-kernels (frame/l3/trsm.c:87-431,
+ * include/mublis_instantiate.h:95-111).
+ */
+int mublis_gfx950_dtrsm_dev(int lower, int unit, int m, int n, double alpha,
+                            const double *a, int64_t rs_a, int64_t cs_a,
+                            double *b, int64_t rs_b, int64_t cs_b, void *stream);
+int mublis_gfx950_strsm_dev(int lower, int unit, int m, int n, float alpha,
+                            const float *a, int64_t rs_a, int64_t cs_a,
+                            float *b, int64_t rs_b, int64_t cs_b, void *stream);
+
+/* Diagonal-block solve (m <= 64) whose stored diagonal already holds the
+ * reciprocals, as the GEMMTRSM micro-kernel contract packs it
+ * (include/mublis_instantiate.h:84-86, frame/l1m/packm.c:143-144). */
+int mublis_gfx950_dtrsm_inverted_dev(int lower, int m, int n, const double *a,
+                                     int64_t rs_a, int64_t cs_a, double *b,
+                                     int64_t rs_b, int64_t cs_b, void *stream);
+int mublis_gfx950_strsm_inverted_dev(int lower, int m, int n, const float *a,
+                                     int64_t rs_a, int64_t cs_a, float *b,
+                                     int64_t rs_b, int64_t cs_b, void *stream);
+
+/* Strided copy of an m x n matrix (dst may have other strides). */
+int mublis_gfx950_dcopy_dev(int m, int n, const double *src, int64_t rs_s,
+                            int64_t cs_s, double *dst, int64_t rs_d,
+                            int64_t cs_d, void *stream);
+int mublis_gfx950_scopy_dev(int m, int n, const float *src, int64_t rs_s,
+                            int64_t cs_s, float *dst, int64_t rs_d,
+                            int64_t cs_d, void *stream);
+/* X := value everywhere (alpha == 0 paths of TRMM/TRSM, frame/l3/trsm.c:27-33). */
+int mublis_gfx950_dfill_dev(int m, int n, double value, double *x, int64_t rs,
+                            int64_t cs, void *stream);
+int mublis_gfx950_sfill_dev(int m, int n, float value, float *x, int64_t rs,
+                            int64_t cs, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MUBLIS_GFX950_H */

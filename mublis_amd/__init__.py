@@ -1,0 +1,117 @@
+"""MuBLIS L3 engine for MI355X (gfx950): Python host mirror.
+
+The product is the C library ``mublis_amd/lib/libmublis.so`` (CBLAS and
+mublis_* entry points, see include/cblas.h and include/mublis.h).  This
+module builds it, loads it with ctypes and declares every entry point's
+argument types, so callers can pass numpy arrays (host memory, staged by the
+library) or torch tensors (device memory, used in place).
+
+Loading fails loudly when the library has not been built: there is no CPU
+fallback.
+"""
+import ctypes
+import os
+import subprocess
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+LIB_PATH = os.path.join(HERE, "lib", "libmublis.so")
+
+_VP, _I, _I64 = ctypes.c_void_p, ctypes.c_int, ctypes.c_int64
+_F, _D = ctypes.c_float, ctypes.c_double
+
+# include/mublis_gfx950.h status codes
+OK, ECONTEXT, ENOMEM, EINVAL, EHIP = 0, 1, 2, 3, 4
+
+_lib = None
+
+
+def build(jobs=8):
+    """Compile libmublis.so for gfx950 (always recompiles what changed)."""
+    subprocess.check_call(["make", "-C", HERE, f"-j{jobs}"])
+
+
+def _declare(lib):
+    for p, ct in (("s", _F), ("d", _D)):
+        cb = {
+            "gemm": [_I, _I, _I, _I, _I, _I, ct, _VP, _I, _VP, _I, ct, _VP, _I],
+            "symm": [_I, _I, _I, _I, _I, ct, _VP, _I, _VP, _I, ct, _VP, _I],
+            "syrk": [_I, _I, _I, _I, _I, ct, _VP, _I, ct, _VP, _I],
+            "syr2k": [_I, _I, _I, _I, _I, ct, _VP, _I, _VP, _I, ct, _VP, _I],
+            "trmm": [_I, _I, _I, _I, _I, _I, _I, ct, _VP, _I, _VP, _I],
+            "trsm": [_I, _I, _I, _I, _I, _I, _I, ct, _VP, _I, _VP, _I],
+        }
+        for r, args in cb.items():
+            f = getattr(lib, f"cblas_{p}{r}")
+            f.argtypes, f.restype = args, None
+        mb = {
+            "gemm": [_I, _I, _I, _I, _I, ct, _VP, _I, _I, _VP, _I, _I, ct, _VP, _I, _I],
+            "symm": [_I, _I, _I, _I, ct, _VP, _I, _I, _VP, _I, _I, ct, _VP, _I, _I],
+            "syrk": [_I, _I, _I, _I, ct, _VP, _I, _I, ct, _VP, _I, _I],
+            "syr2k": [_I, _I, _I, _I, ct, _VP, _I, _I, _VP, _I, _I, ct, _VP, _I, _I],
+            "trmm": [_I, _I, _I, _I, _I, _I, ct, _VP, _I, _I, _VP, _I, _I],
+            "trsm": [_I, _I, _I, _I, _I, _I, ct, _VP, _I, _I, _VP, _I, _I],
+        }
+        for r, args in mb.items():
+            f = getattr(lib, f"mublis_{p}{r}")
+            f.argtypes, f.restype = args, _I
+        f = getattr(lib, f"mublis_l3_{p}driver")
+        f.argtypes = [_I, _I, _I, ct, _VP, _I, _I, _VP, _I, _I, ct, _VP, _I, _I, _VP]
+        f.restype = _I
+        f = getattr(lib, f"mublis_gfx950_{p}gemm_dev")
+        f.argtypes = [_I, _I, _I, ct] + [_VP, _I64, _I64] * 4 + [ct, _VP, _I64, _I64,
+                                                                  _I, _VP]
+        f.restype = _I
+        f = getattr(lib, f"mublis_gfx950_{p}trsm_dev")
+        f.argtypes = [_I, _I, _I, _I, ct, _VP, _I64, _I64, _VP, _I64, _I64, _VP]
+        f.restype = _I
+        f = getattr(lib, f"mublis_gfx950_{p}pack_dev")
+        f.argtypes = [_I, _I, _VP, _I64, _I64, _I, _I, _I, _I, _VP, _I64, _VP]
+        f.restype = _I
+        f = getattr(lib, f"mublis_gfx950_{p}trsm_inverted_dev")
+        f.argtypes = [_I, _I, _I, _VP, _I64, _I64, _VP, _I64, _I64, _VP]
+        f.restype = _I
+        f = getattr(lib, f"mublis_gfx950_{p}copy_dev")
+        f.argtypes = [_I, _I, _VP, _I64, _I64, _VP, _I64, _I64, _VP]
+        f.restype = _I
+        f = getattr(lib, f"mublis_gfx950_{p}fill_dev")
+        f.argtypes = [_I, _I, ct, _VP, _I64, _I64, _VP]
+        f.restype = _I
+    lib.mublis_rt_init.restype = _I
+    lib.mublis_rt_arch.restype = ctypes.c_char_p
+    lib.mublis_rt_last_error.restype = ctypes.c_char_p
+    lib.mublis_rt_stream.restype = _VP
+    lib.mublis_rt_sync.argtypes = [_VP]
+    lib.mublis_rt_sync.restype = _I
+    lib.mublis_rt_is_device_ptr.argtypes = [_VP]
+    lib.mublis_rt_is_device_ptr.restype = _I
+    lib.mublis_get_context.argtypes = [_VP]
+    lib.mublis_get_context.restype = _I
+    return lib
+
+
+def load():
+    """Load libmublis.so (built in-tree).  Raises if it is missing."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(
+                f"{LIB_PATH} is not built; run mublis_amd.build() (no CPU fallback)")
+        _lib = _declare(ctypes.CDLL(LIB_PATH, mode=ctypes.RTLD_LOCAL))
+    return _lib
+
+
+def ptr(x):
+    """Address of a numpy array or torch tensor (or None)."""
+    if x is None:
+        return None
+    if hasattr(x, "data_ptr"):
+        return ctypes.c_void_p(x.data_ptr())
+    return ctypes.c_void_p(x.ctypes.data)
+
+
+def check(status, what="mublis"):
+    if status != OK:
+        msg = load().mublis_rt_last_error().decode(errors="replace")
+        raise RuntimeError(f"{what} failed with status {status}: {msg}")
+    return status

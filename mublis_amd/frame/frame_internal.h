@@ -1,0 +1,30 @@
+/* Internal declarations shared by the host frame's translation units. */
+#ifndef MUBLIS_FRAME_INTERNAL_H
+#define MUBLIS_FRAME_INTERNAL_H
+
+#include "mublis_instantiate.h"
+
+/* 0 once the config's context has been fetched and validated (cached for
+ * the process, failure included); non-zero otherwise. */
+int mublis_frame_context_ok(void);
+
+/* The validated context (NULL before a successful mublis_frame_context_ok). */
+const mublis_context_t *mublis_frame_context(void);
+
+#include <stddef.h>
+#include <stdint.h>
+
+typedef struct {
+  const void *host; /* caller's pointer */
+  void *dev;        /* device pointer used by kernels */
+  size_t bytes;     /* footprint in bytes */
+  int owned;        /* dev is a staging buffer owned by the call */
+} stage_t;
+
+/* Stage an m x n operand with strides (rs, cs); copy == 0 skips the H2D. */
+int mublis_stage_in(const void *p, int m, int n, int64_t rs, int64_t cs,
+                    size_t esz, int copy, stage_t *st, void *stream);
+int mublis_stage_out(stage_t *st, void *stream);
+void mublis_stage_release(stage_t *st);
+
+#endif

@@ -1,0 +1,32 @@
+// Shared declarations of the gfx950 HIP layer (device-side helpers used by
+// more than one translation unit).  Explicit instantiations for float and
+// double live in pack.hip.
+#ifndef MUBLIS_GFX950_COMMON_H
+#define MUBLIS_GFX950_COMMON_H
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "mublis_gfx950.h"
+
+// dst(i, j) := src(i, j) for an m x n matrix; any strides.
+template <typename T>
+int gfx950_copy(int m, int n, const T *src, int64_t rs_s, int64_t cs_s, T *dst,
+                int64_t rs_d, int64_t cs_d, hipStream_t s);
+
+// Materialise an operand view (GENERAL / SYMMETRIC / TRIANGULAR) densely.
+template <typename T>
+int gfx950_pack(int m, int n, const T *src, int64_t rs, int64_t cs, int struc,
+                int uplo, int diag, int diagoff, T *dst, int64_t ld, hipStream_t s);
+
+template <typename T>
+int gfx950_fill(int m, int n, T value, T *x, int64_t rs, int64_t cs, hipStream_t s);
+
+// GEMM entry shared by the TRSM/TRMM schedules (same contract as
+// mublis_gfx950_?gemm_dev).
+template <typename T>
+int gfx950_gemm(int m, int n, int k, T alpha, const T *a, int64_t rs_a, int64_t cs_a,
+                const T *b, int64_t rs_b, int64_t cs_b, T beta, T *c, int64_t rs_c,
+                int64_t cs_c, hipStream_t s);
+
+#endif

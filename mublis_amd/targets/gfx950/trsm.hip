@@ -1,0 +1,241 @@
+// gfx950 triangular solve: B := alpha * A^{-1} * B for a lower or upper
+// triangular A on the left (the form the reference reduces every TRSM to,
+// frame/l3/trsm.c:35-54).
+//
+// The reference walks kc-row diagonal blocks, solving each with the fused
+// GEMMTRSM micro-kernel and pushing a rank-kc GEMM update through the rest
+// of B (frame/l3/trsm.c:91-431).  On the GPU a rank-64 update of all of B
+// per block would re-stream B from HBM m/64 times, so the schedule is
+// recursive instead: split A in halves, solve the first, update the second
+// half of B with ONE large MFMA GEMM (K = m/2), recurse.  All but O(m * NB * n)
+// of the flops run in big GEMMs; the NB x NB diagonal blocks are solved by
+// a leaf kernel in FP64 (also for float data), one lane per column of B,
+// multiplying by reciprocal diagonals like the reference's GEMMTRSM
+// (targets/reference/gemmtrsm_ukr.inc:29-40, frame/l1m/packm.c:143-144).
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <algorithm>
+
+#include "gfx950_common.h"
+
+namespace {
+
+constexpr int NB = 64;  // diagonal block (leaf) size = columns per leaf workgroup
+
+// diag_mode: 0 read the diagonal and use its reciprocal, 1 unit (never read),
+// 2 the stored diagonal is already the reciprocal (GEMMTRSM micro-kernel
+// contract, include/mublis_instantiate.h:84-86).
+//
+// Prep: every NB x NB diagonal block of A, in FP64, with zeros outside the
+// triangle and the reciprocal (or 1) on the diagonal -- the device
+// counterpart of packm_trsm_a (frame/l1m/packm.c:114-157), done once per
+// solve for all blocks in parallel.  Only the stored triangle is read.
+template <typename T, bool LOWER>
+__global__ __launch_bounds__(256) void trsm_prep(int m, const T *__restrict__ a,
+                                                 int64_t rs_a, int64_t cs_a,
+                                                 int diag_mode,
+                                                 double *__restrict__ out) {
+  const int blk = blockIdx.x;
+  const int base = blk * NB;
+  const int nb = min(NB, m - base);
+  double *o = out + (int64_t)blk * NB * NB;
+  for (int t = threadIdx.x; t < NB * NB; t += blockDim.x) {
+    const int i = t / NB, p = t % NB;  // o[i*NB + p] = A(base+i, base+p)
+    double v = 0.0;
+    if (i < nb && p < nb) {
+      const int64_t gi = base + i, gp = base + p;
+      if (i == p) {
+        if (diag_mode == 1) v = 1.0;
+        else {
+          const double d = (double)a[gi * rs_a + gi * cs_a];
+          v = diag_mode == 2 ? d : 1.0 / d;
+        }
+      } else if (LOWER ? (i > p) : (i < p)) {
+        v = (double)a[gi * rs_a + gp * cs_a];
+      }
+    }
+    o[t] = v;
+  }
+}
+
+// Leaf: solve one prepared NB x NB block against NB columns of B per
+// workgroup, one lane per column, in FP64.  The B tile moves through LDS so
+// that global loads and stores run along B's unit-stride dimension; the
+// block of A is read with uniform addresses (scalar loads), so each update
+// is one v_fma_f64 with a scalar operand.
+template <typename T, bool LOWER>
+__global__ __launch_bounds__(NB) void trsm_leaf(int nb, int n, T alpha,
+                                                const double *__restrict__ ad,
+                                                T *__restrict__ b, int64_t rs_b,
+                                                int64_t cs_b) {
+  __shared__ double xs[NB][NB + 1];
+  const int tid = threadIdx.x;
+  const int j0 = blockIdx.x * NB;
+  const bool rows_contig = rs_b == 1;
+  // coalesced load: lanes run along the unit-stride dimension of B
+#pragma unroll 4
+  for (int r = 0; r < NB; ++r) {
+    const int i = rows_contig ? tid : r, jj = rows_contig ? r : tid;
+    double v = 0.0;
+    if (i < nb && j0 + jj < n) v = (double)b[(int64_t)i * rs_b + (int64_t)(j0 + jj) * cs_b];
+    xs[i][jj] = v;
+  }
+  __syncthreads();
+  double x[NB];
+  const double al = (double)alpha;
+#pragma unroll
+  for (int i = 0; i < NB; ++i) x[i] = al * xs[i][tid];
+  if (LOWER) {
+#pragma unroll
+    for (int i = 0; i < NB; ++i) {
+      double s = x[i];
+#pragma unroll
+      for (int p = 0; p < i; ++p) s = fma(-ad[i * NB + p], x[p], s);
+      x[i] = s * ad[i * NB + i];
+    }
+  } else {
+#pragma unroll
+    for (int i = NB - 1; i >= 0; --i) {
+      double s = x[i];
+#pragma unroll
+      for (int p = i + 1; p < NB; ++p) s = fma(-ad[i * NB + p], x[p], s);
+      x[i] = s * ad[i * NB + i];
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < NB; ++i) xs[i][tid] = x[i];
+  __syncthreads();
+#pragma unroll 4
+  for (int r = 0; r < NB; ++r) {
+    const int i = rows_contig ? tid : r, jj = rows_contig ? r : tid;
+    if (i < nb && j0 + jj < n) b[(int64_t)i * rs_b + (int64_t)(j0 + jj) * cs_b] = (T)xs[i][jj];
+  }
+}
+
+template <typename T>
+int prep(bool lower, int m, const T *a, int64_t rs_a, int64_t cs_a, int diag_mode,
+         double *out, hipStream_t s) {
+  const int blocks = (m + NB - 1) / NB;
+  if (lower)
+    hipLaunchKernelGGL((trsm_prep<T, true>), dim3(blocks), dim3(256), 0, s, m, a, rs_a,
+                       cs_a, diag_mode, out);
+  else
+    hipLaunchKernelGGL((trsm_prep<T, false>), dim3(blocks), dim3(256), 0, s, m, a, rs_a,
+                       cs_a, diag_mode, out);
+  return hipGetLastError() == hipSuccess ? MUBLIS_OK : MUBLIS_EHIP;
+}
+
+template <typename T>
+int leaf(bool lower, int nb, int n, T alpha, const double *ad, T *b, int64_t rs_b,
+         int64_t cs_b, hipStream_t s) {
+  const int grid = (n + NB - 1) / NB;
+  if (lower)
+    hipLaunchKernelGGL((trsm_leaf<T, true>), dim3(grid), dim3(NB), 0, s, nb, n, alpha, ad,
+                       b, rs_b, cs_b);
+  else
+    hipLaunchKernelGGL((trsm_leaf<T, false>), dim3(grid), dim3(NB), 0, s, nb, n, alpha,
+                       ad, b, rs_b, cs_b);
+  return hipGetLastError() == hipSuccess ? MUBLIS_OK : MUBLIS_EHIP;
+}
+
+// `ad` holds the prepared diagonal blocks of this sub-problem (block 0 is
+// its top-left NB x NB block).
+template <typename T>
+int trsm_rec(bool lower, const double *ad, int m, int n, T alpha, const T *a,
+             int64_t rs_a, int64_t cs_a, T *b, int64_t rs_b, int64_t cs_b,
+             hipStream_t s) {
+  if (m <= NB) return leaf<T>(lower, m, n, alpha, ad, b, rs_b, cs_b, s);
+  // First part: a multiple of NB close to half, so GEMM K and M stay tile
+  // friendly at every level.
+  int m1 = (m / 2 + NB - 1) / NB * NB;
+  if (m1 >= m) m1 = m - NB;
+  const int m2 = m - m1;
+  const T *a11 = a;
+  const T *a21 = a + (int64_t)m1 * rs_a;
+  const T *a12 = a + (int64_t)m1 * cs_a;
+  const T *a22 = a + (int64_t)m1 * rs_a + (int64_t)m1 * cs_a;
+  T *b1 = b, *b2 = b + (int64_t)m1 * rs_b;
+  const double *ad1 = ad, *ad2 = ad + (int64_t)(m1 / NB) * NB * NB;
+  int st;
+  if (lower) {
+    if ((st = trsm_rec<T>(lower, ad1, m1, n, alpha, a11, rs_a, cs_a, b1, rs_b, cs_b,
+                          s)))
+      return st;
+    // B2 := alpha*B2 - A21 * X1 (the reference's GEMM with alpha -1 and
+    // beta = alpha_eff, frame/l3/trsm.c:165-197, 241-249)
+    if ((st = gfx950_gemm<T>(m2, n, m1, T(-1), a21, rs_a, cs_a, b1, rs_b, cs_b, alpha,
+                             b2, rs_b, cs_b, s)))
+      return st;
+    return trsm_rec<T>(lower, ad2, m2, n, T(1), a22, rs_a, cs_a, b2, rs_b, cs_b, s);
+  }
+  if ((st = trsm_rec<T>(lower, ad2, m2, n, alpha, a22, rs_a, cs_a, b2, rs_b, cs_b,
+                        s)))
+    return st;
+  if ((st = gfx950_gemm<T>(m1, n, m2, T(-1), a12, rs_a, cs_a, b2, rs_b, cs_b, alpha,
+                           b1, rs_b, cs_b, s)))
+    return st;
+  return trsm_rec<T>(lower, ad1, m1, n, T(1), a11, rs_a, cs_a, b1, rs_b, cs_b, s);
+}
+
+// Prepare all diagonal blocks, run the recursive schedule, release the
+// workspace once the stream has drained.
+template <typename T>
+int solve(bool lower, int diag_mode, int m, int n, T alpha, const T *a, int64_t rs_a,
+          int64_t cs_a, T *b, int64_t rs_b, int64_t cs_b, hipStream_t s) {
+  const int blocks = (m + NB - 1) / NB;
+  double *ad = (double *)mublis_rt_alloc(sizeof(double) * (size_t)blocks * NB * NB);
+  if (!ad) return MUBLIS_ENOMEM;
+  int st = prep<T>(lower, m, a, rs_a, cs_a, diag_mode, ad, s);
+  if (st == MUBLIS_OK) st = trsm_rec<T>(lower, ad, m, n, alpha, a, rs_a, cs_a, b, rs_b,
+                                        cs_b, s);
+  if (hipStreamSynchronize(s) != hipSuccess && st == MUBLIS_OK) st = MUBLIS_EHIP;
+  mublis_rt_free(ad);
+  return st;
+}
+
+template <typename T>
+int trsm_dev(int lower, int unit, int m, int n, T alpha, const T *a, int64_t rs_a,
+             int64_t cs_a, T *b, int64_t rs_b, int64_t cs_b, void *stream) {
+  if (m < 0 || n < 0) return MUBLIS_EINVAL;
+  if (m == 0 || n == 0) return MUBLIS_OK;
+  hipStream_t s = (hipStream_t)(stream ? stream : mublis_rt_stream());
+  if (alpha == T(0)) return gfx950_fill<T>(m, n, T(0), b, rs_b, cs_b, s);
+  return solve<T>(lower != 0, unit ? 1 : 0, m, n, alpha, a, rs_a, cs_a, b, rs_b, cs_b, s);
+}
+
+}  // namespace
+
+extern "C" int mublis_gfx950_dtrsm_inverted_dev(int lower, int m, int n,
+                                                const double *a, int64_t rs_a,
+                                                int64_t cs_a, double *b, int64_t rs_b,
+                                                int64_t cs_b, void *stream) {
+  if (m < 0 || n < 0 || m > NB) return MUBLIS_EINVAL;
+  if (m == 0 || n == 0) return MUBLIS_OK;
+  return solve<double>(lower != 0, 2, m, n, 1.0, a, rs_a, cs_a, b, rs_b, cs_b,
+                       (hipStream_t)(stream ? stream : mublis_rt_stream()));
+}
+extern "C" int mublis_gfx950_strsm_inverted_dev(int lower, int m, int n,
+                                                const float *a, int64_t rs_a,
+                                                int64_t cs_a, float *b, int64_t rs_b,
+                                                int64_t cs_b, void *stream) {
+  if (m < 0 || n < 0 || m > NB) return MUBLIS_EINVAL;
+  if (m == 0 || n == 0) return MUBLIS_OK;
+  return solve<float>(lower != 0, 2, m, n, 1.0f, a, rs_a, cs_a, b, rs_b, cs_b,
+                      (hipStream_t)(stream ? stream : mublis_rt_stream()));
+}
+
+extern "C" int mublis_gfx950_dtrsm_dev(int lower, int unit, int m, int n, double alpha,
+                                       const double *a, int64_t rs_a, int64_t cs_a,
+                                       double *b, int64_t rs_b, int64_t cs_b,
+                                       void *stream) {
+  return trsm_dev<double>(lower, unit, m, n, alpha, a, rs_a, cs_a, b, rs_b, cs_b,
+                          stream);
+}
+extern "C" int mublis_gfx950_strsm_dev(int lower, int unit, int m, int n, float alpha,
+                                       const float *a, int64_t rs_a, int64_t cs_a,
+                                       float *b, int64_t rs_b, int64_t cs_b,
+                                       void *stream) {
+  return trsm_dev<float>(lower, unit, m, n, alpha, a, rs_a, cs_a, b, rs_b, cs_b,
+                         stream);
+}

@@ -9,6 +9,7 @@ the untouched triangle checked bit for bit.
 """
 import ctypes
 import itertools
+import json
 import os
 
 import numpy as np
@@ -17,6 +18,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 ORACLE_LIB = os.path.join(ROOT, "oracle", "liboracle.so")
 REF_AVX2_LIB = os.path.join(ROOT, "oracle", "_ref", "libref_avx2.so")
 REF_REF_LIB = os.path.join(ROOT, "oracle", "_ref", "libref_ref.so")
+GOLDEN = os.path.join(ROOT, "tests", "golden")
 
 ROW, COL = 101, 102
 NOTR, TR = 111, 112
@@ -309,3 +311,54 @@ def ragged_cases():
                                 right=right, lower=lower, trans=trans, unit=unit,
                                 m=mm, n=nn))
     return out
+
+
+# ------------------------------------------------------- golden vectors --
+def load_golden(name):
+    z = np.load(os.path.join(GOLDEN, f"{name}.npz"))
+    meta = json.loads(str(z["meta"]))
+    cases = []
+    for i, m in enumerate(meta):
+        c = dict(m)
+        c["dtype"] = np.dtype(c["dtype"])
+        arr = {k: z[f"{i}_{k}"] for k in ("a", "b", "c0", "avx2", "ref") if f"{i}_{k}" in z.files}
+        cases.append((c, arr))
+    return cases
+
+
+def args_from_golden(c, arr, out):
+    """Rebuild the CBLAS argument list for a golden case from its arrays."""
+    r, co, order = c["routine"], c["corder"], c["order"]
+    a, b = arr["a"], arr.get("b")
+    lda = ld_of(a, order)
+    if r == "gemm":
+        al, be = (1.25, -0.5)
+        m, n = out.shape
+        k = a.shape[0] if c["ta"] else a.shape[1]
+        return (co, TR if c["ta"] else NOTR, TR if c["tb"] else NOTR, m, n, k,
+                al, a, lda, b, ld_of(b, order), be, out, ld_of(out, order))
+    if r == "symm":
+        m, n = out.shape
+        return (co, RIGHT if c["right"] else LEFT, LO if c["lower"] else UP, m, n,
+                0.75, a, lda, b, ld_of(b, order), -0.25, out, ld_of(out, order))
+    if r in ("syrk", "syr2k"):
+        n = out.shape[0]
+        k = a.shape[0] if c["trans"] else a.shape[1]
+        al, be = (1.25, -0.5) if r == "syrk" else (0.75, -0.25)
+        lst = [co, LO if c["lower"] else UP, TR if c["trans"] else NOTR, n, k, al,
+               a, lda]
+        if r == "syr2k":
+            lst += [b, ld_of(b, order)]
+        return tuple(lst + [be, out, ld_of(out, order)])
+    m, n = out.shape
+    return (co, RIGHT if c["right"] else LEFT, LO if c["lower"] else UP,
+            TR if c["trans"] else NOTR, UNIT if c["unit"] else NONUNIT, m, n,
+            0.75, a, lda, out, ld_of(out, order))
+
+
+def golden_mask(c, n):
+    if c["routine"] in ("syrk", "syr2k"):
+        return np.tril(np.ones((n, n), bool)) if c["lower"] else np.triu(np.ones((n, n), bool))
+    return None
+
+

@@ -28,7 +28,6 @@ from scipy.linalg import blas as sblas
 
 import l3cases as L
 
-GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
 
 @pytest.fixture(scope="module")
@@ -36,54 +35,6 @@ def orc():
     if not os.path.exists(L.ORACLE_LIB):
         subprocess.check_call(["make", "-C", os.path.join(L.ROOT, "oracle"), "oracle"])
     return L.Cblas(L.ORACLE_LIB, prefix="orc_", returns_status=True)
-
-
-def load_golden(name):
-    z = np.load(os.path.join(GOLDEN, f"{name}.npz"))
-    meta = json.loads(str(z["meta"]))
-    cases = []
-    for i, m in enumerate(meta):
-        c = dict(m)
-        c["dtype"] = np.dtype(c["dtype"])
-        arr = {k: z[f"{i}_{k}"] for k in ("a", "b", "c0", "avx2", "ref") if f"{i}_{k}" in z.files}
-        cases.append((c, arr))
-    return cases
-
-
-def args_from_golden(c, arr, out):
-    """Rebuild the CBLAS argument list for a golden case from its arrays."""
-    r, co, order = c["routine"], c["corder"], c["order"]
-    a, b = arr["a"], arr.get("b")
-    lda = L.ld_of(a, order)
-    if r == "gemm":
-        al, be = (1.25, -0.5)
-        m, n = out.shape
-        k = a.shape[0] if c["ta"] else a.shape[1]
-        return (co, L.TR if c["ta"] else L.NOTR, L.TR if c["tb"] else L.NOTR, m, n, k,
-                al, a, lda, b, L.ld_of(b, order), be, out, L.ld_of(out, order))
-    if r == "symm":
-        m, n = out.shape
-        return (co, L.RIGHT if c["right"] else L.LEFT, L.LO if c["lower"] else L.UP, m, n,
-                0.75, a, lda, b, L.ld_of(b, order), -0.25, out, L.ld_of(out, order))
-    if r in ("syrk", "syr2k"):
-        n = out.shape[0]
-        k = a.shape[0] if c["trans"] else a.shape[1]
-        al, be = (1.25, -0.5) if r == "syrk" else (0.75, -0.25)
-        lst = [co, L.LO if c["lower"] else L.UP, L.TR if c["trans"] else L.NOTR, n, k, al,
-               a, lda]
-        if r == "syr2k":
-            lst += [b, L.ld_of(b, order)]
-        return tuple(lst + [be, out, L.ld_of(out, order)])
-    m, n = out.shape
-    return (co, L.RIGHT if c["right"] else L.LEFT, L.LO if c["lower"] else L.UP,
-            L.TR if c["trans"] else L.NOTR, L.UNIT if c["unit"] else L.NONUNIT, m, n,
-            0.75, a, lda, out, L.ld_of(out, order))
-
-
-def golden_mask(c, n):
-    if c["routine"] in ("syrk", "syr2k"):
-        return np.tril(np.ones((n, n), bool)) if c["lower"] else np.triu(np.ones((n, n), bool))
-    return None
 
 
 def rel_err(x, y):
@@ -95,11 +46,11 @@ def rel_err(x, y):
 @pytest.mark.parametrize("name", ["l3_harness", "l3_ragged"])
 def test_oracle_matches_reference_build(orc, name):
     """Oracle vs the reference's compiled outputs, both configs."""
-    for c, arr in load_golden(name):
+    for c, arr in L.load_golden(name):
         out = np.array(arr["c0"], copy=True, order=c["order"])
-        st = orc.call(c["dtype"], c["routine"], *args_from_golden(c, arr, out))
+        st = orc.call(c["dtype"], c["routine"], *L.args_from_golden(c, arr, out))
         assert st == 0, L.case_id(c)
-        mask = golden_mask(c, out.shape[0])
+        mask = L.golden_mask(c, out.shape[0])
         for key in ("avx2", "ref"):
             try:
                 L.assert_ref_close(out, arr[key], c["dtype"], mask, arr["c0"])
