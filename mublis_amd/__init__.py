@@ -91,9 +91,21 @@ def _declare(lib):
 
 
 def load():
-    """Load libmublis.so (built in-tree).  Raises if it is missing."""
+    """Load libmublis.so (built in-tree).  Raises if it is missing.
+
+    When torch is importable it is imported first: torch ships its own
+    libamdhip64 (SONAME libamdhip64.so.7) and requests it under another
+    name, so loading this library first would put two HIP runtimes in one
+    process and torch would then see no GPU.  With torch loaded first, this
+    library's libamdhip64.so.7 dependency binds to the runtime already
+    present and device pointers are shared.
+    """
     global _lib
     if _lib is None:
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         if not os.path.exists(LIB_PATH):
             raise RuntimeError(
                 f"{LIB_PATH} is not built; run mublis_amd.build() (no CPU fallback)")

@@ -5,8 +5,9 @@
 // buffers and a pthread_once-cached context (frame/base/pool.c:120-318,
 // frame/base/safe_dispatch.c:45-71).  On the GPU the same roles are:
 //   * init once per process, cache success or failure;
-//   * one HIP stream per calling thread (default priority), recycled through
-//     a free list when the thread exits, never destroyed;
+//   * one HIP stream per calling thread (default priority, blocking with
+//     respect to the legacy default stream), recycled through a free list
+//     when the thread exits, never destroyed;
 //   * a caching arena of device blocks keyed by rounded size, under one
 //     mutex, never returned to HIP except to retry a failed hipMalloc.
 #include <hip/hip_runtime.h>
@@ -137,7 +138,11 @@ void *mublis_rt_stream(void) {
       }
     }
     if (!t_stream.s) {
-      hipError_t e = hipStreamCreateWithFlags(&t_stream.s, hipStreamNonBlocking);
+      // A blocking stream: work the caller queued on the legacy default
+      // stream (how most callers produce device operands) completes before
+      // this stream's kernels start, which the CBLAS ABI -- it has no stream
+      // argument -- needs for device pointers.
+      hipError_t e = hipStreamCreateWithFlags(&t_stream.s, hipStreamDefault);
       if (e != hipSuccess) {
         record(e, "hipStreamCreateWithFlags");
         t_stream.s = nullptr;

@@ -40,6 +40,9 @@ template <> struct Mfma<double> {
   __device__ static inline acc_t step(double a, double b, acc_t c) {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
   }
+  // Output row of register r in lane group g (= lane / 16): the FP64 MFMA
+  // interleaves rows across lane groups (measured on gfx950).
+  __device__ static inline int row(int g, int r) { return g + 4 * r; }
 };
 template <> struct Mfma<float> {
   typedef f4 acc_t;
@@ -48,6 +51,8 @@ template <> struct Mfma<float> {
   __device__ static inline acc_t step(float a, float b, acc_t c) {
     return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
   }
+  // The FP32 MFMA keeps 4 consecutive rows per lane group.
+  __device__ static inline int row(int g, int r) { return 4 * g + r; }
 };
 
 constexpr int BM = 128;       // workgroup tile rows
@@ -249,8 +254,8 @@ __global__ __launch_bounds__(NT, 2) void gemm_kernel(GemmArgs<T> p) {
     }
   }
 
-  // Epilogue.  MFMA 16x16 output layout: lane l, register r holds row
-  // 4*(l/16) + r, column l%16 of its 16 x 16 tile.
+  // Epilogue.  MFMA 16x16 output layout: lane l, register r holds column
+  // l%16 and row MF::row(l/16, r) of its 16 x 16 tile.
   const T alpha = p.alpha, beta = p.beta;
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
@@ -259,7 +264,7 @@ __global__ __launch_bounds__(NT, 2) void gemm_kernel(GemmArgs<T> p) {
       const int col = n0 + wn + j * 16 + lr;
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int row = m0 + wm + i * 16 + lk * 4 + r;
+        const int row = m0 + wm + i * 16 + MF::row(lk, r);
         if (row >= p.m || col >= p.n) continue;
         if (p.ji == 1 && col > row) continue;
         if (p.ji == 2 && col < row) continue;

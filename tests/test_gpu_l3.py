@@ -55,6 +55,31 @@ def normwise(actual, expected, bound_scale, k, dtype):
     return err, lim
 
 
+# ------------------------------------------------ exact integer products --
+@pytest.mark.parametrize("dt", L.DTYPES)
+@pytest.mark.parametrize("order,co", L.LAYOUTS)
+def test_exact_small_integer_gemm(prod, dt, order, co):
+    """Small-integer operands make every partial sum exact, so the MFMA
+    tiles' operand and output lane layouts, edge tiles and the epilogue are
+    checked bit for bit (shapes straddle the 16 x 16 MFMA and 128 x 128
+    workgroup tiles)."""
+    rng = np.random.default_rng(12)
+    for (m, n, k) in [(16, 16, 4), (20, 18, 7), (130, 129, 33), (257, 131, 70)]:
+        for ta in (0, 1):
+            for tb in (0, 1):
+                a = np.array(rng.integers(-3, 4, (k, m) if ta else (m, k)), dt, order=order)
+                b = np.array(rng.integers(-3, 4, (n, k) if tb else (k, n)), dt, order=order)
+                c0 = np.array(rng.integers(-3, 4, (m, n)), dt, order=order)
+                c = c0.copy(order=order)
+                prod.call(dt, "gemm", co, L.TR if ta else L.NOTR, L.TR if tb else L.NOTR, m,
+                          n, k, 2.0, a, L.ld_of(a, order), b, L.ld_of(b, order), -1.0, c,
+                          L.ld_of(c, order))
+                A = a.astype(np.int64).T if ta else a.astype(np.int64)
+                B = b.astype(np.int64).T if tb else b.astype(np.int64)
+                np.testing.assert_array_equal(c, (2 * A @ B - c0.astype(np.int64)).astype(dt),
+                                              err_msg=f"{m}x{n}x{k} ta{ta} tb{tb}")
+
+
 # ---------------------------------------------------------- golden cases --
 @pytest.mark.parametrize("name", ["l3_harness", "l3_ragged"])
 def test_golden_vectors_of_the_reference_build(prod, name):
@@ -339,11 +364,11 @@ def test_context_micro_kernels(lib):
         a11 = T + np.diag(1.0 / diag)                 # diagonal pre-inverted
         a11c = np.asfortranarray(a11).ravel(order="F")
         B11 = rng.standard_normal((mr, nr))
-        b11 = np.ascontiguousarray(B11).ravel()     # row-major MR x NR
+        R = 0.5 * B11 - A @ B
+        b11 = B11.ravel().copy()                    # row-major MR x NR, overwritten
         c11 = np.zeros((mr, nr), order="F")
         f(k, 0.5, a.ctypes.data, a11c.ctypes.data, b.ctypes.data, b11.ctypes.data,
           c11.ctypes.data, 1, mr, None)
-        R = 0.5 * B11 - A @ B
         X = np.linalg.solve(T + np.diag(diag), R)
         np.testing.assert_allclose(c11, X, rtol=1e-11, atol=1e-11)
         np.testing.assert_allclose(b11.reshape(mr, nr), X, rtol=1e-11, atol=1e-11)
