@@ -43,6 +43,11 @@ def parse():
     p.add_argument("--no-extra", action="store_true")
     p.add_argument("--cpu-slice", type=int, default=2048,
                    help="M = N of the CPU-baseline DGEMM slice (K stays 8192)")
+    p.add_argument("--only", choices=["dgemm", "dtrsm", "dsyrk", "sgemm"],
+                   help="profiling aid: run just this operation (3 calls) and exit")
+    p.add_argument("--traffic-probe", action="store_true",
+                   help="profiling aid: one 512 MiB device copy (counter "
+                        "calibration) then one DGEMM, for rocprofv3 --pmc runs")
     return p.parse_args()
 
 
@@ -94,6 +99,48 @@ def cpu_baseline(n_k, m_slice):
                       f"{dt:.2f} s"}
 
 
+def profile_mode(args, torch, M, lib, dev, g):
+    """Run one operation of the workload a few times with nothing else on the
+    GPU, for kernel-trace or counter collection."""
+    n = args.n
+    if args.traffic_probe:
+        src = torch.empty(1 << 26, dtype=torch.float64, device=dev)   # 512 MiB
+        src.fill_(1.0)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)                    # calibration: 512 MiB read + written
+        torch.cuda.synchronize()
+        what, calls = "dgemm", 1
+    else:
+        what, calls = args.only, 3
+    if what in ("dgemm", "sgemm"):
+        dt = torch.float64 if what == "dgemm" else torch.float32
+        A = torch.randn((n, n), dtype=dt, device=dev, generator=g)
+        B = torch.randn((n, n), dtype=dt, device=dev, generator=g)
+        C = torch.zeros((n, n), dtype=dt, device=dev)
+        f = lib.cblas_dgemm if what == "dgemm" else lib.cblas_sgemm
+        for _ in range(calls):
+            f(COL, TR, NOTR, n, n, n, 1.0, M.ptr(A), n, M.ptr(B), n, 0.0, M.ptr(C), n)
+    elif what == "dtrsm":
+        m, nrhs = n, n // 2
+        At = torch.randn((m, m), dtype=torch.float64, device=dev, generator=g) / m ** 0.5
+        d = 1.0 + torch.randn(m, dtype=torch.float64, device=dev, generator=g).abs()
+        At = torch.triu(At, 1) + torch.diag(d)
+        B0 = torch.randn((nrhs, m), dtype=torch.float64, device=dev, generator=g)
+        for _ in range(calls):
+            Bt = B0.clone()
+            lib.cblas_dtrsm(COL, LEFT, LO, NOTR, NONUNIT, m, nrhs, 1.0, M.ptr(At), m,
+                            M.ptr(Bt), m)
+    elif what == "dsyrk":
+        k = n // 4
+        As = torch.randn((k, n), dtype=torch.float64, device=dev, generator=g)
+        C = torch.zeros((n, n), dtype=torch.float64, device=dev)
+        for _ in range(calls):
+            lib.cblas_dsyrk(COL, LO, NOTR, n, k, 1.0, M.ptr(As), n, 0.0, M.ptr(C), n)
+    torch.cuda.synchronize()
+    print(f"profile mode done: {what} x{calls}", flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -108,6 +155,8 @@ def main():
     dev = torch.device("cuda:0")
     n = args.n
     g = torch.Generator(device=dev).manual_seed(314159)
+    if args.only or args.traffic_probe:
+        return profile_mode(args, torch, M, lib, dev, g)
     # Column-major K x M A is the row-major (M, K) tensor; op(A) = A^T.
     A = torch.randn((n, n), dtype=torch.float64, device=dev, generator=g)
     B = torch.randn((n, n), dtype=torch.float64, device=dev, generator=g)
