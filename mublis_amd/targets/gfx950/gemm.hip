@@ -55,11 +55,18 @@ template <> struct Mfma<float> {
   __device__ static inline int row(int g, int r) { return 4 * g + r; }
 };
 
-constexpr int BM = 128;       // workgroup tile rows
-constexpr int BN = 128;       // workgroup tile cols
-constexpr int NT = 256;       // threads per workgroup (4 waves)
+constexpr int NT = 256;       // threads per workgroup (4 waves, 2 x 2)
 constexpr int LDS_PAD = 16;   // elements; shifts successive k rows by half the banks
-constexpr int S = BM + LDS_PAD;
+
+// Workgroup tile TILE x TILE: 128 for large problems, 64 when the 128-tile
+// grid would leave most CUs idle (the skinny GEMMs of the TRSM recursion,
+// small batched problems).  Each wave owns (TILE/2)^2 = TI x TI MFMA tiles.
+template <int TILE>
+struct Tile {
+  static constexpr int BM = TILE, BN = TILE, S = TILE + LDS_PAD;
+  static constexpr int TI = TILE / 32;  // 16 x 16 MFMA tiles per wave and dimension
+  static constexpr int MIN_WG_PER_CU = TILE == 128 ? 2 : 4;
+};
 
 template <typename T>
 struct GemmArgs {
@@ -109,7 +116,7 @@ __device__ inline void map_tile(int w, int ntiles, int tiles_m, int tiles_n,
 
 // One operand slice (A: BM x BK, B: BK x BN) in registers.  `c` runs along the
 // operand's unit-stride dimension, `o` along the other; V elements per vector.
-template <typename T, int LC, int LO>
+template <typename T, int LC, int LO, int S>
 struct Slice {
   static constexpr int V = 16 / sizeof(T);
   static constexpr int NV = LC * LO / (V * NT);
@@ -158,18 +165,19 @@ struct Slice {
   }
 };
 
-template <typename T, bool A_KC, bool B_NC>
+template <typename T, int TILE, bool A_KC, bool B_NC>
 struct Operands {
   static constexpr int BK = Mfma<T>::BK;
+  static constexpr int BM = Tile<TILE>::BM, BN = Tile<TILE>::BN, S = Tile<TILE>::S;
   // A is m x k: K-contiguous -> (c = k, o = m); else (c = m, o = k).
-  typedef Slice<T, A_KC ? BK : BM, A_KC ? BM : BK> SA;
+  typedef Slice<T, A_KC ? BK : BM, A_KC ? BM : BK, S> SA;
   // B is k x n: N-contiguous -> (c = n, o = k); else (c = k, o = n).
-  typedef Slice<T, B_NC ? BN : BK, B_NC ? BK : BN> SB;
+  typedef Slice<T, B_NC ? BN : BK, B_NC ? BK : BN, S> SB;
 };
 
-template <typename T, bool A_KC, bool B_NC>
-__device__ inline void load_slices(typename Operands<T, A_KC, B_NC>::SA &ra,
-                                   typename Operands<T, A_KC, B_NC>::SB &rb,
+template <typename T, int TILE, bool A_KC, bool B_NC>
+__device__ inline void load_slices(typename Operands<T, TILE, A_KC, B_NC>::SA &ra,
+                                   typename Operands<T, TILE, A_KC, B_NC>::SB &rb,
                                    const T *a, int64_t lda, const T *b,
                                    int64_t ldb, int m0, int n0, int k0, int m,
                                    int n, int k, bool va, bool vb, int tid) {
@@ -179,12 +187,14 @@ __device__ inline void load_slices(typename Operands<T, A_KC, B_NC>::SA &ra,
   else rb.load(b, ldb, k0, n0, k, n, vb, tid);
 }
 
-template <typename T, bool A_KC, bool B_NC>
-__global__ __launch_bounds__(NT, 2) void gemm_kernel(GemmArgs<T> p) {
+template <typename T, int TILE, bool A_KC, bool B_NC>
+__global__ __launch_bounds__(NT, Tile<TILE>::MIN_WG_PER_CU) void gemm_kernel(GemmArgs<T> p) {
   typedef Mfma<T> MF;
   typedef typename MF::acc_t acc_t;
   constexpr int BK = MF::BK;
-  typedef Operands<T, A_KC, B_NC> Ops;
+  constexpr int BM = Tile<TILE>::BM, BN = Tile<TILE>::BN, S = Tile<TILE>::S;
+  constexpr int TI = Tile<TILE>::TI, WT = TILE / 2;
+  typedef Operands<T, TILE, A_KC, B_NC> Ops;
   __shared__ T As[2][BK * S];
   __shared__ T Bs[2][BK * S];
 
@@ -193,14 +203,14 @@ __global__ __launch_bounds__(NT, 2) void gemm_kernel(GemmArgs<T> p) {
   const int m0 = tm * BM, n0 = tn * BN;
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
-  const int wm = (wave & 1) * 64, wn = (wave >> 1) * 64;
+  const int wm = (wave & 1) * WT, wn = (wave >> 1) * WT;
   const int lr = lane & 15, lk = lane >> 4;
 
-  acc_t acc[4][4];
+  acc_t acc[TI][TI];
 #pragma unroll
-  for (int i = 0; i < 4; ++i)
+  for (int i = 0; i < TI; ++i)
 #pragma unroll
-    for (int j = 0; j < 4; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
+    for (int j = 0; j < TI; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
 
   typename Ops::SA ra;
   typename Ops::SB rb;
@@ -217,8 +227,8 @@ __global__ __launch_bounds__(NT, 2) void gemm_kernel(GemmArgs<T> p) {
     const int ktiles = (p.k + BK - 1) / BK;
     if (ktiles == 0) continue;
 
-    load_slices<T, A_KC, B_NC>(ra, rb, a, lda, b, ldb, m0, n0, 0, p.m, p.n, p.k, va,
-                               vb, tid);
+    load_slices<T, TILE, A_KC, B_NC>(ra, rb, a, lda, b, ldb, m0, n0, 0, p.m, p.n, p.k,
+                                     va, vb, tid);
     __syncthreads();  // previous segment's readers are done with buffer 0
     ra.template store<A_KC>(As[0], tid);
     rb.template store<!B_NC>(Bs[0], tid);
@@ -229,22 +239,22 @@ __global__ __launch_bounds__(NT, 2) void gemm_kernel(GemmArgs<T> p) {
       const int cur = kt & 1;
       const bool more = kt + 1 < ktiles;
       if (more)
-        load_slices<T, A_KC, B_NC>(ra, rb, a, lda, b, ldb, m0, n0, (kt + 1) * BK, p.m,
-                                   p.n, p.k, va, vb, tid);
+        load_slices<T, TILE, A_KC, B_NC>(ra, rb, a, lda, b, ldb, m0, n0, (kt + 1) * BK,
+                                         p.m, p.n, p.k, va, vb, tid);
       const T *as = As[cur];
       const T *bs = Bs[cur];
 #pragma unroll
       for (int kk = 0; kk < BK / 4; ++kk) {
-        T af[4], bf[4];
+        T af[TI], bf[TI];
         const int row = (kk * 4 + lk) * S;
 #pragma unroll
-        for (int i = 0; i < 4; ++i) af[i] = as[row + wm + i * 16 + lr];
+        for (int i = 0; i < TI; ++i) af[i] = as[row + wm + i * 16 + lr];
 #pragma unroll
-        for (int j = 0; j < 4; ++j) bf[j] = bs[row + wn + j * 16 + lr];
+        for (int j = 0; j < TI; ++j) bf[j] = bs[row + wn + j * 16 + lr];
 #pragma unroll
-        for (int i = 0; i < 4; ++i)
+        for (int i = 0; i < TI; ++i)
 #pragma unroll
-          for (int j = 0; j < 4; ++j) acc[i][j] = MF::step(af[i], bf[j], acc[i][j]);
+          for (int j = 0; j < TI; ++j) acc[i][j] = MF::step(af[i], bf[j], acc[i][j]);
       }
       if (more) {
         ra.template store<A_KC>(As[cur ^ 1], tid);
@@ -258,9 +268,9 @@ __global__ __launch_bounds__(NT, 2) void gemm_kernel(GemmArgs<T> p) {
   // l%16 and row MF::row(l/16, r) of its 16 x 16 tile.
   const T alpha = p.alpha, beta = p.beta;
 #pragma unroll
-  for (int i = 0; i < 4; ++i) {
+  for (int i = 0; i < TI; ++i) {
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
+    for (int j = 0; j < TI; ++j) {
       const int col = n0 + wn + j * 16 + lr;
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
@@ -300,9 +310,16 @@ bool aligned16(const T *x, int64_t ld) {
   return ((uintptr_t)x % 16 == 0) && ((ld * (int64_t)sizeof(T)) % 16 == 0);
 }
 
-template <typename T, bool AK, bool BN_>
-hipError_t launch(const GemmArgs<T> &p, int grid, hipStream_t s) {
-  hipLaunchKernelGGL((gemm_kernel<T, AK, BN_>), dim3(grid), dim3(NT), 0, s, p);
+template <typename T, int TILE>
+hipError_t launch(GemmArgs<T> p, bool akc, bool bnc, hipStream_t s) {
+  p.tiles_m = (p.m + TILE - 1) / TILE;
+  p.tiles_n = (p.n + TILE - 1) / TILE;
+  p.ntiles = p.tri ? p.tiles_m * (p.tiles_m + 1) / 2 : p.tiles_m * p.tiles_n;
+  const dim3 grid(p.ntiles), block(NT);
+  if (akc && bnc) hipLaunchKernelGGL((gemm_kernel<T, TILE, true, true>), grid, block, 0, s, p);
+  else if (akc) hipLaunchKernelGGL((gemm_kernel<T, TILE, true, false>), grid, block, 0, s, p);
+  else if (bnc) hipLaunchKernelGGL((gemm_kernel<T, TILE, false, true>), grid, block, 0, s, p);
+  else hipLaunchKernelGGL((gemm_kernel<T, TILE, false, false>), grid, block, 0, s, p);
   return hipGetLastError();
 }
 
@@ -365,19 +382,17 @@ int gemm_dev(int m, int n, int k, T alpha, const T *a, int64_t rs_a, int64_t cs_
     p.a2 = same2 ? a2 : nullptr; p.rs_a2 = rs_a2; p.cs_a2 = cs_a2;
     p.b2 = same2 ? b2 : nullptr; p.rs_b2 = rs_b2; p.cs_b2 = cs_b2;
     p.c = c; p.rs_c = rs_c; p.cs_c = cs_c; p.ji = ji;
-    p.tiles_m = (m + BM - 1) / BM;
-    p.tiles_n = (n + BN - 1) / BN;
     p.tri = (ji != 0 && m == n) ? 1 : 0;
-    p.ntiles = p.tri ? p.tiles_m * (p.tiles_m + 1) / 2 : p.tiles_m * p.tiles_n;
     p.vec_a = aligned16(a, akc ? rs_a : cs_a);
     p.vec_b = aligned16(b, bnc ? rs_b : cs_b);
     p.vec_a2 = a2 ? aligned16(a2, akc ? rs_a2 : cs_a2) : 0;
     p.vec_b2 = b2 ? aligned16(b2, bnc ? rs_b2 : cs_b2) : 0;
-    hipError_t e;
-    if (akc && bnc) e = launch<T, true, true>(p, p.ntiles, s);
-    else if (akc) e = launch<T, true, false>(p, p.ntiles, s);
-    else if (bnc) e = launch<T, false, true>(p, p.ntiles, s);
-    else e = launch<T, false, false>(p, p.ntiles, s);
+    // 128 x 128 tiles unless that grid would give fewer than two
+    // workgroups per CU (256 CUs).
+    const int64_t t128 = (int64_t)((m + 127) / 128) * ((n + 127) / 128);
+    const int64_t n128 = p.tri ? ((m + 127) / 128) * ((m + 127) / 128 + 1) / 2 : t128;
+    hipError_t e = n128 >= 512 ? launch<T, 128>(p, akc, bnc, s)
+                               : launch<T, 64>(p, akc, bnc, s);
     if (e != hipSuccess) st = MUBLIS_EHIP;
     if (st == MUBLIS_OK && a2 && !same2) {
       // Layouts differ between the two products: add the second one in a

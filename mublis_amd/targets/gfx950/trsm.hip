@@ -27,10 +27,13 @@ constexpr int NB = 64;  // diagonal block (leaf) size = columns per leaf workgro
 // 2 the stored diagonal is already the reciprocal (GEMMTRSM micro-kernel
 // contract, include/mublis_instantiate.h:84-86).
 //
-// Prep: every NB x NB diagonal block of A, in FP64, with zeros outside the
-// triangle and the reciprocal (or 1) on the diagonal -- the device
-// counterpart of packm_trsm_a (frame/l1m/packm.c:114-157), done once per
-// solve for all blocks in parallel.  Only the stored triangle is read.
+// Prep: every NB x NB diagonal block of A in FP64, column-major, with zeros
+// outside the triangle AND on the diagonal, followed by the NB reciprocal
+// diagonal entries (1 for UNIT, never read) -- the device counterpart of
+// packm_trsm_a (frame/l1m/packm.c:114-157), for all blocks in one launch.
+// Only the stored triangle of A is read.
+constexpr int BLK = NB * NB + NB;  // doubles per prepared block
+
 template <typename T, bool LOWER>
 __global__ __launch_bounds__(256) void trsm_prep(int m, const T *__restrict__ a,
                                                  int64_t rs_a, int64_t cs_a,
@@ -39,77 +42,133 @@ __global__ __launch_bounds__(256) void trsm_prep(int m, const T *__restrict__ a,
   const int blk = blockIdx.x;
   const int base = blk * NB;
   const int nb = min(NB, m - base);
-  double *o = out + (int64_t)blk * NB * NB;
+  double *o = out + (int64_t)blk * BLK;
   for (int t = threadIdx.x; t < NB * NB; t += blockDim.x) {
-    const int i = t / NB, p = t % NB;  // o[i*NB + p] = A(base+i, base+p)
+    const int p = t / NB, i = t % NB;  // o[p*NB + i] = A(base+i, base+p)
     double v = 0.0;
-    if (i < nb && p < nb) {
-      const int64_t gi = base + i, gp = base + p;
-      if (i == p) {
-        if (diag_mode == 1) v = 1.0;
-        else {
-          const double d = (double)a[gi * rs_a + gi * cs_a];
-          v = diag_mode == 2 ? d : 1.0 / d;
-        }
-      } else if (LOWER ? (i > p) : (i < p)) {
-        v = (double)a[gi * rs_a + gp * cs_a];
+    if (i < nb && p < nb && (LOWER ? (i > p) : (i < p)))
+      v = (double)a[(int64_t)(base + i) * rs_a + (int64_t)(base + p) * cs_a];
+    o[t] = v;
+  }
+  for (int i = threadIdx.x; i < NB; i += blockDim.x) {
+    double v = 0.0;
+    if (i < nb) {
+      if (diag_mode == 1) v = 1.0;
+      else {
+        const int64_t g = base + i;
+        const double d = (double)a[g * rs_a + g * cs_a];
+        v = diag_mode == 2 ? d : 1.0 / d;
       }
     }
-    o[t] = v;
+    o[NB * NB + i] = v;
   }
 }
 
-// Leaf: solve one prepared NB x NB block against NB columns of B per
-// workgroup, one lane per column, in FP64.  The B tile moves through LDS so
-// that global loads and stores run along B's unit-stride dimension; the
-// block of A is read with uniform addresses (scalar loads), so each update
-// is one v_fma_f64 with a scalar operand.
+// Broadcast lane `src` of every group of 4 lanes to the whole group
+// (DPP quad permute; `src` must be a constant after unrolling).
+__device__ inline double quad_bcast(double v, int src) {
+  int lo = __double2loint(v), hi = __double2hiint(v);
+  switch (src) {
+    case 0:
+      lo = __builtin_amdgcn_update_dpp(lo, lo, 0x00, 0xF, 0xF, false);
+      hi = __builtin_amdgcn_update_dpp(hi, hi, 0x00, 0xF, 0xF, false);
+      break;
+    case 1:
+      lo = __builtin_amdgcn_update_dpp(lo, lo, 0x55, 0xF, 0xF, false);
+      hi = __builtin_amdgcn_update_dpp(hi, hi, 0x55, 0xF, 0xF, false);
+      break;
+    case 2:
+      lo = __builtin_amdgcn_update_dpp(lo, lo, 0xAA, 0xF, 0xF, false);
+      hi = __builtin_amdgcn_update_dpp(hi, hi, 0xAA, 0xF, 0xF, false);
+      break;
+    default:
+      lo = __builtin_amdgcn_update_dpp(lo, lo, 0xFF, 0xF, 0xF, false);
+      hi = __builtin_amdgcn_update_dpp(hi, hi, 0xFF, 0xF, 0xF, false);
+      break;
+  }
+  return __hiloint2double(hi, lo);
+}
+
+// Leaf: solve one prepared NB x NB block against LC = 16 columns of B per
+// workgroup of 64 lanes, in FP64.  Lane 4c + q owns column c and rows
+// 16q .. 16q + 15.  The substitution runs column by column: x_p is finished
+// by its owner lane (times the reciprocal diagonal), broadcast to the other
+// three lanes of the column with a DPP quad permute, and every lane then
+// updates its 16 rows with 16 independent FMAs (rows the triangle does not
+// reach have a zero coefficient).  The B tile moves through LDS so global
+// accesses run along B's unit stride.
+constexpr int LC = 16;
+
 template <typename T, bool LOWER>
-__global__ __launch_bounds__(NB) void trsm_leaf(int nb, int n, T alpha,
-                                                const double *__restrict__ ad,
+__global__ __launch_bounds__(64) void trsm_leaf(int nb, int n, T alpha,
+                                                const double *__restrict__ blk,
                                                 T *__restrict__ b, int64_t rs_b,
                                                 int64_t cs_b) {
-  __shared__ double xs[NB][NB + 1];
+  __shared__ __attribute__((aligned(16))) double as[NB * NB];
+  __shared__ double dinv[NB];
+  __shared__ double xs[NB][LC + 1];
   const int tid = threadIdx.x;
-  const int j0 = blockIdx.x * NB;
-  const bool rows_contig = rs_b == 1;
-  // coalesced load: lanes run along the unit-stride dimension of B
-#pragma unroll 4
-  for (int r = 0; r < NB; ++r) {
-    const int i = rows_contig ? tid : r, jj = rows_contig ? r : tid;
-    double v = 0.0;
-    if (i < nb && j0 + jj < n) v = (double)b[(int64_t)i * rs_b + (int64_t)(j0 + jj) * cs_b];
-    xs[i][jj] = v;
+  {
+    // 32 independent 16-byte loads per lane in flight, then the LDS stores
+    typedef double d2 __attribute__((ext_vector_type(2)));
+    const d2 *src = reinterpret_cast<const d2 *>(blk);
+    d2 *dst = reinterpret_cast<d2 *>(as);
+    d2 tmp[NB * NB / 2 / 64];
+#pragma unroll
+    for (int u = 0; u < NB * NB / 2 / 64; ++u) tmp[u] = src[u * 64 + tid];
+#pragma unroll
+    for (int u = 0; u < NB * NB / 2 / 64; ++u) dst[u * 64 + tid] = tmp[u];
   }
-  __syncthreads();
-  double x[NB];
-  const double al = (double)alpha;
+  dinv[tid] = blk[NB * NB + tid];
+  const int j0 = blockIdx.x * LC;
+  if (rs_b == 1) {
 #pragma unroll
-  for (int i = 0; i < NB; ++i) x[i] = al * xs[i][tid];
-  if (LOWER) {
-#pragma unroll
-    for (int i = 0; i < NB; ++i) {
-      double s = x[i];
-#pragma unroll
-      for (int p = 0; p < i; ++p) s = fma(-ad[i * NB + p], x[p], s);
-      x[i] = s * ad[i * NB + i];
+    for (int c = 0; c < LC; ++c) {
+      double v = 0.0;
+      if (tid < nb && j0 + c < n) v = (double)b[(int64_t)tid + (int64_t)(j0 + c) * cs_b];
+      xs[tid][c] = v;
     }
   } else {
 #pragma unroll
-    for (int i = NB - 1; i >= 0; --i) {
-      double s = x[i];
-#pragma unroll
-      for (int p = i + 1; p < NB; ++p) s = fma(-ad[i * NB + p], x[p], s);
-      x[i] = s * ad[i * NB + i];
+    for (int it = 0; it < NB * LC / 64; ++it) {
+      const int lin = it * 64 + tid, i = lin / LC, c = lin % LC;
+      double v = 0.0;
+      if (i < nb && j0 + c < n)
+        v = (double)b[(int64_t)i * rs_b + (int64_t)(j0 + c) * cs_b];
+      xs[i][c] = v;
     }
   }
-#pragma unroll
-  for (int i = 0; i < NB; ++i) xs[i][tid] = x[i];
   __syncthreads();
-#pragma unroll 4
-  for (int r = 0; r < NB; ++r) {
-    const int i = rows_contig ? tid : r, jj = rows_contig ? r : tid;
-    if (i < nb && j0 + jj < n) b[(int64_t)i * rs_b + (int64_t)(j0 + jj) * cs_b] = (T)xs[i][jj];
+  const int c = tid >> 2, q = tid & 3;
+  const double al = (double)alpha;
+  double x[16];
+#pragma unroll
+  for (int r = 0; r < 16; ++r) x[r] = al * xs[16 * q + r][c];
+#pragma unroll
+  for (int s = 0; s < NB; ++s) {
+    const int p = LOWER ? s : NB - 1 - s;
+    const int qp = p >> 4, rp = p & 15;
+    const double xp = quad_bcast(x[rp] * dinv[p], qp);
+    x[rp] = (q == qp) ? xp : x[rp];
+    const double *colp = &as[p * NB + 16 * q];
+#pragma unroll
+    for (int r = 0; r < 16; ++r) x[r] = fma(-colp[r], xp, x[r]);
+  }
+#pragma unroll
+  for (int r = 0; r < 16; ++r) xs[16 * q + r][c] = x[r];
+  __syncthreads();
+  if (rs_b == 1) {
+#pragma unroll
+    for (int cc = 0; cc < LC; ++cc)
+      if (tid < nb && j0 + cc < n)
+        b[(int64_t)tid + (int64_t)(j0 + cc) * cs_b] = (T)xs[tid][cc];
+  } else {
+#pragma unroll
+    for (int it = 0; it < NB * LC / 64; ++it) {
+      const int lin = it * 64 + tid, i = lin / LC, cc = lin % LC;
+      if (i < nb && j0 + cc < n)
+        b[(int64_t)i * rs_b + (int64_t)(j0 + cc) * cs_b] = (T)xs[i][cc];
+    }
   }
 }
 
@@ -129,7 +188,7 @@ int prep(bool lower, int m, const T *a, int64_t rs_a, int64_t cs_a, int diag_mod
 template <typename T>
 int leaf(bool lower, int nb, int n, T alpha, const double *ad, T *b, int64_t rs_b,
          int64_t cs_b, hipStream_t s) {
-  const int grid = (n + NB - 1) / NB;
+  const int grid = (n + LC - 1) / LC;
   if (lower)
     hipLaunchKernelGGL((trsm_leaf<T, true>), dim3(grid), dim3(NB), 0, s, nb, n, alpha, ad,
                        b, rs_b, cs_b);
@@ -156,7 +215,7 @@ int trsm_rec(bool lower, const double *ad, int m, int n, T alpha, const T *a,
   const T *a12 = a + (int64_t)m1 * cs_a;
   const T *a22 = a + (int64_t)m1 * rs_a + (int64_t)m1 * cs_a;
   T *b1 = b, *b2 = b + (int64_t)m1 * rs_b;
-  const double *ad1 = ad, *ad2 = ad + (int64_t)(m1 / NB) * NB * NB;
+  const double *ad1 = ad, *ad2 = ad + (int64_t)(m1 / NB) * BLK;
   int st;
   if (lower) {
     if ((st = trsm_rec<T>(lower, ad1, m1, n, alpha, a11, rs_a, cs_a, b1, rs_b, cs_b,
@@ -184,7 +243,7 @@ template <typename T>
 int solve(bool lower, int diag_mode, int m, int n, T alpha, const T *a, int64_t rs_a,
           int64_t cs_a, T *b, int64_t rs_b, int64_t cs_b, hipStream_t s) {
   const int blocks = (m + NB - 1) / NB;
-  double *ad = (double *)mublis_rt_alloc(sizeof(double) * (size_t)blocks * NB * NB);
+  double *ad = (double *)mublis_rt_alloc(sizeof(double) * (size_t)blocks * BLK);
   if (!ad) return MUBLIS_ENOMEM;
   int st = prep<T>(lower, m, a, rs_a, cs_a, diag_mode, ad, s);
   if (st == MUBLIS_OK) st = trsm_rec<T>(lower, ad, m, n, alpha, a, rs_a, cs_a, b, rs_b,
