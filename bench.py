@@ -41,9 +41,11 @@ def parse():
     p.add_argument("--n", type=int, default=8192)
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--no-extra", action="store_true")
+    p.add_argument("--precision-n", type=int, default=16384,
+                   help="SGEMM size of the native-vs-BF16x3 comparison (0: skip)")
     p.add_argument("--cpu-slice", type=int, default=2048,
                    help="M = N of the CPU-baseline DGEMM slice (K stays 8192)")
-    p.add_argument("--only", choices=["dgemm", "dtrsm", "dsyrk", "sgemm"],
+    p.add_argument("--only", choices=["dgemm", "dtrsm", "dsyrk", "sgemm", "sgemm_bf16x3"],
                    help="profiling aid: run just this operation (3 calls) and exit")
     p.add_argument("--traffic-probe", action="store_true",
                    help="profiling aid: one 512 MiB device copy (counter "
@@ -113,7 +115,19 @@ def profile_mode(args, torch, M, lib, dev, g):
         what, calls = "dgemm", 1
     else:
         what, calls = args.only, 3
-    if what in ("dgemm", "sgemm"):
+    if what == "sgemm_bf16x3":
+        nn = args.precision_n or n
+        A = torch.randn((nn, nn), device=dev, generator=g)
+        B = torch.randn((nn, nn), device=dev, generator=g)
+        C = torch.zeros((nn, nn), device=dev)
+        f = lib.mublis_sgemm_bf16x3
+        f.argtypes = [ctypes.c_int] * 5 + [ctypes.c_float, ctypes.c_void_p, ctypes.c_int,
+                                           ctypes.c_int, ctypes.c_void_p, ctypes.c_int,
+                                           ctypes.c_int, ctypes.c_float, ctypes.c_void_p,
+                                           ctypes.c_int, ctypes.c_int]
+        for _ in range(calls):
+            f(1, 0, nn, nn, nn, 1.0, M.ptr(A), 1, nn, M.ptr(B), 1, nn, 0.0, M.ptr(C), 1, nn)
+    elif what in ("dgemm", "sgemm"):
         dt = torch.float64 if what == "dgemm" else torch.float32
         A = torch.randn((n, n), dtype=dt, device=dev, generator=g)
         B = torch.randn((n, n), dtype=dt, device=dev, generator=g)
@@ -139,6 +153,173 @@ def profile_mode(args, torch, M, lib, dev, g):
             lib.cblas_dsyrk(COL, LO, NOTR, n, k, 1.0, M.ptr(As), n, 0.0, M.ptr(C), n)
     torch.cuda.synchronize()
     print(f"profile mode done: {what} x{calls}", flush=True)
+
+
+def sampled_gemm_error(torch, At, Bt, Ct, count=64, seed=0):
+    """Normwise relative error of `count` sampled entries of C = op(A) B
+    (tensors in the benchmark's convention: C(i,j) = Ct[j, i] = sum_p
+    At[i, p] Bt[j, p]) against long-double sums of the same inputs."""
+    n_i, n_j = At.shape[0], Bt.shape[0]
+    rng = np.random.default_rng(seed)
+    I = torch.from_numpy(rng.integers(0, n_i, count)).to(At.device)
+    J = torch.from_numpy(rng.integers(0, n_j, count)).to(At.device)
+    rows = At[I].double().cpu().numpy().astype(np.longdouble)
+    cols = Bt[J].double().cpu().numpy().astype(np.longdouble)
+    exact = np.sum(rows * cols, axis=1)
+    got = Ct[J, I].double().cpu().numpy()
+    return float(np.linalg.norm(got - exact) / np.linalg.norm(exact))
+
+
+def trsm_residual(torch, At, Xt, Bt, seed=1):
+    """||A (X v) - B v|| / (||A||_F ||X||_F ||v||) for a random v, in long
+    double via the oracle's matrix-vector helpers (lower, no transpose).
+    At, Xt, Bt are column-major matrices stored as transposed tensors."""
+    lib = ctypes.CDLL(os.path.join(ROOT, "oracle", "liboracle.so"))
+    double = At.dtype == torch.float64
+    gemv = lib.orc_dgemv_ld if double else lib.orc_sgemv_ld
+    trmv = lib.orc_dtrmv_ld if double else lib.orc_strmv_ld
+    vp, ci = ctypes.c_void_p, ctypes.c_int
+    gemv.argtypes = [ci, ci, vp, ci, vp, vp]
+    trmv.argtypes = [ci, ci, ci, ci, vp, ci, vp, vp]
+    m, n = At.shape[0], Xt.shape[0]
+    A, X, B = At.cpu().numpy(), Xt.cpu().numpy(), Bt.cpu().numpy()
+    v = np.random.default_rng(seed).standard_normal(n)
+    xv, bv, ay = np.zeros(m), np.zeros(m), np.zeros(m)
+    gemv(m, n, X.ctypes.data, m, v.ctypes.data, xv.ctypes.data)
+    gemv(m, n, B.ctypes.data, m, v.ctypes.data, bv.ctypes.data)
+    trmv(LO, NOTR, NONUNIT, m, A.ctypes.data, m, xv.ctypes.data, ay.ctypes.data)
+    a64 = np.linalg.norm(A.astype(np.float64))
+    x64 = np.linalg.norm(X.astype(np.float64))
+    return float(np.linalg.norm(ay - bv) / (a64 * x64 * np.linalg.norm(v)))
+
+
+def run_extras(args, torch, M, lib, dev, g, A, B, C, extra):
+    """DTRSM, DSYRK and SGEMM at 8192; the precision workloads (SGEMM 16384
+    native vs BF16x3 split, STRSM 8192 x 1024 plain vs FP64-refined)."""
+    n = args.n
+    errs = extra["errors_vs_fp64"]
+    steps = max(3, args.steps // 2)
+    flops = 2.0 * n ** 3
+    # DTRSM L/L/N/NonUnit m=8192 nrhs=4096 (off-diag N(0,1)/sqrt(m), diag 1+|N|)
+    m, nrhs = n, n // 2
+    At = torch.randn((m, m), dtype=torch.float64, device=dev, generator=g) / m ** 0.5
+    d = 1.0 + torch.randn(m, dtype=torch.float64, device=dev, generator=g).abs()
+    At = torch.triu(At, 1) + torch.diag(d)
+    B0 = torch.randn((nrhs, m), dtype=torch.float64, device=dev, generator=g)
+    Bt = B0.clone()
+
+    def dtrsm():
+        Bt.copy_(B0)
+        lib.cblas_dtrsm(COL, LEFT, LO, NOTR, NONUNIT, m, nrhs, 1.0, M.ptr(At), m,
+                        M.ptr(Bt), m)
+
+    def copy_only():
+        Bt.copy_(B0)
+    w_t, _ = timed_calls(torch, lib, dtrsm, steps, 1)
+    w_c, _ = timed_calls(torch, lib, copy_only, steps, 1)
+    t_trsm = max(w_t - w_c, 1e-6)
+    extra["dtrsm_8192x4096_tflops"] = round(m * m * nrhs / (t_trsm * 1e-3) / 1e12, 3)
+    extra["dtrsm_pct_fp64_peak"] = round(100 * extra["dtrsm_8192x4096_tflops"] /
+                                         PEAK["f64"], 2)
+    dtrsm()  # the copy-only timing above restored B; solve once more to check it
+    torch.cuda.synchronize()
+    errs["dtrsm_8192x4096_residual"] = trsm_residual(torch, At, Bt, B0)
+    del At, B0, Bt
+    # DSYRK lower n=8192 k=2048
+    k = n // 4
+    As = torch.randn((k, n), dtype=torch.float64, device=dev, generator=g)  # A: n x k
+
+    def dsyrk():
+        lib.cblas_dsyrk(COL, LO, NOTR, n, k, 1.0, M.ptr(As), n, 0.0, M.ptr(C), n)
+    w_s, _ = timed_calls(torch, lib, dsyrk, steps, 1)
+    extra["dsyrk_8192_2048_tflops"] = round(n * (n + 1) * k / (w_s * 1e-3) / 1e12, 3)
+    extra["dsyrk_pct_fp64_peak"] = round(100 * extra["dsyrk_8192_2048_tflops"] /
+                                         PEAK["f64"], 2)
+    del As
+    # native FP32 SGEMM 8192 (op(A)=A^T, same shape)
+    Af, Bf = A.float(), B.float()
+    Cf = torch.zeros((n, n), dtype=torch.float32, device=dev)
+
+    def sgemm():
+        lib.cblas_sgemm(COL, TR, NOTR, n, n, n, 1.0, M.ptr(Af), n, M.ptr(Bf), n, 0.0,
+                        M.ptr(Cf), n)
+    w_f, _ = timed_calls(torch, lib, sgemm, steps, 1)
+    extra["sgemm_8192_tflops"] = round(flops / (w_f * 1e-3) / 1e12, 3)
+    extra["sgemm_pct_fp32_peak"] = round(100 * extra["sgemm_8192_tflops"] / PEAK["f32"], 2)
+    errs["sgemm_8192"] = sampled_gemm_error(torch, Af, Bf, Cf)
+    del Af, Bf, Cf
+    if args.precision_n:
+        run_precision(args, torch, M, lib, dev, g, extra)
+
+
+def run_precision(args, torch, M, lib, dev, g, extra):
+    errs = extra["errors_vs_fp64"]
+    n = args.precision_n
+    steps = 3
+    # rows of op(A) and of B scaled by 10^U(-3,3); op(A) = A^T so the tensor
+    # rows are the rows of op(A)
+    At = torch.randn((n, n), device=dev, generator=g) * \
+        10.0 ** (torch.rand((n, 1), device=dev, generator=g) * 6 - 3)
+    Bt = torch.randn((n, n), device=dev, generator=g) * \
+        10.0 ** (torch.rand((1, n), device=dev, generator=g) * 6 - 3)
+    Ct = torch.empty((n, n), device=dev)
+    f = lib.mublis_sgemm_bf16x3
+    f.argtypes = [ctypes.c_int] * 5 + [ctypes.c_float, ctypes.c_void_p, ctypes.c_int,
+                                       ctypes.c_int, ctypes.c_void_p, ctypes.c_int,
+                                       ctypes.c_int, ctypes.c_float, ctypes.c_void_p,
+                                       ctypes.c_int, ctypes.c_int]
+    f.restype = ctypes.c_int
+
+    def native():
+        lib.cblas_sgemm(COL, TR, NOTR, n, n, n, 1.0, M.ptr(At), n, M.ptr(Bt), n, 0.0,
+                        M.ptr(Ct), n)
+
+    def split():
+        f(1, 0, n, n, n, 1.0, M.ptr(At), 1, n, M.ptr(Bt), 1, n, 0.0, M.ptr(Ct), 1, n)
+    fl = 2.0 * n ** 3
+    w_n, _ = timed_calls(torch, lib, native, steps, 1)
+    errs[f"sgemm_{n}_native_rowscaled"] = sampled_gemm_error(torch, At, Bt, Ct, 128)
+    w_s, _ = timed_calls(torch, lib, split, steps, 1)
+    errs[f"sgemm_{n}_bf16x3_rowscaled"] = sampled_gemm_error(torch, At, Bt, Ct, 128)
+    extra[f"sgemm_{n}_native_tflops"] = round(fl / (w_n * 1e-3) / 1e12, 3)
+    extra[f"sgemm_{n}_bf16x3_tflops"] = round(fl / (w_s * 1e-3) / 1e12, 3)
+    extra[f"sgemm_{n}_bf16x3_bf16_mfma_util_pct"] = round(
+        100 * 6 * fl / (w_s * 1e-3) / 2516.6e12, 2)
+    del At, Bt, Ct
+    torch.cuda.empty_cache()
+    # STRSM L/L m=8192 nrhs=1024: diag U[1,2], off-diag N(0,1)/sqrt(m), 2 sweeps
+    m, nrhs = 8192, 1024
+    At = torch.randn((m, m), device=dev, generator=g) / m ** 0.5
+    At = torch.triu(At, 1) + torch.diag(1 + torch.rand(m, device=dev, generator=g))
+    B0 = torch.randn((nrhs, m), device=dev, generator=g)
+    Xp, Xr = B0.clone(), B0.clone()
+    r = lib.mublis_strsm_refined
+    r.argtypes = [ctypes.c_int] * 6 + [ctypes.c_float, ctypes.c_void_p, ctypes.c_int,
+                                       ctypes.c_int, ctypes.c_void_p, ctypes.c_int,
+                                       ctypes.c_int, ctypes.c_int]
+    r.restype = ctypes.c_int
+
+    def plain():
+        Xp.copy_(B0)
+        lib.cblas_strsm(COL, LEFT, LO, NOTR, NONUNIT, m, nrhs, 1.0, M.ptr(At), m,
+                        M.ptr(Xp), m)
+
+    def refined():
+        Xr.copy_(B0)
+        r(0, 1, 0, 0, m, nrhs, 1.0, M.ptr(At), 1, m, M.ptr(Xr), 1, m, 2)
+    w_p, _ = timed_calls(torch, lib, plain, steps, 1)
+    w_r, _ = timed_calls(torch, lib, refined, steps, 1)
+    fl = float(m) * m * nrhs
+    extra["strsm_8192x1024_tflops"] = round(fl / (w_p * 1e-3) / 1e12, 3)
+    extra["strsm_8192x1024_refined2_tflops"] = round(fl / (w_r * 1e-3) / 1e12, 3)
+    Ad, Xd = At.double(), B0.double()
+    lib.cblas_dtrsm(COL, LEFT, LO, NOTR, NONUNIT, m, nrhs, 1.0, M.ptr(Ad), m, M.ptr(Xd), m)
+    torch.cuda.synchronize()
+    errs["strsm_8192x1024_plain_fwd_vs_dtrsm"] = ((Xp.double() - Xd).norm() / Xd.norm()).item()
+    errs["strsm_8192x1024_refined_fwd_vs_dtrsm"] = ((Xr.double() - Xd).norm() /
+                                                    Xd.norm()).item()
+    errs["strsm_8192x1024_refined_residual"] = trsm_residual(torch, At, Xr, B0)
+    errs["dtrsm_8192x1024_residual"] = trsm_residual(torch, Ad, Xd, B0.double())
 
 
 def main():
@@ -172,54 +353,11 @@ def main():
     achieved = flops / (ev * 1e-3) / 1e12
     assert torch.isfinite(C[:64, :64]).all().item()
 
-    extra = {}
+    extra = {"errors_vs_fp64": {}}
+    errs = extra["errors_vs_fp64"]
+    errs["dgemm_8192"] = sampled_gemm_error(torch, A, B, C)
     if not args.no_extra:
-        # DTRSM L/L/N/NonUnit m=8192 nrhs=4096 (off-diag N(0,1)/sqrt(m), diag 1+|N|)
-        m, nrhs = n, n // 2
-        At = torch.randn((m, m), dtype=torch.float64, device=dev, generator=g) / m ** 0.5
-        d = 1.0 + torch.randn(m, dtype=torch.float64, device=dev, generator=g).abs()
-        At = torch.triu(At, 1) + torch.diag(d)
-        B0 = torch.randn((nrhs, m), dtype=torch.float64, device=dev, generator=g)
-        Bt = B0.clone()
-
-        def dtrsm():
-            Bt.copy_(B0)
-            lib.cblas_dtrsm(COL, LEFT, LO, NOTR, NONUNIT, m, nrhs, 1.0, M.ptr(At), m,
-                            M.ptr(Bt), m)
-        # time the copy separately and subtract it
-        def copy_only():
-            Bt.copy_(B0)
-        steps = max(3, args.steps // 2)
-        w_t, _ = timed_calls(torch, lib, dtrsm, steps, 1)
-        w_c, _ = timed_calls(torch, lib, copy_only, steps, 1)
-        t_trsm = max(w_t - w_c, 1e-6)
-        extra["dtrsm_8192x4096_tflops"] = round(m * m * nrhs / (t_trsm * 1e-3) / 1e12, 3)
-        extra["dtrsm_pct_fp64_peak"] = round(100 * extra["dtrsm_8192x4096_tflops"] /
-                                             PEAK["f64"], 2)
-        del At, B0, Bt
-        # DSYRK lower n=8192 k=2048
-        k = n // 4
-        As = torch.randn((k, n), dtype=torch.float64, device=dev, generator=g)
-
-        def dsyrk():
-            lib.cblas_dsyrk(COL, LO, NOTR, n, k, 1.0, M.ptr(As), n, 0.0, pc, n)
-        w_s, _ = timed_calls(torch, lib, dsyrk, steps, 1)
-        extra["dsyrk_8192_2048_tflops"] = round(n * (n + 1) * k / (w_s * 1e-3) / 1e12, 3)
-        extra["dsyrk_pct_fp64_peak"] = round(100 * extra["dsyrk_8192_2048_tflops"] /
-                                             PEAK["f64"], 2)
-        del As
-        # native FP32 SGEMM 8192 (op(A)=A^T, same shape)
-        Af, Bf = A.float(), B.float()
-        Cf = torch.zeros((n, n), dtype=torch.float32, device=dev)
-
-        def sgemm():
-            lib.cblas_sgemm(COL, TR, NOTR, n, n, n, 1.0, M.ptr(Af), n, M.ptr(Bf), n, 0.0,
-                            M.ptr(Cf), n)
-        w_f, _ = timed_calls(torch, lib, sgemm, steps, 1)
-        extra["sgemm_8192_tflops"] = round(flops / (w_f * 1e-3) / 1e12, 3)
-        extra["sgemm_pct_fp32_peak"] = round(100 * extra["sgemm_8192_tflops"] /
-                                             PEAK["f32"], 2)
-        del Af, Bf, Cf
+        run_extras(args, torch, M, lib, dev, g, A, B, C, extra)
 
     traffic = None
     tfile = os.path.join(ROOT, "profiles", "dgemm_8192_transA_traffic.json")

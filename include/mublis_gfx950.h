@@ -17,6 +17,8 @@
 #include <stddef.h>
 #include <stdint.h>
 
+#include "mublis.h"
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -131,6 +133,63 @@ int mublis_gfx950_dfill_dev(int m, int n, double value, double *x, int64_t rs,
                             int64_t cs, void *stream);
 int mublis_gfx950_sfill_dev(int m, int n, float value, float *x, int64_t rs,
                             int64_t cs, void *stream);
+
+/* GEMM whose A (tri_a) or B (tri_b) operand is a materialised triangle:
+ * tri_a 1 means A(i,p) = 0 for p > i, 2 for p < i; tri_b 1 means B(p,j) = 0
+ * for p < j, 2 for p > j.  K slices that are entirely zero for a tile are
+ * skipped, halving the work of TRMM-shaped products (frame/l3/trmm.c:51-82
+ * expresses the same skipping through the jp/pi domain relations). */
+int mublis_gfx950_dgemm_tri_dev(int m, int n, int k, double alpha, const double *a,
+                                int64_t rs_a, int64_t cs_a, const double *b,
+                                int64_t rs_b, int64_t cs_b, double beta, double *c,
+                                int64_t rs_c, int64_t cs_c, int tri_a, int tri_b,
+                                void *stream);
+int mublis_gfx950_sgemm_tri_dev(int m, int n, int k, float alpha, const float *a,
+                                int64_t rs_a, int64_t cs_a, const float *b,
+                                int64_t rs_b, int64_t cs_b, float beta, float *c,
+                                int64_t rs_c, int64_t cs_c, int tri_a, int tri_b,
+                                void *stream);
+
+/* ---------------------------------------------- precision extensions -- */
+/*
+ * SGEMM as a 3-term BF16 split product on BF16 MFMA with FP32 accumulation
+ * (hi/mid/lo planes, six partial products).  Same contract as
+ * mublis_gfx950_sgemm_dev with ji = 0; it replaces the FP32 micro-kernel
+ * arithmetic of the GEMM path (include/mublis_instantiate.h:53-66) when the
+ * caller selects it.
+ */
+int mublis_gfx950_sgemm_bf16x3_dev(int m, int n, int k, float alpha, const float *a,
+                                   int64_t rs_a, int64_t cs_a, const float *b,
+                                   int64_t rs_b, int64_t cs_b, float beta, float *c,
+                                   int64_t rs_c, int64_t cs_c, void *stream);
+/*
+ * Left-side STRSM (same form as mublis_gfx950_strsm_dev) followed by `sweeps`
+ * rounds of iterative refinement whose residual alpha*B - op(A)*X and update
+ * X += D run in FP64.  Replaces the TRSM path (frame/l3/trsm.c:12-441) when
+ * the caller asks for refinement.
+ */
+int mublis_gfx950_strsm_refined_dev(int lower, int unit, int m, int n, float alpha,
+                                    const float *a, int64_t rs_a, int64_t cs_a,
+                                    float *b, int64_t rs_b, int64_t cs_b, int sweeps,
+                                    void *stream);
+
+/* Host-callable forms with the conventions of include/mublis.h (host or
+ * device pointers, synchronous, 0 on success).  No counterpart in the
+ * reference: they add the precision variants beside mublis_sgemm and
+ * mublis_strsm. */
+int mublis_sgemm_bf16x3(mublis_trans_t trans_a, mublis_trans_t trans_b, int m, int n,
+                        int k, float alpha, const float *a, int rs_a, int cs_a,
+                        const float *b, int rs_b, int cs_b, float beta, float *c,
+                        int rs_c, int cs_c);
+int mublis_strsm_refined(mublis_side_t side, mublis_uplo_t uplo, mublis_trans_t trans_a,
+                         mublis_diag_t diag, int m, int n, float alpha, const float *a,
+                         int rs_a, int cs_a, float *b, int rs_b, int cs_b, int sweeps);
+/* Arithmetic used by cblas_sgemm / mublis_sgemm for plain GEMM products:
+ * 0 native FP32 MFMA (default), 1 BF16x3 split.  The initial value comes
+ * from the environment variable MUBLIS_SGEMM_MODE ("native" or "bf16x3").
+ * Returns the previous mode, or -1 for an unknown mode. */
+int mublis_set_sgemm_mode(int mode);
+int mublis_get_sgemm_mode(void);
 
 #ifdef __cplusplus
 }

@@ -142,8 +142,20 @@ static const mublis_l3_product_t k_general = {
     return driver(m, n, k, alpha, a, rs_a, cs_a, b, rs_b, cs_b, beta, c, rs_c, \
                   cs_c, &k_general);                                           \
   }
-GEMM_IMPL(float, mublis_sgemm, mublis_l3_sdriver)
+GEMM_IMPL(float, mublis_sgemm_native, mublis_l3_sdriver)
 GEMM_IMPL(double, mublis_dgemm, mublis_l3_ddriver)
+
+/* mublis_sgemm: native FP32 MFMA, or the BF16x3 split product when selected
+ * with mublis_set_sgemm_mode(1) / MUBLIS_SGEMM_MODE=bf16x3. */
+int mublis_sgemm(mublis_trans_t trans_a, mublis_trans_t trans_b, int m, int n, int k,
+                 float alpha, const float *a, int rs_a, int cs_a, const float *b,
+                 int rs_b, int cs_b, float beta, float *c, int rs_c, int cs_c) {
+  if (mublis_get_sgemm_mode() == 1)
+    return mublis_sgemm_bf16x3(trans_a, trans_b, m, n, k, alpha, a, rs_a, cs_a, b, rs_b,
+                               cs_b, beta, c, rs_c, cs_c);
+  return mublis_sgemm_native(trans_a, trans_b, m, n, k, alpha, a, rs_a, cs_a, b, rs_b,
+                             cs_b, beta, c, rs_c, cs_c);
+}
 
 /* ---------------------------------------------------------------- SYMM -- */
 #define SYMM_IMPL(T, name, driver)                                              \
@@ -222,7 +234,7 @@ SYR2K_IMPL(double, mublis_dsyr2k, mublis_gfx950_dgemm_dev)
  * never read), B is copied to a dense workspace, and one GEMM writes the
  * product over B, so the in-place hazard that dictates the reference's
  * traversal order (frame/l3/trmm.c:146-160) does not arise. */
-#define TRMM_IMPL(T, name, gemm_dev, pack_dev, copy_dev, fill_dev)              \
+#define TRMM_IMPL(T, name, gemm_tri_dev, pack_dev, copy_dev, fill_dev)          \
   int name(mublis_side_t side, mublis_uplo_t uplo, mublis_trans_t trans_a,     \
            mublis_diag_t diag, int m, int n, T alpha, const T *a, int rs_a,    \
            int cs_a, T *b, int rs_b, int cs_b) {                               \
@@ -262,22 +274,25 @@ SYR2K_IMPL(double, mublis_dsyr2k, mublis_gfx950_dgemm_dev)
                     0, tri, sz, s);                                            \
     if (st == MUBLIS_OK)                                                       \
       st = copy_dev(m, n, (const T *)sb.dev, rs_b, cs_b, bw, 1, m, s);         \
+    /* K slices outside the triangle are skipped (tri codes of            \
+     * include/mublis_gfx950.h); uplo DENSE multiplies the full operand. */    \
+    const int tcode = uplo == MUBLIS_LOWER ? 1 : (uplo == MUBLIS_UPPER ? 2 : 0); \
     if (st == MUBLIS_OK) {                                                     \
       if (side == MUBLIS_LEFT)                                                 \
-        st = gemm_dev(m, n, m, alpha, tri, 1, sz, bw, 1, m, NULL, 0, 0, NULL,  \
-                      0, 0, (T)0, (T *)sb.dev, rs_b, cs_b, 0, s);              \
+        st = gemm_tri_dev(m, n, m, alpha, tri, 1, sz, bw, 1, m, (T)0,          \
+                          (T *)sb.dev, rs_b, cs_b, tcode, 0, s);               \
       else                                                                     \
-        st = gemm_dev(m, n, n, alpha, bw, 1, m, tri, 1, sz, NULL, 0, 0, NULL,  \
-                      0, 0, (T)0, (T *)sb.dev, rs_b, cs_b, 0, s);              \
+        st = gemm_tri_dev(m, n, n, alpha, bw, 1, m, tri, 1, sz, (T)0,          \
+                          (T *)sb.dev, rs_b, cs_b, 0, tcode, s);               \
     }                                                                          \
     st = finish(st, s, &sb, tmp, 1);                                           \
     mublis_rt_free(tri);                                                       \
     mublis_rt_free(bw);                                                        \
     return st;                                                                 \
   }
-TRMM_IMPL(float, mublis_strmm, mublis_gfx950_sgemm_dev, mublis_gfx950_spack_dev,
+TRMM_IMPL(float, mublis_strmm, mublis_gfx950_sgemm_tri_dev, mublis_gfx950_spack_dev,
           mublis_gfx950_scopy_dev, mublis_gfx950_sfill_dev)
-TRMM_IMPL(double, mublis_dtrmm, mublis_gfx950_dgemm_dev, mublis_gfx950_dpack_dev,
+TRMM_IMPL(double, mublis_dtrmm, mublis_gfx950_dgemm_tri_dev, mublis_gfx950_dpack_dev,
           mublis_gfx950_dcopy_dev, mublis_gfx950_dfill_dev)
 
 /* ---------------------------------------------------------------- TRSM -- */

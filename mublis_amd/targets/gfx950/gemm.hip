@@ -82,6 +82,11 @@ struct GemmArgs {
   int tri;         // grid enumerates only tiles touching the ji triangle
   int ntiles;
   int vec_a, vec_b, vec_a2, vec_b2;  // 16-byte loads are aligned
+  // Triangular operand structure, used only to skip K slices that are zero:
+  // tri_a 1: A(i,p) = 0 for p > i, 2: for p < i; tri_b 1: B(p,j) = 0 for
+  // p < j, 2: for p > j.  Values are still read as stored (the caller has
+  // materialised the zeros), so rounding K bounds outward is harmless.
+  int tri_a, tri_b;
 };
 
 // Map a hardware workgroup id to an output tile.  Consecutive hardware ids
@@ -224,11 +229,17 @@ __global__ __launch_bounds__(NT, Tile<TILE>::MIN_WG_PER_CU) void gemm_kernel(Gem
     const int64_t ldb = seg ? (B_NC ? p.rs_b2 : p.cs_b2) : (B_NC ? p.rs_b : p.cs_b);
     const bool va = seg ? p.vec_a2 : p.vec_a;
     const bool vb = seg ? p.vec_b2 : p.vec_b;
-    const int ktiles = (p.k + BK - 1) / BK;
-    if (ktiles == 0) continue;
+    int kbeg = 0, kend = p.k;
+    if (p.tri_a == 1) kend = min(kend, m0 + BM);
+    if (p.tri_a == 2) kbeg = max(kbeg, m0);
+    if (p.tri_b == 1) kbeg = max(kbeg, n0);
+    if (p.tri_b == 2) kend = min(kend, n0 + BN);
+    const int kt0 = kbeg / BK;
+    const int ktiles = kend > kbeg ? (kend + BK - 1) / BK - kt0 : 0;
+    if (ktiles <= 0) continue;
 
-    load_slices<T, TILE, A_KC, B_NC>(ra, rb, a, lda, b, ldb, m0, n0, 0, p.m, p.n, p.k,
-                                     va, vb, tid);
+    load_slices<T, TILE, A_KC, B_NC>(ra, rb, a, lda, b, ldb, m0, n0, kt0 * BK, p.m, p.n,
+                                     p.k, va, vb, tid);
     __syncthreads();  // previous segment's readers are done with buffer 0
     ra.template store<A_KC>(As[0], tid);
     rb.template store<!B_NC>(Bs[0], tid);
@@ -239,8 +250,8 @@ __global__ __launch_bounds__(NT, Tile<TILE>::MIN_WG_PER_CU) void gemm_kernel(Gem
       const int cur = kt & 1;
       const bool more = kt + 1 < ktiles;
       if (more)
-        load_slices<T, TILE, A_KC, B_NC>(ra, rb, a, lda, b, ldb, m0, n0, (kt + 1) * BK,
-                                         p.m, p.n, p.k, va, vb, tid);
+        load_slices<T, TILE, A_KC, B_NC>(ra, rb, a, lda, b, ldb, m0, n0,
+                                         (kt0 + kt + 1) * BK, p.m, p.n, p.k, va, vb, tid);
       const T *as = As[cur];
       const T *bs = Bs[cur];
 #pragma unroll
@@ -332,7 +343,8 @@ template <typename T>
 int gemm_dev(int m, int n, int k, T alpha, const T *a, int64_t rs_a, int64_t cs_a,
              const T *b, int64_t rs_b, int64_t cs_b, const T *a2, int64_t rs_a2,
              int64_t cs_a2, const T *b2, int64_t rs_b2, int64_t cs_b2, T beta,
-             T *c, int64_t rs_c, int64_t cs_c, int ji, void *stream) {
+             T *c, int64_t rs_c, int64_t cs_c, int ji, void *stream, int tri_a = 0,
+             int tri_b = 0) {
   if (m < 0 || n < 0 || k < 0) return MUBLIS_EINVAL;
   if (m == 0 || n == 0) return MUBLIS_OK;
   hipStream_t s = (hipStream_t)(stream ? stream : mublis_rt_stream());
@@ -383,6 +395,8 @@ int gemm_dev(int m, int n, int k, T alpha, const T *a, int64_t rs_a, int64_t cs_
     p.b2 = same2 ? b2 : nullptr; p.rs_b2 = rs_b2; p.cs_b2 = cs_b2;
     p.c = c; p.rs_c = rs_c; p.cs_c = cs_c; p.ji = ji;
     p.tri = (ji != 0 && m == n) ? 1 : 0;
+    p.tri_a = a2 ? 0 : tri_a;
+    p.tri_b = a2 ? 0 : tri_b;
     p.vec_a = aligned16(a, akc ? rs_a : cs_a);
     p.vec_b = aligned16(b, bnc ? rs_b : cs_b);
     p.vec_a2 = a2 ? aligned16(a2, akc ? rs_a2 : cs_a2) : 0;
@@ -456,3 +470,22 @@ template int gfx950_gemm<float>(int, int, int, float, const float *, int64_t, in
 template int gfx950_gemm<double>(int, int, int, double, const double *, int64_t,
                                  int64_t, const double *, int64_t, int64_t, double,
                                  double *, int64_t, int64_t, hipStream_t);
+
+extern "C" int mublis_gfx950_dgemm_tri_dev(int m, int n, int k, double alpha,
+                                           const double *a, int64_t rs_a, int64_t cs_a,
+                                           const double *b, int64_t rs_b, int64_t cs_b,
+                                           double beta, double *c, int64_t rs_c,
+                                           int64_t cs_c, int tri_a, int tri_b,
+                                           void *stream) {
+  return gemm_dev<double>(m, n, k, alpha, a, rs_a, cs_a, b, rs_b, cs_b, nullptr, 0, 0,
+                          nullptr, 0, 0, beta, c, rs_c, cs_c, 0, stream, tri_a, tri_b);
+}
+extern "C" int mublis_gfx950_sgemm_tri_dev(int m, int n, int k, float alpha,
+                                           const float *a, int64_t rs_a, int64_t cs_a,
+                                           const float *b, int64_t rs_b, int64_t cs_b,
+                                           float beta, float *c, int64_t rs_c,
+                                           int64_t cs_c, int tri_a, int tri_b,
+                                           void *stream) {
+  return gemm_dev<float>(m, n, k, alpha, a, rs_a, cs_a, b, rs_b, cs_b, nullptr, 0, 0,
+                         nullptr, 0, 0, beta, c, rs_c, cs_c, 0, stream, tri_a, tri_b);
+}

@@ -1,0 +1,293 @@
+// SGEMM as a 3-term BF16 split product on BF16 MFMA with FP32 accumulation.
+//
+// Every FP32 operand element is split exactly as x = hi + mid + lo with hi,
+// mid and lo BF16 (8 significant bits each, round to nearest even; the
+// remainders x - hi and x - hi - mid are exact in FP32).  Then
+//   A*B ~= Ah*Bh + (Ah*Bm + Am*Bh) + (Ah*Bl + Al*Bh + Am*Bm)
+// keeps every partial product down to 2^-16 relative; the three dropped
+// terms are below FP32 rounding.  Products of two BF16 values are exact in
+// FP32.  Ah*Bh accumulates in its own FP32 accumulator and the five smaller
+// products in a second one (2^8 smaller, so its rounding is negligible), so
+// the result rounds like a native FP32 GEMM.
+//
+// Layout: a split pass writes each operand as three K-contiguous BF16 planes
+// ([row][kp], rows padded to the 128 tile, K to 32, zero filled), so the GEMM
+// kernel streams aligned 16-byte vectors with no bounds checks.  The GEMM
+// kernel: 128 x 128 output tile per workgroup, 4 waves of 64 x 64, each a
+// 2 x 2 grid of v_mfma_f32_32x32x16_bf16 tiles (6 MFMAs per tile per k=16),
+// LDS double-buffered, the XCD-aware tile order of the FP64/FP32 kernel.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <algorithm>
+
+#include "gfx950_common.h"
+#include "mublis_gfx950.h"
+
+namespace {
+
+typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
+typedef float f16v __attribute__((ext_vector_type(16)));
+typedef unsigned short u16;
+typedef u16 u16x8 __attribute__((ext_vector_type(8)));
+
+constexpr int XT = 128;    // output tile
+constexpr int XBK = 16;    // k per stage (one MFMA k step)
+constexpr int XLD = 24;    // LDS row stride in BF16 (48 B: conflict-free b128 reads)
+constexpr int XNT = 256;
+
+__device__ inline u16 bf16_rne(float x) {
+  const uint32_t u = __float_as_uint(x);
+  const uint32_t bias = 0x7FFFu + ((u >> 16) & 1u);
+  return (u16)((u + bias) >> 16);
+}
+__device__ inline float bf16_to_f(u16 h) { return __uint_as_float((uint32_t)h << 16); }
+
+// Split an m x k FP32 view (strides rs, cs) into three K-contiguous BF16
+// planes of mp x kp.  A 64 x 64 tile goes through LDS so that reads run
+// along the source's unit stride and writes along k.
+__global__ __launch_bounds__(256) void split3_kernel(int m, int k, const float *__restrict__ x,
+                                                     int64_t rs, int64_t cs, int mp, int kp,
+                                                     u16 *__restrict__ hi, u16 *__restrict__ mid,
+                                                     u16 *__restrict__ lo) {
+  __shared__ float t[64][65];
+  const int r0 = blockIdx.y * 64, k0 = blockIdx.x * 64;
+  const int tid = threadIdx.x;
+  const bool rows_contig = rs == 1;
+#pragma unroll
+  for (int it = 0; it < 16; ++it) {
+    const int lin = it * 256 + tid;
+    const int a = lin & 63, b = lin >> 6;  // a runs along the unit stride
+    const int r = rows_contig ? a : b, c = rows_contig ? b : a;
+    float v = 0.0f;
+    if (r0 + r < m && k0 + c < k) v = x[(int64_t)(r0 + r) * rs + (int64_t)(k0 + c) * cs];
+    t[r][c] = v;
+  }
+  __syncthreads();
+  // each thread writes 16 consecutive k of one row into each plane
+  const int r = tid >> 2, c0 = (tid & 3) * 16;
+  if (r0 + r >= mp) return;
+  u16x8 vh[2], vm[2], vl[2];
+#pragma unroll
+  for (int e = 0; e < 16; ++e) {
+    const float v = t[r][c0 + e];
+    const u16 h = bf16_rne(v);
+    const float r1 = v - bf16_to_f(h);
+    const u16 md = bf16_rne(r1);
+    const float r2 = r1 - bf16_to_f(md);
+    const u16 l = bf16_rne(r2);
+    vh[e / 8][e % 8] = h;
+    vm[e / 8][e % 8] = md;
+    vl[e / 8][e % 8] = l;
+  }
+  const int64_t off = (int64_t)(r0 + r) * kp + k0 + c0;
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    if (k0 + c0 + 8 * h < kp) {
+      *reinterpret_cast<u16x8 *>(hi + off + 8 * h) = vh[h];
+      *reinterpret_cast<u16x8 *>(mid + off + 8 * h) = vm[h];
+      *reinterpret_cast<u16x8 *>(lo + off + 8 * h) = vl[h];
+    }
+  }
+}
+
+struct XArgs {
+  int m, n, kp, mp, np;
+  float alpha, beta;
+  const u16 *a[3];  // [mp][kp]
+  const u16 *b[3];  // [np][kp]
+  float *c;
+  int64_t rs_c, cs_c;
+  int tiles_m, tiles_n, ntiles;
+};
+
+__device__ inline void map_tile_x(int w, int ntiles, int tiles_m, int tiles_n, int &tm,
+                                  int &tn) {
+  const int q = ntiles / 8, r = ntiles % 8;
+  const int xcd = w % 8, local = w / 8;
+  const int id = (xcd < r ? xcd * (q + 1) : r * (q + 1) + (xcd - r) * q) + local;
+  const int G = 8, per = G * tiles_n, g = id / per, first = g * G;
+  const int gs = min(tiles_m - first, G), in = id - g * per;
+  tm = first + in % gs;
+  tn = in / gs;
+}
+
+__global__ __launch_bounds__(XNT, 2) void gemm_bf16x3_kernel(XArgs p) {
+  __shared__ __attribute__((aligned(16))) u16 As[2][3][XT * XLD];
+  __shared__ __attribute__((aligned(16))) u16 Bs[2][3][XT * XLD];
+  int tm, tn;
+  map_tile_x(blockIdx.x, p.ntiles, p.tiles_m, p.tiles_n, tm, tn);
+  const int m0 = tm * XT, n0 = tn * XT;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = (wave & 1) * 64, wn = (wave >> 1) * 64;
+  // loader: thread -> (row, 8-element half of the 16-k stage)
+  const int lrow = tid >> 1, lhalf = (tid & 1) * 8;
+
+  f16v big[2][2], small[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      big[i][j] = f16v{};
+      small[i][j] = f16v{};
+    }
+
+  // Two register sets: the loads of stage st + 2 are issued while stage st
+  // is computed, and land in LDS only at the end of iteration st + 1, so
+  // every load has two compute phases to arrive.
+  u16x8 ra[2][3], rb[2][3];
+  const int64_t aoff = (int64_t)(m0 + lrow) * p.kp + lhalf;
+  const int64_t boff = (int64_t)(n0 + lrow) * p.kp + lhalf;
+  const int stages = p.kp / XBK;
+  // Loads are issued unconditionally (a stage past the end re-reads the last
+  // one, in bounds and never used): conditional loads would make the
+  // compiler's wait counts conservative and serialise the pipeline.
+#pragma unroll
+  for (int t = 0; t < 2; ++t) {
+    const int64_t k0 = (int64_t)min(t, stages - 1) * XBK;
+#pragma unroll
+    for (int s = 0; s < 3; ++s) {
+      ra[t][s] = *reinterpret_cast<const u16x8 *>(p.a[s] + aoff + k0);
+      rb[t][s] = *reinterpret_cast<const u16x8 *>(p.b[s] + boff + k0);
+    }
+  }
+#pragma unroll
+  for (int s = 0; s < 3; ++s) {
+    *reinterpret_cast<u16x8 *>(&As[0][s][lrow * XLD + lhalf]) = ra[0][s];
+    *reinterpret_cast<u16x8 *>(&Bs[0][s][lrow * XLD + lhalf]) = rb[0][s];
+  }
+  __syncthreads();
+
+  const int fr = lane & 31, fk = (lane >> 5) * 8;
+  // kp is a multiple of 2 * XBK, so stages come in pairs and no branch
+  // interrupts the pipeline (branches make the compiler's wait counts
+  // conservative).  The store after the last stage writes a slot that is
+  // never read again.
+#pragma unroll 1
+  for (int st = 0; st < stages; st += 2) {
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int cur_st = st + h;
+      const int cur = h;  // LDS slot and register set of stage cur_st
+      {
+        const int64_t k0 = (int64_t)min(cur_st + 2, stages - 1) * XBK;
+#pragma unroll
+        for (int s = 0; s < 3; ++s) {
+          ra[h][s] = *reinterpret_cast<const u16x8 *>(p.a[s] + aoff + k0);
+          rb[h][s] = *reinterpret_cast<const u16x8 *>(p.b[s] + boff + k0);
+        }
+      }
+      // Keep the order loads -> MFMAs -> LDS stores: the scheduler would
+      // otherwise hoist the stores and sink the loads, shortening the time
+      // each load has to arrive.
+      __builtin_amdgcn_sched_barrier(0);
+      bf8 af[3][2], bf[3][2];
+#pragma unroll
+      for (int s = 0; s < 3; ++s)
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+          af[s][i] = __builtin_bit_cast(
+              bf8, *reinterpret_cast<const u16x8 *>(&As[cur][s][(wm + i * 32 + fr) * XLD + fk]));
+          bf[s][i] = __builtin_bit_cast(
+              bf8, *reinterpret_cast<const u16x8 *>(&Bs[cur][s][(wn + i * 32 + fr) * XLD + fk]));
+        }
+      // Six products per tile, issued product-major so that consecutive
+      // MFMAs update independent accumulators: (mid, mid), (lo, hi),
+      // (hi, lo), (mid, hi), (hi, mid) into `small`, then (hi, hi) into `big`.
+#pragma unroll
+      for (int t = 0; t < 6; ++t) {
+        const int pa = t == 0 ? 1 : t == 1 ? 2 : t == 2 ? 0 : t == 3 ? 1 : 0;
+        const int pb = t == 0 ? 1 : t == 1 ? 0 : t == 2 ? 2 : t == 3 ? 0 : t == 4 ? 1 : 0;
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+          for (int j = 0; j < 2; ++j) {
+            if (t < 5)
+              small[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(af[pa][i], bf[pb][j],
+                                                                    small[i][j], 0, 0, 0);
+            else
+              big[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(af[0][i], bf[0][j],
+                                                                  big[i][j], 0, 0, 0);
+          }
+      }
+      __builtin_amdgcn_sched_barrier(0);
+      // stage cur_st + 1 (loaded one iteration earlier) -> the other slot
+#pragma unroll
+      for (int s = 0; s < 3; ++s) {
+        *reinterpret_cast<u16x8 *>(&As[cur ^ 1][s][lrow * XLD + lhalf]) = ra[h ^ 1][s];
+        *reinterpret_cast<u16x8 *>(&Bs[cur ^ 1][s][lrow * XLD + lhalf]) = rb[h ^ 1][s];
+      }
+      __syncthreads();
+    }
+  }
+
+  // 32 x 32 MFMA output: lane l, register r holds column l%32 and row
+  // 8*(r/4) + 4*(l/32) + r%4 of its tile (measured on gfx950).
+  const float alpha = p.alpha, beta = p.beta;
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      const int col = n0 + wn + j * 32 + (lane & 31);
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const int row = m0 + wm + i * 32 + 8 * (r / 4) + 4 * (lane >> 5) + (r % 4);
+        if (row >= p.m || col >= p.n) continue;
+        float *cp = p.c + (int64_t)row * p.rs_c + (int64_t)col * p.cs_c;
+        const float prod = alpha * (big[i][j][r] + small[i][j][r]);
+        if (beta == 0.0f) *cp = prod;
+        else if (beta == 1.0f) *cp = *cp + prod;
+        else *cp = prod + beta * *cp;
+      }
+    }
+}
+
+int split3(int rows, int k, const float *x, int64_t rs, int64_t cs, int rp, int kp,
+           u16 *planes, hipStream_t s) {
+  const size_t plane = (size_t)rp * kp;
+  dim3 grid((kp + 63) / 64, (rp + 63) / 64);
+  hipLaunchKernelGGL(split3_kernel, grid, dim3(256), 0, s, rows, k, x, rs, cs, rp, kp,
+                     planes, planes + plane, planes + 2 * plane);
+  return hipGetLastError() == hipSuccess ? MUBLIS_OK : MUBLIS_EHIP;
+}
+
+}  // namespace
+
+extern "C" int mublis_gfx950_sgemm_bf16x3_dev(int m, int n, int k, float alpha,
+                                              const float *a, int64_t rs_a, int64_t cs_a,
+                                              const float *b, int64_t rs_b, int64_t cs_b,
+                                              float beta, float *c, int64_t rs_c,
+                                              int64_t cs_c, void *stream) {
+  if (m < 0 || n < 0 || k < 0) return MUBLIS_EINVAL;
+  if (m == 0 || n == 0) return MUBLIS_OK;
+  hipStream_t s = (hipStream_t)(stream ? stream : mublis_rt_stream());
+  if (alpha == 0.0f || k == 0)
+    return mublis_gfx950_sgemm_dev(m, n, 0, 0.0f, a, rs_a, cs_a, b, rs_b, cs_b, nullptr, 0,
+                                   0, nullptr, 0, 0, beta, c, rs_c, cs_c, 0, s);
+  const int mp = (m + XT - 1) / XT * XT, np = (n + XT - 1) / XT * XT;
+  const int kp = (k + 2 * XBK - 1) / (2 * XBK) * (2 * XBK);  // whole stage pairs
+  u16 *pa = (u16 *)mublis_rt_alloc(sizeof(u16) * 3 * (size_t)mp * kp);
+  u16 *pb = (u16 *)mublis_rt_alloc(sizeof(u16) * 3 * (size_t)np * kp);
+  int st = (pa && pb) ? MUBLIS_OK : MUBLIS_ENOMEM;
+  if (st == MUBLIS_OK) st = split3(m, k, a, rs_a, cs_a, mp, kp, pa, s);
+  // B is split as its transpose: row j of the plane holds B(:, j).
+  if (st == MUBLIS_OK) st = split3(n, k, b, cs_b, rs_b, np, kp, pb, s);
+  if (st == MUBLIS_OK) {
+    XArgs p;
+    p.m = m; p.n = n; p.kp = kp; p.mp = mp; p.np = np;
+    p.alpha = alpha; p.beta = beta;
+    const size_t pa_plane = (size_t)mp * kp, pb_plane = (size_t)np * kp;
+    for (int q = 0; q < 3; ++q) {
+      p.a[q] = pa + q * pa_plane;
+      p.b[q] = pb + q * pb_plane;
+    }
+    p.c = c; p.rs_c = rs_c; p.cs_c = cs_c;
+    p.tiles_m = mp / XT; p.tiles_n = np / XT; p.ntiles = p.tiles_m * p.tiles_n;
+    hipLaunchKernelGGL(gemm_bf16x3_kernel, dim3(p.ntiles), dim3(XNT), 0, s, p);
+    if (hipGetLastError() != hipSuccess) st = MUBLIS_EHIP;
+  }
+  if (hipStreamSynchronize(s) != hipSuccess && st == MUBLIS_OK) st = MUBLIS_EHIP;
+  mublis_rt_free(pa);
+  mublis_rt_free(pb);
+  return st;
+}

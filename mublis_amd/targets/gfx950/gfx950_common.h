@@ -29,4 +29,15 @@ int gfx950_gemm(int m, int n, int k, T alpha, const T *a, int64_t rs_a, int64_t 
                 const T *b, int64_t rs_b, int64_t cs_b, T beta, T *c, int64_t rs_c,
                 int64_t cs_c, hipStream_t s);
 
+// Precision helpers (refined STRSM): FP32 <-> FP64 copies, FP32 triangular
+// view materialised in FP64, and x64 += d; out := (float) x64.
+int gfx950_convert_f2d(int m, int n, const float *src, int64_t rs_s, int64_t cs_s,
+                       double *dst, int64_t rs_d, int64_t cs_d, hipStream_t s);
+int gfx950_convert_d2f(int m, int n, const double *src, int64_t rs_s, int64_t cs_s,
+                       float *dst, int64_t rs_d, int64_t cs_d, hipStream_t s);
+int gfx950_pack_f2d(int m, int n, const float *src, int64_t rs, int64_t cs, int struc,
+                    int uplo, int diag, double *dst, int64_t ld, hipStream_t s);
+int gfx950_refine_update(int m, int n, const float *d, double *x64, float *out,
+                         int64_t rs_o, int64_t cs_o, hipStream_t s);
+
 #endif

@@ -25,42 +25,56 @@ __device__ inline bool stored(int uplo, int64_t delta) {
   return uplo == 1 ? delta >= 0 : (uplo == 2 ? delta <= 0 : true);
 }
 
-template <typename T>
+template <typename T, typename D = T>
 __global__ void pack_kernel(int m, int n, const T *__restrict__ src, int64_t rs,
                             int64_t cs, int struc, int uplo, int diag, int diagoff,
-                            T *__restrict__ dst, int64_t ld) {
+                            D *__restrict__ dst, int64_t ld) {
   const int64_t total = (int64_t)m * n;
   for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
        t += (int64_t)gridDim.x * blockDim.x) {
     const int64_t i = t % m, j = t / m;
     const int64_t d = diagoff + i - j;
-    T v;
+    D v;
     if (struc == 1) {  // SYMMETRIC: reflect when outside the stored triangle
-      v = stored(uplo, d) ? src[i * rs + j * cs]
-                          : src[(j - diagoff) * rs + (i + diagoff) * cs];
+      v = (D)(stored(uplo, d) ? src[i * rs + j * cs]
+                              : src[(j - diagoff) * rs + (i + diagoff) * cs]);
     } else if (struc == 2) {  // TRIANGULAR
-      if (!stored(uplo, d)) v = T(0);
-      else if (d == 0 && diag == 1) v = T(1);
+      if (!stored(uplo, d)) v = D(0);
+      else if (d == 0 && diag == 1) v = D(1);
       else {
-        v = src[i * rs + j * cs];
-        if (d == 0 && diag == 2) v = T(1) / v;
+        v = (D)src[i * rs + j * cs];
+        if (d == 0 && diag == 2) v = D(1) / v;
       }
     } else {
-      v = src[i * rs + j * cs];
+      v = (D)src[i * rs + j * cs];
     }
     dst[i + j * ld] = v;
   }
 }
 
-template <typename T>
+template <typename T, typename D = T>
 __global__ void copy_kernel(int m, int n, const T *__restrict__ src, int64_t rs_s,
-                            int64_t cs_s, T *__restrict__ dst, int64_t rs_d,
+                            int64_t cs_s, D *__restrict__ dst, int64_t rs_d,
                             int64_t cs_d) {
   const int64_t total = (int64_t)m * n;
   for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
        t += (int64_t)gridDim.x * blockDim.x) {
     const int64_t i = t % m, j = t / m;
-    dst[i * rs_d + j * cs_d] = src[i * rs_s + j * cs_s];
+    dst[i * rs_d + j * cs_d] = (D)src[i * rs_s + j * cs_s];
+  }
+}
+
+// Refinement update: x64 += d (dense m x n), then out := (float) x64.
+__global__ void refine_update_kernel(int m, int n, const float *__restrict__ d,
+                                     double *__restrict__ x64, float *__restrict__ out,
+                                     int64_t rs_o, int64_t cs_o) {
+  const int64_t total = (int64_t)m * n;
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t i = t % m, j = t / m;
+    const double v = x64[t] + (double)d[t];
+    x64[t] = v;
+    out[i * rs_o + j * cs_o] = (float)v;
   }
 }
 
@@ -99,6 +113,35 @@ int gfx950_fill(int m, int n, T value, T *x, int64_t rs, int64_t cs, hipStream_t
   if (m <= 0 || n <= 0) return MUBLIS_OK;
   hipLaunchKernelGGL(fill_kernel<T>, dim3(grid_for((int64_t)m * n)), dim3(256), 0, s,
                      m, n, value, x, rs, cs);
+  return hipGetLastError() == hipSuccess ? MUBLIS_OK : MUBLIS_EHIP;
+}
+
+int gfx950_convert_f2d(int m, int n, const float *src, int64_t rs_s, int64_t cs_s,
+                       double *dst, int64_t rs_d, int64_t cs_d, hipStream_t s) {
+  if (m <= 0 || n <= 0) return MUBLIS_OK;
+  hipLaunchKernelGGL((copy_kernel<float, double>), dim3(grid_for((int64_t)m * n)),
+                     dim3(256), 0, s, m, n, src, rs_s, cs_s, dst, rs_d, cs_d);
+  return hipGetLastError() == hipSuccess ? MUBLIS_OK : MUBLIS_EHIP;
+}
+int gfx950_convert_d2f(int m, int n, const double *src, int64_t rs_s, int64_t cs_s,
+                       float *dst, int64_t rs_d, int64_t cs_d, hipStream_t s) {
+  if (m <= 0 || n <= 0) return MUBLIS_OK;
+  hipLaunchKernelGGL((copy_kernel<double, float>), dim3(grid_for((int64_t)m * n)),
+                     dim3(256), 0, s, m, n, src, rs_s, cs_s, dst, rs_d, cs_d);
+  return hipGetLastError() == hipSuccess ? MUBLIS_OK : MUBLIS_EHIP;
+}
+int gfx950_pack_f2d(int m, int n, const float *src, int64_t rs, int64_t cs, int struc,
+                    int uplo, int diag, double *dst, int64_t ld, hipStream_t s) {
+  if (m <= 0 || n <= 0) return MUBLIS_OK;
+  hipLaunchKernelGGL((pack_kernel<float, double>), dim3(grid_for((int64_t)m * n)),
+                     dim3(256), 0, s, m, n, src, rs, cs, struc, uplo, diag, 0, dst, ld);
+  return hipGetLastError() == hipSuccess ? MUBLIS_OK : MUBLIS_EHIP;
+}
+int gfx950_refine_update(int m, int n, const float *d, double *x64, float *out,
+                         int64_t rs_o, int64_t cs_o, hipStream_t s) {
+  if (m <= 0 || n <= 0) return MUBLIS_OK;
+  hipLaunchKernelGGL(refine_update_kernel, dim3(grid_for((int64_t)m * n)), dim3(256), 0, s,
+                     m, n, d, x64, out, rs_o, cs_o);
   return hipGetLastError() == hipSuccess ? MUBLIS_OK : MUBLIS_EHIP;
 }
 

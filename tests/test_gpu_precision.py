@@ -1,0 +1,224 @@
+"""GPU checks of the precision path: BF16x3 split SGEMM and FP64-refined
+STRSM, through the C ABI.
+
+Bars:
+  * BF16x3 SGEMM: the FP32 normwise bound of the native path,
+    ||C - C*||_F <= 2 k eps32 (|alpha| ||A|| ||B|| + |beta| ||C0||), against the
+    oracle (FP32 data, double accumulation); and its error against FP64 no
+    worse than 1.5x the native FP32-MFMA path's error on the same inputs
+    (the split's target is "no worse than native"; 1.5x absorbs the
+    run-to-run spread of two independent roundings).
+  * Refined STRSM: forward error against an FP64 solve of the same FP32
+    inputs <= 2 eps32 (the FP32 rounding of the answer), and no larger than
+    the unrefined STRSM's.
+"""
+import ctypes
+import itertools
+
+import numpy as np
+import pytest
+from scipy.linalg import solve_triangular
+
+import l3cases as L
+import mublis_amd as M
+
+pytestmark = pytest.mark.gpu
+EPS32 = float(np.finfo(np.float32).eps)
+VP, I, F = ctypes.c_void_p, ctypes.c_int, ctypes.c_float
+
+
+@pytest.fixture(scope="module")
+def lib():
+    lib = M.load()
+    assert lib.mublis_rt_init() == 0, lib.mublis_rt_last_error()
+    lib.mublis_sgemm_bf16x3.argtypes = [I] * 5 + [F, VP, I, I, VP, I, I, F, VP, I, I]
+    lib.mublis_sgemm_bf16x3.restype = I
+    lib.mublis_strsm_refined.argtypes = [I] * 6 + [F, VP, I, I, VP, I, I, I]
+    lib.mublis_strsm_refined.restype = I
+    lib.mublis_set_sgemm_mode.argtypes = [I]
+    lib.mublis_set_sgemm_mode.restype = I
+    lib.mublis_get_sgemm_mode.restype = I
+    return lib
+
+
+@pytest.fixture(scope="module")
+def orc():
+    return L.Cblas(L.ORACLE_LIB, prefix="orc_", returns_status=True)
+
+
+def col(x):
+    return np.asfortranarray(x)
+
+
+def native_and_split(lib, ta, tb, m, n, k, alpha, a, b, beta, c0):
+    """Column-major SGEMM through cblas_sgemm (native) and
+    mublis_sgemm_bf16x3 (split), host arrays."""
+    lda, ldb = a.shape[0], b.shape[0]
+    nat = c0.copy(order="F")
+    lib.cblas_sgemm(102, 112 if ta else 111, 112 if tb else 111, m, n, k, alpha, M.ptr(a),
+                    lda, M.ptr(b), ldb, beta, M.ptr(nat), m)
+    spl = c0.copy(order="F")
+    st = lib.mublis_sgemm_bf16x3(int(ta), int(tb), m, n, k, alpha, M.ptr(a), 1, lda,
+                                 M.ptr(b), 1, ldb, beta, M.ptr(spl), 1, m)
+    assert st == 0
+    return nat, spl
+
+
+@pytest.mark.parametrize("ta,tb", list(itertools.product((0, 1), repeat=2)))
+def test_bf16x3_vs_oracle_and_native(lib, orc, ta, tb):
+    rng = np.random.default_rng(21)
+    m, n, k = 300, 257, 190
+    alpha, beta = 0.7, -1.3
+    a = col(rng.standard_normal((k, m) if ta else (m, k)).astype(np.float32))
+    b = col(rng.standard_normal((n, k) if tb else (k, n)).astype(np.float32))
+    c0 = col(rng.standard_normal((m, n)).astype(np.float32))
+    want = c0.copy(order="F")
+    assert orc.call(np.float32, "gemm", 102, 112 if ta else 111, 112 if tb else 111, m, n, k,
+                    alpha, a, a.shape[0], b, b.shape[0], beta, want, m) == 0
+    nat, spl = native_and_split(lib, ta, tb, m, n, k, alpha, a, b, beta, c0)
+    A = a.astype(np.float64).T if ta else a.astype(np.float64)
+    B = b.astype(np.float64).T if tb else b.astype(np.float64)
+    exact = alpha * A @ B + beta * c0.astype(np.float64)
+    scale = abs(alpha) * np.linalg.norm(a) * np.linalg.norm(b) + abs(beta) * np.linalg.norm(c0)
+    lim = 2 * k * EPS32 * scale
+    e_spl = np.linalg.norm(spl - want)
+    assert e_spl <= lim, (e_spl, lim)
+    en = np.linalg.norm(nat - exact)
+    es = np.linalg.norm(spl - exact)
+    print(f"ta{ta} tb{tb}: error vs FP64 native {en:.3e} split {es:.3e}")
+    assert es <= 1.5 * en + 1e-30
+
+
+def test_bf16x3_row_scaled_inputs(lib):
+    """Entries N(0,1) * 10^U(-3,3) per row (the precision workload's
+    distribution) at 512^3."""
+    rng = np.random.default_rng(22)
+    m = n = k = 512
+    a = rng.standard_normal((m, k)) * 10.0 ** rng.uniform(-3, 3, (m, 1))
+    b = rng.standard_normal((k, n)) * 10.0 ** rng.uniform(-3, 3, (k, 1))
+    a, b = col(a.astype(np.float32)), col(b.astype(np.float32))
+    c0 = np.zeros((m, n), np.float32, order="F")
+    nat, spl = native_and_split(lib, 0, 0, m, n, k, 1.0, a, b, 0.0, c0)
+    exact = a.astype(np.float64) @ b.astype(np.float64)
+    en = np.linalg.norm(nat - exact) / np.linalg.norm(exact)
+    es = np.linalg.norm(spl - exact) / np.linalg.norm(exact)
+    print(f"row-scaled 512^3: relative error native {en:.3e} split {es:.3e}")
+    assert es <= 1.5 * en
+
+
+def test_sgemm_mode_switch(lib):
+    """cblas_sgemm follows mublis_set_sgemm_mode: mode 1 gives exactly the
+    BF16x3 result, mode 0 the native one."""
+    rng = np.random.default_rng(23)
+    m, n, k = 130, 70, 90
+    a = col(rng.standard_normal((m, k)).astype(np.float32))
+    b = col(rng.standard_normal((k, n)).astype(np.float32))
+    c0 = np.zeros((m, n), np.float32, order="F")
+    nat, spl = native_and_split(lib, 0, 0, m, n, k, 1.0, a, b, 0.0, c0)
+    old = lib.mublis_set_sgemm_mode(1)
+    try:
+        assert lib.mublis_get_sgemm_mode() == 1
+        c = c0.copy(order="F")
+        lib.cblas_sgemm(102, 111, 111, m, n, k, 1.0, M.ptr(a), m, M.ptr(b), k, 0.0, M.ptr(c), m)
+        np.testing.assert_array_equal(c, spl)
+    finally:
+        lib.mublis_set_sgemm_mode(old)
+    c = c0.copy(order="F")
+    lib.cblas_sgemm(102, 111, 111, m, n, k, 1.0, M.ptr(a), m, M.ptr(b), k, 0.0, M.ptr(c), m)
+    np.testing.assert_array_equal(c, nat)
+    assert lib.mublis_set_sgemm_mode(7) == -1
+
+
+def conditioned(rng, s, lower, unit):
+    """Diagonal U[1,2], off-diagonal N(0,1)/sqrt(s) (the refined-STRSM
+    workload's distribution); NaN outside the triangle and on a unit
+    diagonal."""
+    v = rng.standard_normal((s, s)) / np.sqrt(s)
+    v[np.triu_indices(s, 1) if lower else np.tril_indices(s, -1)] = np.nan
+    np.fill_diagonal(v, np.nan if unit else rng.uniform(1, 2, s))
+    return col(v.astype(np.float32))
+
+
+@pytest.mark.parametrize("right,lower,trans,unit", list(itertools.product((0, 1), repeat=4)))
+def test_refined_strsm_all_flags(lib, right, lower, trans, unit):
+    rng = np.random.default_rng(31)
+    m, n = 200, 67
+    s = n if right else m
+    a = conditioned(rng, s, lower, unit)
+    b0 = col(rng.standard_normal((m, n)).astype(np.float32))
+    alpha = 0.75
+    t = L.tri_dense(a, lower, unit)
+    op = t.T if trans else t
+    B = b0.astype(np.float64)
+    exact = alpha * (solve_triangular(op.T, B.T, lower=bool(lower) == bool(trans)).T if right
+                     else solve_triangular(op, B, lower=bool(lower) != bool(trans)))
+    flags = (141 + right, 122 if lower else 121, 112 if trans else 111,
+             132 if unit else 131)
+    plain = b0.copy(order="F")
+    lib.cblas_strsm(102, *flags, m, n, alpha, M.ptr(a), s, M.ptr(plain), m)
+    ref = b0.copy(order="F")
+    st = lib.mublis_strsm_refined(right, 1 if lower else 2, trans, unit, m, n, alpha,
+                                  M.ptr(a), 1, s, M.ptr(ref), 1, m, 2)
+    assert st == 0
+    e_plain = np.linalg.norm(plain - exact) / np.linalg.norm(exact)
+    e_ref = np.linalg.norm(ref - exact) / np.linalg.norm(exact)
+    assert e_ref <= 2 * EPS32, (e_ref, e_plain)
+    assert e_ref <= e_plain * 1.0001 + 1e-30
+
+
+def test_full_size_precision_workloads(lib):
+    """SGEMM M=N=K=16384 (N(0,1)*10^U(-3,3) per row) native vs BF16x3, and
+    STRSM L/L m=8192 nrhs=1024 with 2 refinement sweeps vs DTRSM on the
+    same FP32 inputs."""
+    torch = pytest.importorskip("torch")
+    n = 16384
+    g = torch.Generator(device="cuda").manual_seed(314159)
+    # column-major M x K A is the tensor (K, M); rows of A are tensor columns
+    At = torch.randn((n, n), device="cuda", generator=g) * \
+        10.0 ** (torch.rand((1, n), device="cuda", generator=g) * 6 - 3)
+    Bt = torch.randn((n, n), device="cuda", generator=g) * \
+        10.0 ** (torch.rand((n, 1), device="cuda", generator=g) * 6 - 3)
+    # B is K x N column-major = tensor (N, K): its rows (k index) are tensor columns
+    Bt = Bt.t().contiguous()
+    Cn = torch.empty((n, n), device="cuda")
+    Cs = torch.empty((n, n), device="cuda")
+    lib.cblas_sgemm(102, 111, 111, n, n, n, 1.0, M.ptr(At), n, M.ptr(Bt), n, 0.0, M.ptr(Cn), n)
+    st = lib.mublis_sgemm_bf16x3(0, 0, n, n, n, 1.0, M.ptr(At), 1, n, M.ptr(Bt), 1, n, 0.0,
+                                 M.ptr(Cs), 1, n)
+    torch.cuda.synchronize()
+    assert st == 0
+    rng = np.random.default_rng(5)
+    I_ = torch.from_numpy(rng.integers(0, n, 256)).cuda()
+    J_ = torch.from_numpy(rng.integers(0, n, 256)).cuda()
+    arow = At[:, I_].t().double().cpu().numpy()       # A(i, :) for sampled i
+    bcol = Bt[J_, :].double().cpu().numpy()           # B(:, j) for sampled j
+    exact = np.einsum("tk,tk->t", arow, bcol)
+    gn = Cn[J_, I_].double().cpu().numpy()
+    gs = Cs[J_, I_].double().cpu().numpy()
+    bound = 2 * n * EPS32 * np.einsum("tk,tk->t", np.abs(arow), np.abs(bcol))
+    assert np.all(np.abs(gn - exact) <= bound)
+    assert np.all(np.abs(gs - exact) <= bound)
+    en = np.linalg.norm(gn - exact) / np.linalg.norm(exact)
+    es = np.linalg.norm(gs - exact) / np.linalg.norm(exact)
+    print(f"SGEMM 16384 sampled relative error: native {en:.3e} split {es:.3e}")
+    assert es <= 1.5 * en
+    del At, Bt, Cn, Cs
+    torch.cuda.empty_cache()
+
+    m, nrhs = 8192, 1024
+    At = torch.randn((m, m), device="cuda", generator=g) / m ** 0.5   # A(i,p) = At[p, i]
+    At = torch.triu(At, 1) + torch.diag(1 + torch.rand(m, device="cuda", generator=g))
+    B0 = torch.randn((nrhs, m), device="cuda", generator=g)
+    Xp = B0.clone()
+    lib.cblas_strsm(102, 141, 122, 111, 131, m, nrhs, 1.0, M.ptr(At), m, M.ptr(Xp), m)
+    Xr = B0.clone()
+    st = lib.mublis_strsm_refined(0, 1, 0, 0, m, nrhs, 1.0, M.ptr(At), 1, m, M.ptr(Xr), 1, m, 2)
+    Xd = B0.double()
+    Ad = At.double()
+    lib.cblas_dtrsm(102, 141, 122, 111, 131, m, nrhs, 1.0, M.ptr(Ad), m, M.ptr(Xd), m)
+    torch.cuda.synchronize()
+    assert st == 0
+    ep = ((Xp.double() - Xd).norm() / Xd.norm()).item()
+    er = ((Xr.double() - Xd).norm() / Xd.norm()).item()
+    print(f"STRSM 8192x1024 forward error vs DTRSM: plain {ep:.3e} refined {er:.3e}")
+    assert er <= 2 * EPS32 and er <= ep
