@@ -41,6 +41,7 @@ def parse():
     p.add_argument("--n", type=int, default=8192)
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--no-extra", action="store_true")
+    p.add_argument("--no-batched", action="store_true")
     p.add_argument("--precision-n", type=int, default=16384,
                    help="SGEMM size of the native-vs-BF16x3 comparison (0: skip)")
     p.add_argument("--cpu-slice", type=int, default=2048,
@@ -250,6 +251,47 @@ def run_extras(args, torch, M, lib, dev, g, A, B, C, extra):
     del Af, Bf, Cf
     if args.precision_n:
         run_precision(args, torch, M, lib, dev, g, extra)
+    if not args.no_batched:
+        run_batched(args, torch, M, lib, extra)
+
+
+def run_batched(args, torch, M, lib, extra):
+    """Batched workload: 4096 SGEMM (M, N, K log-uniform [16, 512]) + 1024
+    SSYRK lower (n log-uniform [32, 1024], k [16, 512]), alpha = beta = 1,
+    U[-1, 1], one grouped launch replayed from a hipGraph; against the
+    same problems issued as serial cblas_* calls."""
+    from mublis_amd.workloads import grouped_problems
+    ops, keep, specs = grouped_problems(torch)
+    plan = M.GroupedPlan(ops)
+    info = plan.info()
+    stream = lib.mublis_rt_stream()
+
+    def replay():
+        plan.execute(stream)
+    w_g, ev_g = timed_calls(torch, lib, replay, 10, 2)
+    plan.close()
+
+    def serial():
+        for s in specs:
+            if s["op"] == "gemm":
+                lib.cblas_sgemm(COL, NOTR, NOTR, s["m"], s["n"], s["k"], 1.0, M.ptr(s["A"]),
+                                s["lda"], M.ptr(s["B"]), s["ldb"], 1.0, M.ptr(s["C"]), s["m"])
+            else:
+                lib.cblas_ssyrk(COL, LO, NOTR, s["n"], s["k"], 1.0, M.ptr(s["A"]), s["lda"],
+                                1.0, M.ptr(s["C"]), s["n"])
+    w_s, _ = timed_calls(torch, lib, serial, 1, 0)
+    ideal = max(info["flops"] / (PEAK["f32"] * 1e12), info["bytes"] / 8e12)
+    extra["batched_4096sgemm_1024ssyrk"] = {
+        "tflops": round(info["flops"] / (ev_g * 1e-3) / 1e12, 3),
+        "ms_per_launch": round(ev_g, 4),
+        "flops": info["flops"], "min_hbm_bytes": info["bytes"],
+        "roofline_frac": round(ideal / (ev_g * 1e-3), 4),
+        "bound": "mfma" if info["flops"] / (PEAK["f32"] * 1e12) >= info["bytes"] / 8e12
+                 else "hbm",
+        "work_items": info["items"], "problems_per_tile_shape_32_64_128": info["tiles"],
+        "serial_cblas_calls_ms": round(w_s, 3),
+        "speedup_vs_serial_calls": round(w_s / ev_g, 2),
+    }
 
 
 def run_precision(args, torch, M, lib, dev, g, extra):

@@ -191,6 +191,61 @@ int mublis_strsm_refined(mublis_side_t side, mublis_uplo_t uplo, mublis_trans_t 
 int mublis_set_sgemm_mode(int mode);
 int mublis_get_sgemm_mode(void);
 
+/* ---------------------------------------------- batched extensions ---- */
+/*
+ * Grouped batched FP32 GEMM / SYRK: many uneven problems in ONE persistent
+ * launch, captured once in a hipGraph and replayed.  No counterpart in the
+ * reference (which is one call per problem, frame/l3/l3_driver.c:49-249);
+ * each problem's result is bit-identical to the corresponding cblas_sgemm
+ * or cblas_ssyrk call on the same device operands.
+ *
+ * Operands are column-major DEVICE pointers.  GEMM: C := alpha*op(A)*op(B)
+ * + beta*C with op from trans_a / trans_b (0 or 1), C m x n.  SYRK:
+ * C := alpha*op(A)*op(A)' + beta*C on the `uplo` triangle (MUBLIS_LOWER /
+ * MUBLIS_UPPER) of the n x n C, op(A) n x k (trans_a), b unused.
+ */
+enum { MUBLIS_BATCH_GEMM = 0, MUBLIS_BATCH_SYRK = 1 };
+typedef struct {
+  int op;
+  int trans_a, trans_b;
+  int uplo;
+  int m, n, k;
+  float alpha, beta;
+  const float *a;
+  int lda;
+  const float *b;
+  int ldb;
+  float *c;
+  int ldc;
+} mublis_sbatch_op_t;
+typedef struct mublis_grouped_plan_s *mublis_grouped_plan_t;
+/* flags for plan creation: force one tile shape (0: 32, 1: 64, 2: 128) */
+#define MUBLIS_GROUPED_FORCE_TILE(t) (((t) + 1) << 4)
+
+/* Expand the problems into a cost-sorted tile work list (tile shapes from
+ * the autotuner table, else a size heuristic), upload it and capture the
+ * launch.  Allocates device memory; executing the plan does not. */
+int mublis_sgrouped_plan_create(const mublis_sbatch_op_t *ops, int count, int flags,
+                                mublis_grouped_plan_t *plan);
+/* Replay the captured launch on `stream` (NULL: this thread's stream);
+ * asynchronous. */
+int mublis_grouped_plan_execute(mublis_grouped_plan_t plan, void *stream);
+/* Algorithmic flops and minimum HBM bytes of the plan, work items, and how
+ * many problems got each tile shape. */
+int mublis_grouped_plan_info(mublis_grouped_plan_t plan, double *flops, double *bytes,
+                             int *nitems, int *tile_hist3);
+/* Release a plan whose executions have completed. */
+int mublis_grouped_plan_destroy(mublis_grouped_plan_t plan);
+/* One-shot: plan, execute, synchronise, destroy. */
+int mublis_sgemm_grouped(const mublis_sbatch_op_t *ops, int count);
+/* Autotuner: time each tile shape per (log2 m, log2 n, log2 k) bucket over
+ * [lo, hi] and write/load the table at `path`.  The environment variable
+ * MUBLIS_TUNE_TABLE names the table loaded at first use;
+ * mublis_grouped_load_table switches tables (NULL: heuristic only) and
+ * returns the number of buckets read. */
+int mublis_grouped_autotune(const char *path, int lo, int hi, int per_bucket, int reps);
+int mublis_grouped_load_table(const char *path);
+
 #ifdef __cplusplus
 }
 #endif

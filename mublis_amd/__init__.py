@@ -25,6 +25,16 @@ OK, ECONTEXT, ENOMEM, EINVAL, EHIP = 0, 1, 2, 3, 4
 
 _lib = None
 
+# include/mublis_gfx950.h batched extensions
+BATCH_GEMM, BATCH_SYRK = 0, 1
+
+
+class SbatchOp(ctypes.Structure):
+    """mublis_sbatch_op_t: one FP32 GEMM or SYRK problem on device memory."""
+    _fields_ = [("op", _I), ("trans_a", _I), ("trans_b", _I), ("uplo", _I),
+                ("m", _I), ("n", _I), ("k", _I), ("alpha", _F), ("beta", _F),
+                ("a", _VP), ("lda", _I), ("b", _VP), ("ldb", _I), ("c", _VP), ("ldc", _I)]
+
 
 def build(jobs=8):
     """Compile libmublis.so for gfx950 (always recompiles what changed)."""
@@ -77,6 +87,28 @@ def _declare(lib):
         f = getattr(lib, f"mublis_gfx950_{p}fill_dev")
         f.argtypes = [_I, _I, ct, _VP, _I64, _I64, _VP]
         f.restype = _I
+    lib.mublis_sgrouped_plan_create.argtypes = [_VP, _I, _I, _VP]
+    lib.mublis_sgrouped_plan_create.restype = _I
+    lib.mublis_grouped_plan_execute.argtypes = [_VP, _VP]
+    lib.mublis_grouped_plan_execute.restype = _I
+    lib.mublis_grouped_plan_info.argtypes = [_VP, _VP, _VP, _VP, _VP]
+    lib.mublis_grouped_plan_info.restype = _I
+    lib.mublis_grouped_plan_destroy.argtypes = [_VP]
+    lib.mublis_grouped_plan_destroy.restype = _I
+    lib.mublis_sgemm_grouped.argtypes = [_VP, _I]
+    lib.mublis_sgemm_grouped.restype = _I
+    lib.mublis_grouped_autotune.argtypes = [ctypes.c_char_p, _I, _I, _I, _I]
+    lib.mublis_grouped_autotune.restype = _I
+    lib.mublis_grouped_load_table.argtypes = [ctypes.c_char_p]
+    lib.mublis_grouped_load_table.restype = _I
+    lib.mublis_sgemm_bf16x3.argtypes = [_I] * 5 + [_F, _VP, _I, _I, _VP, _I, _I, _F, _VP, _I,
+                                                  _I]
+    lib.mublis_sgemm_bf16x3.restype = _I
+    lib.mublis_strsm_refined.argtypes = [_I] * 6 + [_F, _VP, _I, _I, _VP, _I, _I, _I]
+    lib.mublis_strsm_refined.restype = _I
+    lib.mublis_set_sgemm_mode.argtypes = [_I]
+    lib.mublis_set_sgemm_mode.restype = _I
+    lib.mublis_get_sgemm_mode.restype = _I
     lib.mublis_rt_init.restype = _I
     lib.mublis_rt_arch.restype = ctypes.c_char_p
     lib.mublis_rt_last_error.restype = ctypes.c_char_p
@@ -127,3 +159,32 @@ def check(status, what="mublis"):
         msg = load().mublis_rt_last_error().decode(errors="replace")
         raise RuntimeError(f"{what} failed with status {status}: {msg}")
     return status
+
+
+class GroupedPlan:
+    """A captured grouped launch (mublis_sgrouped_plan_create / execute)."""
+
+    def __init__(self, ops, flags=0):
+        self.lib = load()
+        self.ops = (SbatchOp * len(ops))(*ops)
+        self.handle = ctypes.c_void_p()
+        check(self.lib.mublis_sgrouped_plan_create(ctypes.cast(self.ops, _VP), len(ops),
+                                                   flags, ctypes.byref(self.handle)),
+              "mublis_sgrouped_plan_create")
+
+    def execute(self, stream=None):
+        check(self.lib.mublis_grouped_plan_execute(self.handle, stream),
+              "mublis_grouped_plan_execute")
+
+    def info(self):
+        f, b = ctypes.c_double(), ctypes.c_double()
+        n = ctypes.c_int()
+        h = (ctypes.c_int * 3)()
+        check(self.lib.mublis_grouped_plan_info(self.handle, ctypes.byref(f), ctypes.byref(b),
+                                                ctypes.byref(n), h))
+        return {"flops": f.value, "bytes": b.value, "items": n.value, "tiles": list(h)}
+
+    def close(self):
+        if self.handle:
+            self.lib.mublis_grouped_plan_destroy(self.handle)
+            self.handle = ctypes.c_void_p()

@@ -1,0 +1,33 @@
+"""Run the grouped-kernel autotuner on the current GPU and write its table.
+
+    python -m mublis_amd.autotune [--out PATH] [--lo 4] [--hi 10]
+
+The table (lines "lm ln lk tile_index", buckets by floor(log2) of m, n, k)
+is what the library loads from mublis_amd/tune/grouped_gfx950.txt (or from
+$MUBLIS_TUNE_TABLE) when it plans a grouped launch.
+"""
+import argparse
+import os
+import time
+
+from . import HERE, check, load
+
+
+def main():
+    p = argparse.ArgumentParser()
+    p.add_argument("--out", default=os.path.join(HERE, "tune", "grouped_gfx950.txt"))
+    p.add_argument("--lo", type=int, default=4)
+    p.add_argument("--hi", type=int, default=10)
+    p.add_argument("--per-bucket", type=int, default=32)
+    p.add_argument("--reps", type=int, default=3)
+    a = p.parse_args()
+    lib = load()
+    check(lib.mublis_rt_init(), "mublis_rt_init")
+    t0 = time.time()
+    check(lib.mublis_grouped_autotune(a.out.encode(), a.lo, a.hi, a.per_bucket, a.reps),
+          "mublis_grouped_autotune")
+    print(f"wrote {a.out} in {time.time() - t0:.1f} s", flush=True)
+
+
+if __name__ == "__main__":
+    main()

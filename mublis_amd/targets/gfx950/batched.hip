@@ -1,0 +1,396 @@
+// Grouped persistent batched L3 kernel for gfx950 (FP32 GEMM and SYRK).
+//
+// Thousands of uneven problems run as ONE launch: the host expands every
+// problem into output tiles (SYRK: only the tiles that touch its ji
+// triangle), tags each with a tile shape chosen by the autotuner's table and
+// the problem's operand layout, and sorts the work list by estimated cost
+// (largest first).  A persistent grid of two workgroups per CU walks the
+// list cyclically; each item runs the same per-tile MFMA routine as the
+// one-problem kernel (gemm_tile.h), so a problem's result is bit-identical
+// to what cblas_sgemm / cblas_ssyrk compute for it.  The launch is captured
+// once in a hipGraph and replayed.
+//
+// The reference has no batched interface; this is the batched addition on
+// top of its per-call driver (frame/l3/l3_driver.c:49-249).
+#include <dlfcn.h>
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "gemm_tile.h"
+#include "gfx950_common.h"
+#include "mublis_gfx950.h"
+
+using namespace gfx950;
+
+namespace {
+
+struct GroupItem {
+  int prob, tm, tn, kind;  // kind = tile index * 4 + layout (akc | bnc << 1)
+};
+
+constexpr int kTiles[3] = {32, 64, 128};
+
+template <int TILE>
+__device__ inline void run_tile(const GemmArgs<float> &p, int tm, int tn, int layout,
+                                float *lds) {
+  switch (layout) {
+    case 0: gemm_tile<float, TILE, false, false>(p, tm, tn, lds); break;
+    case 1: gemm_tile<float, TILE, true, false>(p, tm, tn, lds); break;
+    case 2: gemm_tile<float, TILE, false, true>(p, tm, tn, lds); break;
+    default: gemm_tile<float, TILE, true, true>(p, tm, tn, lds); break;
+  }
+}
+
+__global__ __launch_bounds__(NT, 2) void grouped_kernel(const GemmArgs<float> *__restrict__ probs,
+                                                        const GroupItem *__restrict__ items,
+                                                        int nitems) {
+  __shared__ float lds[LdsSize<float, 128>::value];
+  for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
+    const GroupItem w = items[it];
+    const GemmArgs<float> p = probs[w.prob];
+    const int tile = w.kind >> 2, layout = w.kind & 3;
+    if (tile == 0) run_tile<32>(p, w.tm, w.tn, layout, lds);
+    else if (tile == 1) run_tile<64>(p, w.tm, w.tn, layout, lds);
+    else run_tile<128>(p, w.tm, w.tn, layout, lds);
+    __syncthreads();  // the next item restages lds
+  }
+}
+
+// ------------------------------------------------------------ tune table --
+// Bucket = (floor log2 m, floor log2 n, floor log2 k), each clamped to
+// [4, 13]; value = index into kTiles.
+std::mutex g_tab_mu;
+bool g_tab_loaded = false;
+std::map<std::tuple<int, int, int>, int> g_tab;
+
+int lg(int v) {
+  int l = 0;
+  while ((2 << l) <= v) ++l;
+  return std::min(std::max(l, 4), 13);
+}
+
+int heuristic_tile(int m, int n, int k) {
+  (void)k;
+  const int mn = std::min(m, n);
+  if (mn >= 384) return 2;
+  if (mn >= 96) return 1;
+  return 0;
+}
+
+void load_table_locked(const char *path) {
+  g_tab.clear();
+  FILE *f = fopen(path, "r");
+  if (!f) return;
+  char line[256];
+  while (fgets(line, sizeof line, f)) {
+    int a, b, c, t;
+    if (line[0] == '#') continue;
+    if (sscanf(line, "%d %d %d %d", &a, &b, &c, &t) == 4 && t >= 0 && t < 3)
+      g_tab[std::make_tuple(a, b, c)] = t;
+  }
+  fclose(f);
+}
+
+// $MUBLIS_TUNE_TABLE, else tune/grouped_gfx950.txt beside the library's
+// lib/ directory (the table the autotuner wrote for this device family).
+std::string default_table_path() {
+  const char *e = getenv("MUBLIS_TUNE_TABLE");
+  if (e && *e) return e;
+  Dl_info info;
+  if (dladdr((void *)&heuristic_tile, &info) && info.dli_fname) {
+    std::string lib = info.dli_fname;
+    const size_t slash = lib.rfind('/');
+    const std::string dir = slash == std::string::npos ? "." : lib.substr(0, slash);
+    return dir + "/../tune/grouped_gfx950.txt";
+  }
+  return "";
+}
+
+int choose_tile(int m, int n, int k) {
+  std::lock_guard<std::mutex> lk(g_tab_mu);
+  if (!g_tab_loaded) {
+    const std::string p = default_table_path();
+    if (!p.empty()) load_table_locked(p.c_str());
+    g_tab_loaded = true;
+  }
+  auto it = g_tab.find(std::make_tuple(lg(m), lg(n), lg(k)));
+  return it != g_tab.end() ? it->second : heuristic_tile(m, n, k);
+}
+
+bool aligned16f(const float *x, int64_t ld) {
+  return ((uintptr_t)x % 16 == 0) && ((ld * 4) % 16 == 0);
+}
+
+}  // namespace
+
+struct mublis_grouped_plan_s {
+  GemmArgs<float> *d_probs = nullptr;
+  GroupItem *d_items = nullptr;
+  int nitems = 0, grid = 0;
+  hipStream_t cap = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  double flops = 0, bytes = 0;
+  int tile_hist[3] = {0, 0, 0};
+};
+
+extern "C" {
+
+int mublis_grouped_load_table(const char *path) {
+  std::lock_guard<std::mutex> lk(g_tab_mu);
+  if (path) load_table_locked(path);
+  else g_tab.clear();
+  g_tab_loaded = true;
+  return (int)g_tab.size();
+}
+
+int mublis_sgrouped_plan_create(const mublis_sbatch_op_t *ops, int count, int flags,
+                                mublis_grouped_plan_t *out) {
+  if (!ops || count < 0 || !out) return MUBLIS_EINVAL;
+  int st = mublis_rt_init();
+  if (st) return st;
+  auto *plan = new mublis_grouped_plan_s();
+  std::vector<GemmArgs<float>> probs;
+  std::vector<GroupItem> items;
+  std::vector<double> cost;
+  probs.reserve(count);
+  const int forced = (flags >> 4) & 3;  // MUBLIS_GROUPED_FORCE_TILE(t) stores t + 1
+  for (int i = 0; i < count; ++i) {
+    const mublis_sbatch_op_t &o = ops[i];
+    const bool syrk = o.op == MUBLIS_BATCH_SYRK;
+    const int m = syrk ? o.n : o.m, n = o.n, k = o.k;
+    if (m < 0 || n < 0 || k < 0 || (!syrk && o.op != MUBLIS_BATCH_GEMM)) {
+      delete plan;
+      return MUBLIS_EINVAL;
+    }
+    GemmArgs<float> p;
+    memset(&p, 0, sizeof p);
+    // alpha == 0 reads neither A nor B (frame/l3/l3_driver.c:123-124): K = 0
+    p.m = m; p.n = n; p.k = o.alpha == 0.0f ? 0 : k;
+    p.alpha = o.alpha; p.beta = o.beta;
+    // column-major operands; transposes are stride swaps (frame/l3/gemm.c:14-24)
+    int64_t rsa = 1, csa = o.lda, rsb = 1, csb = o.ldb;
+    if (o.trans_a) std::swap(rsa, csa);
+    if (syrk) {
+      p.a = o.a; p.rs_a = rsa; p.cs_a = csa;
+      p.b = o.a; p.rs_b = csa; p.cs_b = rsa;  // B = A' (frame/l3/syrk.c:43-51)
+      p.ji = o.uplo == MUBLIS_UPPER ? 2 : 1;
+    } else {
+      if (o.trans_b) std::swap(rsb, csb);
+      p.a = o.a; p.rs_a = rsa; p.cs_a = csa;
+      p.b = o.b; p.rs_b = rsb; p.cs_b = csb;
+      p.ji = 0;
+    }
+    p.c = o.c; p.rs_c = 1; p.cs_c = o.ldc;
+    const bool akc = p.cs_a == 1, bnc = p.rs_b != 1;
+    if ((p.rs_a != 1 && p.cs_a != 1) || (p.rs_b != 1 && p.cs_b != 1)) {
+      delete plan;
+      return MUBLIS_EINVAL;
+    }
+    p.vec_a = aligned16f(p.a, akc ? p.rs_a : p.cs_a);
+    p.vec_b = aligned16f(p.b, bnc ? p.rs_b : p.cs_b);
+    const int tix = forced ? forced - 1 : choose_tile(m, n, k);
+    const int T = kTiles[tix];
+    plan->tile_hist[tix]++;
+    const int layout = (akc ? 1 : 0) | (bnc ? 2 : 0);
+    const int tmn = (m + T - 1) / T, tnn = (n + T - 1) / T;
+    const int pid = (int)probs.size();
+    probs.push_back(p);
+    plan->flops += syrk ? (double)n * (n + 1) * k : 2.0 * m * n * k;
+    plan->bytes += 4.0 * ((double)m * k + (syrk ? 0.0 : (double)k * n) +
+                          (o.beta != 0.0f ? 2.0 : 1.0) * (syrk ? n * (n + 1) / 2.0
+                                                               : (double)m * n));
+    if (m == 0 || n == 0) continue;
+    for (int tn = 0; tn < tnn; ++tn)
+      for (int tm = 0; tm < tmn; ++tm) {
+        if (syrk && p.ji == 1 && tn > tm) continue;  // tile entirely above
+        if (syrk && p.ji == 2 && tm > tn) continue;  // tile entirely below
+        items.push_back({pid, tm, tn, tix * 4 + layout});
+        cost.push_back((double)T * T * std::max(k, 1) + 64.0 * T * T);
+      }
+  }
+  // largest items first, so the cyclic walk of the persistent grid ends
+  // with small items (LPT order)
+  std::vector<int> order(items.size());
+  for (size_t i = 0; i < order.size(); ++i) order[i] = (int)i;
+  std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return cost[x] > cost[y]; });
+  std::vector<GroupItem> sorted(items.size());
+  for (size_t i = 0; i < order.size(); ++i) sorted[i] = items[order[i]];
+  plan->nitems = (int)sorted.size();
+  int dev = 0, cus = 256;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+    cus = 256;
+  plan->grid = std::max(1, std::min(plan->nitems, 2 * cus));
+  st = MUBLIS_OK;
+  if (!probs.empty() && hipMalloc(&plan->d_probs, sizeof(GemmArgs<float>) * probs.size()))
+    st = MUBLIS_ENOMEM;
+  if (!st && !sorted.empty() && hipMalloc(&plan->d_items, sizeof(GroupItem) * sorted.size()))
+    st = MUBLIS_ENOMEM;
+  if (!st && !probs.empty() &&
+      hipMemcpy(plan->d_probs, probs.data(), sizeof(GemmArgs<float>) * probs.size(),
+                hipMemcpyHostToDevice))
+    st = MUBLIS_EHIP;
+  if (!st && !sorted.empty() &&
+      hipMemcpy(plan->d_items, sorted.data(), sizeof(GroupItem) * sorted.size(),
+                hipMemcpyHostToDevice))
+    st = MUBLIS_EHIP;
+  // Capture the launch once; executions replay the graph.
+  if (!st && hipStreamCreateWithFlags(&plan->cap, hipStreamNonBlocking)) st = MUBLIS_EHIP;
+  if (!st && hipStreamBeginCapture(plan->cap, hipStreamCaptureModeThreadLocal)) st = MUBLIS_EHIP;
+  if (!st) {
+    if (plan->nitems > 0)
+      hipLaunchKernelGGL(grouped_kernel, dim3(plan->grid), dim3(NT), 0, plan->cap,
+                         plan->d_probs, plan->d_items, plan->nitems);
+    hipError_t e1 = hipGetLastError();
+    hipError_t e2 = hipStreamEndCapture(plan->cap, &plan->graph);
+    if (e1 || e2) st = MUBLIS_EHIP;
+  }
+  if (!st && hipGraphInstantiate(&plan->exec, plan->graph, nullptr, nullptr, 0))
+    st = MUBLIS_EHIP;
+  if (st) {
+    mublis_grouped_plan_destroy(plan);
+    return st;
+  }
+  *out = plan;
+  return MUBLIS_OK;
+}
+
+int mublis_grouped_plan_execute(mublis_grouped_plan_t plan, void *stream) {
+  if (!plan) return MUBLIS_EINVAL;
+  hipStream_t s = (hipStream_t)(stream ? stream : mublis_rt_stream());
+  return hipGraphLaunch(plan->exec, s) == hipSuccess ? MUBLIS_OK : MUBLIS_EHIP;
+}
+
+int mublis_grouped_plan_info(mublis_grouped_plan_t plan, double *flops, double *bytes,
+                             int *nitems, int *tile_hist3) {
+  if (!plan) return MUBLIS_EINVAL;
+  if (flops) *flops = plan->flops;
+  if (bytes) *bytes = plan->bytes;
+  if (nitems) *nitems = plan->nitems;
+  if (tile_hist3)
+    for (int i = 0; i < 3; ++i) tile_hist3[i] = plan->tile_hist[i];
+  return MUBLIS_OK;
+}
+
+int mublis_grouped_plan_destroy(mublis_grouped_plan_t plan) {
+  if (!plan) return MUBLIS_OK;
+  if (plan->cap) (void)hipStreamSynchronize(plan->cap);
+  if (plan->exec) (void)hipGraphExecDestroy(plan->exec);
+  if (plan->graph) (void)hipGraphDestroy(plan->graph);
+  if (plan->cap) (void)hipStreamDestroy(plan->cap);
+  if (plan->d_probs) (void)hipFree(plan->d_probs);
+  if (plan->d_items) (void)hipFree(plan->d_items);
+  delete plan;
+  return MUBLIS_OK;
+}
+
+int mublis_sgemm_grouped(const mublis_sbatch_op_t *ops, int count) {
+  mublis_grouped_plan_t plan = nullptr;
+  int st = mublis_sgrouped_plan_create(ops, count, 0, &plan);
+  if (st) return st;
+  void *s = mublis_rt_stream();
+  st = mublis_grouped_plan_execute(plan, s);
+  int sy = mublis_rt_sync(s);
+  if (!st) st = sy;
+  mublis_grouped_plan_destroy(plan);
+  return st;
+}
+
+/*
+ * Autotuner: for every size bucket (floor log2 of m, n, k over [lo, hi]),
+ * time a grouped launch of `per_bucket` random problems of that bucket with
+ * each tile shape forced, and keep the fastest.  Writes "lm ln lk tile"
+ * lines to `path` and loads the table.  Operands are device buffers owned by
+ * the tuner; values are irrelevant to timing.
+ */
+int mublis_grouped_autotune(const char *path, int lo, int hi, int per_bucket, int reps) {
+  int st = mublis_rt_init();
+  if (st) return st;
+  if (!path || lo < 4 || hi > 13 || lo > hi || per_bucket < 1 || reps < 1) return MUBLIS_EINVAL;
+  const int maxd = 2 << hi;  // exclusive bound of the largest bucket
+  const size_t per = (size_t)maxd * maxd;
+  float *buf = nullptr;
+  const size_t total = per * 3 * per_bucket;
+  if (hipMalloc(&buf, total * sizeof(float))) return MUBLIS_ENOMEM;
+  if (hipMemset(buf, 0, total * sizeof(float))) {
+    (void)hipFree(buf);
+    return MUBLIS_EHIP;
+  }
+  hipStream_t s = (hipStream_t)mublis_rt_stream();
+  hipEvent_t e0, e1;
+  if (hipEventCreate(&e0) || hipEventCreate(&e1)) {
+    (void)hipFree(buf);
+    return MUBLIS_EHIP;
+  }
+  FILE *f = fopen(path, "w");
+  if (!f) {
+    (void)hipFree(buf);
+    return MUBLIS_EINVAL;
+  }
+  fprintf(f, "# gfx950 grouped-kernel tile table: lm ln lk tile_index (tiles 32 64 128)\n");
+  fprintf(f, "# bucket = floor(log2) of m, n, k; %d problems per bucket, %d reps\n",
+          per_bucket, reps);
+  uint64_t seed = 314159;
+  auto rnd = [&](int l) {
+    seed = seed * 6364136223846793005ULL + 1442695040888963407ULL;
+    const int span = 1 << l;
+    return span + (int)((seed >> 33) % (uint64_t)span);
+  };
+  for (int lm = lo; lm <= hi && !st; ++lm)
+    for (int ln = lo; ln <= hi && !st; ++ln)
+      for (int lk = lo; lk <= hi && !st; ++lk) {
+        std::vector<mublis_sbatch_op_t> ops(per_bucket);
+        for (int i = 0; i < per_bucket; ++i) {
+          mublis_sbatch_op_t &o = ops[i];
+          memset(&o, 0, sizeof o);
+          o.op = MUBLIS_BATCH_GEMM;
+          o.m = std::min(rnd(lm), maxd - 1);
+          o.n = std::min(rnd(ln), maxd - 1);
+          o.k = std::min(rnd(lk), maxd - 1);
+          o.alpha = 1.0f; o.beta = 1.0f;
+          float *base = buf + per * 3 * i;
+          o.a = base; o.lda = o.m;
+          o.b = base + per; o.ldb = o.k;
+          o.c = base + 2 * per; o.ldc = o.m;
+        }
+        float best = 1e30f;
+        int best_t = heuristic_tile(1 << lm, 1 << ln, 1 << lk);
+        for (int t = 0; t < 3 && !st; ++t) {
+          mublis_grouped_plan_t plan = nullptr;
+          st = mublis_sgrouped_plan_create(ops.data(), per_bucket, (t + 1) << 4, &plan);
+          if (st) break;
+          st = mublis_grouped_plan_execute(plan, s);  // warm-up
+          if (!st && hipEventRecord(e0, s)) st = MUBLIS_EHIP;
+          for (int r = 0; r < reps && !st; ++r) st = mublis_grouped_plan_execute(plan, s);
+          float ms = 0;
+          if (!st && (hipEventRecord(e1, s) || hipEventSynchronize(e1) ||
+                      hipEventElapsedTime(&ms, e0, e1)))
+            st = MUBLIS_EHIP;
+          mublis_grouped_plan_destroy(plan);
+          if (!st && ms < best) {
+            best = ms;
+            best_t = t;
+          }
+        }
+        if (!st) fprintf(f, "%d %d %d %d\n", lm, ln, lk, best_t);
+      }
+  fclose(f);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipFree(buf);
+  if (!st) mublis_grouped_load_table(path);
+  return st;
+}
+
+}  // extern "C"

@@ -26,277 +26,18 @@
 #include "mublis_gfx950.h"
 #include "gfx950_common.h"
 
+#include "gemm_tile.h"
+
 namespace {
 
-typedef double d4 __attribute__((ext_vector_type(4)));
-typedef float f4 __attribute__((ext_vector_type(4)));
-typedef double d2 __attribute__((ext_vector_type(2)));
-
-template <typename T> struct Mfma;
-template <> struct Mfma<double> {
-  typedef d4 acc_t;
-  typedef d2 vec_t;  // 16-byte global vector
-  static constexpr int BK = 16;
-  __device__ static inline acc_t step(double a, double b, acc_t c) {
-    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
-  }
-  // Output row of register r in lane group g (= lane / 16): the FP64 MFMA
-  // interleaves rows across lane groups (measured on gfx950).
-  __device__ static inline int row(int g, int r) { return g + 4 * r; }
-};
-template <> struct Mfma<float> {
-  typedef f4 acc_t;
-  typedef f4 vec_t;
-  static constexpr int BK = 32;
-  __device__ static inline acc_t step(float a, float b, acc_t c) {
-    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
-  }
-  // The FP32 MFMA keeps 4 consecutive rows per lane group.
-  __device__ static inline int row(int g, int r) { return 4 * g + r; }
-};
-
-constexpr int NT = 256;       // threads per workgroup (4 waves, 2 x 2)
-constexpr int LDS_PAD = 16;   // elements; shifts successive k rows by half the banks
-
-// Workgroup tile TILE x TILE: 128 for large problems, 64 when the 128-tile
-// grid would leave most CUs idle (the skinny GEMMs of the TRSM recursion,
-// small batched problems).  Each wave owns (TILE/2)^2 = TI x TI MFMA tiles.
-template <int TILE>
-struct Tile {
-  static constexpr int BM = TILE, BN = TILE, S = TILE + LDS_PAD;
-  static constexpr int TI = TILE / 32;  // 16 x 16 MFMA tiles per wave and dimension
-  static constexpr int MIN_WG_PER_CU = TILE == 128 ? 2 : 4;
-};
-
-template <typename T>
-struct GemmArgs {
-  int m, n, k;
-  T alpha, beta;
-  const T *a; int64_t rs_a, cs_a;
-  const T *b; int64_t rs_b, cs_b;
-  const T *a2; int64_t rs_a2, cs_a2;
-  const T *b2; int64_t rs_b2, cs_b2;
-  T *c; int64_t rs_c, cs_c;
-  int ji;          // 0 all, 1 lower (j <= i), 2 upper (j >= i)
-  int tiles_m, tiles_n;
-  int tri;         // grid enumerates only tiles touching the ji triangle
-  int ntiles;
-  int vec_a, vec_b, vec_a2, vec_b2;  // 16-byte loads are aligned
-  // Triangular operand structure, used only to skip K slices that are zero:
-  // tri_a 1: A(i,p) = 0 for p > i, 2: for p < i; tri_b 1: B(p,j) = 0 for
-  // p < j, 2: for p > j.  Values are still read as stored (the caller has
-  // materialised the zeros), so rounding K bounds outward is harmless.
-  int tri_a, tri_b;
-};
-
-// Map a hardware workgroup id to an output tile.  Consecutive hardware ids
-// land on different XCDs (round robin over 8), so ids are first regrouped so
-// that each XCD owns a contiguous range, and that range is walked in groups
-// of 8 tile rows, so the workgroups resident on one XCD share A and B panels
-// in its L2.
-__device__ inline void map_tile(int w, int ntiles, int tiles_m, int tiles_n,
-                                int tri, int ji, int &tm, int &tn) {
-  const int q = ntiles / 8, r = ntiles % 8;
-  const int xcd = w % 8, local = w / 8;
-  const int id = (xcd < r ? xcd * (q + 1) : r * (q + 1) + (xcd - r) * q) + local;
-  if (tri) {
-    // Row-major walk over the lower triangle of tiles (tn <= tm); for an
-    // upper ji the same walk is transposed.
-    int t = (int)((sqrtf(8.0f * (float)id + 1.0f) - 1.0f) * 0.5f);
-    while ((t + 1) * (t + 2) / 2 <= id) ++t;
-    while (t * (t + 1) / 2 > id) --t;
-    int a = t, b = id - t * (t + 1) / 2;
-    if (ji == 2) { tm = b; tn = a; } else { tm = a; tn = b; }
-    return;
-  }
-  const int GROUP = 8;
-  const int per_group = GROUP * tiles_n;
-  const int g = id / per_group;
-  const int first_m = g * GROUP;
-  const int gsize = min(tiles_m - first_m, GROUP);
-  const int in = id - g * per_group;
-  tm = first_m + in % gsize;
-  tn = in / gsize;
-}
-
-// One operand slice (A: BM x BK, B: BK x BN) in registers.  `c` runs along the
-// operand's unit-stride dimension, `o` along the other; V elements per vector.
-template <typename T, int LC, int LO, int S>
-struct Slice {
-  static constexpr int V = 16 / sizeof(T);
-  static constexpr int NV = LC * LO / (V * NT);
-  T v[NV][V];
-
-  __device__ inline void load(const T *__restrict__ x, int64_t ld, int c0, int o0,
-                              int c_lim, int o_lim, bool vec_ok, int tid) {
-    typedef typename Mfma<T>::vec_t vec_t;
-    const bool full = vec_ok && (c0 + LC <= c_lim) && (o0 + LO <= o_lim);
-#pragma unroll
-    for (int i = 0; i < NV; ++i) {
-      const int lin = i * NT + tid;
-      const int c = (lin % (LC / V)) * V;
-      const int o = lin / (LC / V);
-      const T *p = x + (int64_t)(o0 + o) * ld + (c0 + c);
-      if (full) {
-        vec_t t = *reinterpret_cast<const vec_t *>(p);
-#pragma unroll
-        for (int e = 0; e < V; ++e) v[i][e] = t[e];
-      } else {
-#pragma unroll
-        for (int e = 0; e < V; ++e) {
-          T t = T(0);
-          if ((c0 + c + e < c_lim) && (o0 + o < o_lim)) t = p[e];
-          v[i][e] = t;
-        }
-      }
-    }
-  }
-
-  // LDS tile is [k][mn] with row stride S.  When the unit-stride dimension
-  // is k (CK), element (c, o) goes to [c][o]; otherwise to [o][c].
-  template <bool CK>
-  __device__ inline void store(T *lds, int tid) const {
-#pragma unroll
-    for (int i = 0; i < NV; ++i) {
-      const int lin = i * NT + tid;
-      const int c = (lin % (LC / V)) * V;
-      const int o = lin / (LC / V);
-#pragma unroll
-      for (int e = 0; e < V; ++e) {
-        if (CK) lds[(c + e) * S + o] = v[i][e];
-        else lds[o * S + c + e] = v[i][e];
-      }
-    }
-  }
-};
-
-template <typename T, int TILE, bool A_KC, bool B_NC>
-struct Operands {
-  static constexpr int BK = Mfma<T>::BK;
-  static constexpr int BM = Tile<TILE>::BM, BN = Tile<TILE>::BN, S = Tile<TILE>::S;
-  // A is m x k: K-contiguous -> (c = k, o = m); else (c = m, o = k).
-  typedef Slice<T, A_KC ? BK : BM, A_KC ? BM : BK, S> SA;
-  // B is k x n: N-contiguous -> (c = n, o = k); else (c = k, o = n).
-  typedef Slice<T, B_NC ? BN : BK, B_NC ? BK : BN, S> SB;
-};
-
-template <typename T, int TILE, bool A_KC, bool B_NC>
-__device__ inline void load_slices(typename Operands<T, TILE, A_KC, B_NC>::SA &ra,
-                                   typename Operands<T, TILE, A_KC, B_NC>::SB &rb,
-                                   const T *a, int64_t lda, const T *b,
-                                   int64_t ldb, int m0, int n0, int k0, int m,
-                                   int n, int k, bool va, bool vb, int tid) {
-  if (A_KC) ra.load(a, lda, k0, m0, k, m, va, tid);
-  else ra.load(a, lda, m0, k0, m, k, va, tid);
-  if (B_NC) rb.load(b, ldb, n0, k0, n, k, vb, tid);
-  else rb.load(b, ldb, k0, n0, k, n, vb, tid);
-}
+using namespace gfx950;
 
 template <typename T, int TILE, bool A_KC, bool B_NC>
 __global__ __launch_bounds__(NT, Tile<TILE>::MIN_WG_PER_CU) void gemm_kernel(GemmArgs<T> p) {
-  typedef Mfma<T> MF;
-  typedef typename MF::acc_t acc_t;
-  constexpr int BK = MF::BK;
-  constexpr int BM = Tile<TILE>::BM, BN = Tile<TILE>::BN, S = Tile<TILE>::S;
-  constexpr int TI = Tile<TILE>::TI, WT = TILE / 2;
-  typedef Operands<T, TILE, A_KC, B_NC> Ops;
-  __shared__ T As[2][BK * S];
-  __shared__ T Bs[2][BK * S];
-
+  __shared__ T lds[LdsSize<T, TILE>::value];
   int tm, tn;
   map_tile(blockIdx.x, p.ntiles, p.tiles_m, p.tiles_n, p.tri, p.ji, tm, tn);
-  const int m0 = tm * BM, n0 = tn * BN;
-  const int tid = threadIdx.x;
-  const int lane = tid & 63, wave = tid >> 6;
-  const int wm = (wave & 1) * WT, wn = (wave >> 1) * WT;
-  const int lr = lane & 15, lk = lane >> 4;
-
-  acc_t acc[TI][TI];
-#pragma unroll
-  for (int i = 0; i < TI; ++i)
-#pragma unroll
-    for (int j = 0; j < TI; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
-
-  typename Ops::SA ra;
-  typename Ops::SB rb;
-
-#pragma unroll 1
-  for (int seg = 0; seg < 2; ++seg) {
-    const T *a = seg ? p.a2 : p.a;
-    const T *b = seg ? p.b2 : p.b;
-    if (!a) break;
-    const int64_t lda = seg ? (A_KC ? p.rs_a2 : p.cs_a2) : (A_KC ? p.rs_a : p.cs_a);
-    const int64_t ldb = seg ? (B_NC ? p.rs_b2 : p.cs_b2) : (B_NC ? p.rs_b : p.cs_b);
-    const bool va = seg ? p.vec_a2 : p.vec_a;
-    const bool vb = seg ? p.vec_b2 : p.vec_b;
-    int kbeg = 0, kend = p.k;
-    if (p.tri_a == 1) kend = min(kend, m0 + BM);
-    if (p.tri_a == 2) kbeg = max(kbeg, m0);
-    if (p.tri_b == 1) kbeg = max(kbeg, n0);
-    if (p.tri_b == 2) kend = min(kend, n0 + BN);
-    const int kt0 = kbeg / BK;
-    const int ktiles = kend > kbeg ? (kend + BK - 1) / BK - kt0 : 0;
-    if (ktiles <= 0) continue;
-
-    load_slices<T, TILE, A_KC, B_NC>(ra, rb, a, lda, b, ldb, m0, n0, kt0 * BK, p.m, p.n,
-                                     p.k, va, vb, tid);
-    __syncthreads();  // previous segment's readers are done with buffer 0
-    ra.template store<A_KC>(As[0], tid);
-    rb.template store<!B_NC>(Bs[0], tid);
-    __syncthreads();
-
-#pragma unroll 1
-    for (int kt = 0; kt < ktiles; ++kt) {
-      const int cur = kt & 1;
-      const bool more = kt + 1 < ktiles;
-      if (more)
-        load_slices<T, TILE, A_KC, B_NC>(ra, rb, a, lda, b, ldb, m0, n0,
-                                         (kt0 + kt + 1) * BK, p.m, p.n, p.k, va, vb, tid);
-      const T *as = As[cur];
-      const T *bs = Bs[cur];
-#pragma unroll
-      for (int kk = 0; kk < BK / 4; ++kk) {
-        T af[TI], bf[TI];
-        const int row = (kk * 4 + lk) * S;
-#pragma unroll
-        for (int i = 0; i < TI; ++i) af[i] = as[row + wm + i * 16 + lr];
-#pragma unroll
-        for (int j = 0; j < TI; ++j) bf[j] = bs[row + wn + j * 16 + lr];
-#pragma unroll
-        for (int i = 0; i < TI; ++i)
-#pragma unroll
-          for (int j = 0; j < TI; ++j) acc[i][j] = MF::step(af[i], bf[j], acc[i][j]);
-      }
-      if (more) {
-        ra.template store<A_KC>(As[cur ^ 1], tid);
-        rb.template store<!B_NC>(Bs[cur ^ 1], tid);
-      }
-      __syncthreads();
-    }
-  }
-
-  // Epilogue.  MFMA 16x16 output layout: lane l, register r holds column
-  // l%16 and row MF::row(l/16, r) of its 16 x 16 tile.
-  const T alpha = p.alpha, beta = p.beta;
-#pragma unroll
-  for (int i = 0; i < TI; ++i) {
-#pragma unroll
-    for (int j = 0; j < TI; ++j) {
-      const int col = n0 + wn + j * 16 + lr;
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int row = m0 + wm + i * 16 + MF::row(lk, r);
-        if (row >= p.m || col >= p.n) continue;
-        if (p.ji == 1 && col > row) continue;
-        if (p.ji == 2 && col < row) continue;
-        T *cp = p.c + (int64_t)row * p.rs_c + (int64_t)col * p.cs_c;
-        const T prod = alpha * acc[i][j][r];
-        if (beta == T(0)) *cp = prod;
-        else if (beta == T(1)) *cp = *cp + prod;
-        else *cp = prod + beta * *cp;
-      }
-    }
-  }
+  gemm_tile<T, TILE, A_KC, B_NC>(p, tm, tn, lds);
 }
 
 // beta-only update on the ji part of C (alpha == 0 or k == 0): the
