@@ -31,6 +31,7 @@ PEAK = {"f64": 78.6, "f32": 157.3}  # dense MFMA TFLOP/s, MI355X
 METRIC = ("TFLOP/s & %MFMA peak, D/SGEMM+DTRSM n=8192; p50/p99 mixed-L3 request "
           "latency")
 COL, ROW, NOTR, TR, UP, LO, NONUNIT, LEFT = 102, 101, 111, 112, 121, 122, 131, 141
+SERVE_RATE = 8000.0  # requests/s of the serving workload's Poisson arrivals
 
 
 def parse():
@@ -42,11 +43,16 @@ def parse():
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--no-extra", action="store_true")
     p.add_argument("--no-batched", action="store_true")
+    p.add_argument("--no-serving", action="store_true")
+    p.add_argument("--serve-requests", type=int, default=2000)
+    p.add_argument("--serve-rate", type=float, default=SERVE_RATE,
+                   help="Poisson arrival rate of the serving workload, requests/s")
     p.add_argument("--precision-n", type=int, default=16384,
                    help="SGEMM size of the native-vs-BF16x3 comparison (0: skip)")
     p.add_argument("--cpu-slice", type=int, default=2048,
                    help="M = N of the CPU-baseline DGEMM slice (K stays 8192)")
-    p.add_argument("--only", choices=["dgemm", "dtrsm", "dsyrk", "sgemm", "sgemm_bf16x3"],
+    p.add_argument("--only", choices=["dgemm", "dtrsm", "dsyrk", "sgemm", "sgemm_bf16x3",
+                                      "serving"],
                    help="profiling aid: run just this operation (3 calls) and exit")
     p.add_argument("--traffic-probe", action="store_true",
                    help="profiling aid: one 512 MiB device copy (counter "
@@ -253,6 +259,8 @@ def run_extras(args, torch, M, lib, dev, g, A, B, C, extra):
         run_precision(args, torch, M, lib, dev, g, extra)
     if not args.no_batched:
         run_batched(args, torch, M, lib, extra)
+    if not args.no_serving:
+        run_serving(args, torch, M, lib, extra)
 
 
 def run_batched(args, torch, M, lib, extra):
@@ -292,6 +300,124 @@ def run_batched(args, torch, M, lib, extra):
         "serial_cblas_calls_ms": round(w_s, 3),
         "speedup_vs_serial_calls": round(w_s / ev_g, 2),
     }
+
+
+def serial_call(lib, S, r):
+    """One serving request as the plain synchronous cblas_* call on the same
+    device memory."""
+    a, b, c = (ctypes.c_void_p(r[x].ptr) if r.get(x) is not None else None for x in "abc")
+    la, lb, lc = (int(r[x].ld) if r.get(x) is not None else 0 for x in "abc")
+    op = r["op"]
+    if op == S.SGEMM:
+        lib.cblas_sgemm(COL, NOTR, NOTR, r["m"], r["n"], r["k"], r["alpha"], a, la, b, lb,
+                        r["beta"], c, lc)
+    elif op == S.SSYRK:
+        lib.cblas_ssyrk(COL, LO if r["uplo"] == S.LOWER else UP, NOTR, r["n"], r["k"],
+                        r["alpha"], a, la, r["beta"], c, lc)
+    elif op == S.SSYR2K:
+        lib.cblas_ssyr2k(COL, LO if r["uplo"] == S.LOWER else UP, NOTR, r["n"], r["k"],
+                         r["alpha"], a, la, b, lb, r["beta"], c, lc)
+    else:
+        lib.cblas_dtrsm(COL, LEFT, LO, NOTR, NONUNIT, r["m"], r["n"], r["alpha"], a, la, b,
+                        lb)
+
+
+def fifo_latencies(arrivals, service):
+    """Latencies of a single FIFO server with these arrival times and service
+    times (the serial per-call path at the same arrival process)."""
+    free, out = 0.0, np.empty(len(arrivals))
+    for i, (t, d) in enumerate(zip(arrivals, service)):
+        start = max(t, free)
+        free = start + d
+        out[i] = free - t
+    return out
+
+
+def serve_run(S, srv, reqs, arrivals, clients=8):
+    """Submit reqs from `clients` threads at the given arrival offsets
+    (seconds), wait for all; returns (stats, max arrival lag in ms)."""
+    import threading
+    srv.reset_stats()
+    tickets, lag = [None] * len(reqs), [0.0] * len(reqs)
+    kw = [{k: v for k, v in r.items() if k not in ("op", "flop")} for r in reqs]
+    start = time.perf_counter() + 0.02
+
+    def client(c):
+        for i in range(c, len(reqs), clients):
+            t_arr = start + arrivals[i]
+            d = t_arr - time.perf_counter()
+            if d > 0:
+                time.sleep(d)
+            tickets[i] = srv.submit(reqs[i]["op"], **kw[i])
+            lag[i] = time.perf_counter() - t_arr
+    old = sys.getswitchinterval()
+    sys.setswitchinterval(1e-4)
+    try:
+        ts = [threading.Thread(target=client, args=(c,)) for c in range(clients)]
+        for t in ts:
+            t.start()
+        for t in ts:
+            t.join()
+        for t in tickets:
+            st, _ = srv.wait(t)
+            assert st == 0, st
+    finally:
+        sys.setswitchinterval(old)
+    return srv.stats(), 1e3 * max(lag)
+
+
+def run_serving(args, torch, M, lib, extra):
+    """Serving workload: Poisson arrivals at a fixed rate from 8 client
+    threads, 60% SGEMM / 20% SSYRK+SSYR2K / 20% DTRSM (sizes in
+    mublis_amd.workloads.serving_requests), operands device-resident by
+    handle in a 64 GB budget; against the same requests issued as serial
+    synchronous cblas_* calls."""
+    import mublis_amd.serving as S
+    from mublis_amd.workloads import serving_requests
+    srv = S.Server(hbm_budget=64 << 30)
+    try:
+        reqs, _ = serving_requests(torch, lib, srv, count=args.serve_requests)
+        flop = sum(r["flop"] for r in reqs)
+        for r in reqs[:40]:  # warm both paths
+            serial_call(lib, S, r)
+        serve_run(S, srv, reqs[:40], np.zeros(40))
+        service = np.empty(len(reqs))
+        t_all = time.perf_counter()
+        for i, r in enumerate(reqs):
+            t = time.perf_counter()
+            serial_call(lib, S, r)
+            service[i] = time.perf_counter() - t
+        serial_s = time.perf_counter() - t_all
+        rng = np.random.default_rng(7)
+        arrivals = np.cumsum(rng.exponential(1.0 / args.serve_rate, len(reqs)))
+        st, lag = serve_run(S, srv, reqs, arrivals)
+        fifo = 1e3 * fifo_latencies(arrivals, service)
+        burst, _ = serve_run(S, srv, reqs, np.zeros(len(reqs)))
+        mix = {}
+        for r in reqs:
+            mix[S.OP_NAMES[r["op"]]] = mix.get(S.OP_NAMES[r["op"]], 0) + 1
+        extra["serving_mixed_l3"] = {
+            "requests": len(reqs), "mix": mix, "arrival_rate_per_s": args.serve_rate,
+            "tflops": round(st["tflops"], 3),
+            "p50_ms": round(st["p50_ms"], 3), "p99_ms": round(st["p99_ms"], 3),
+            "mean_ms": round(st["mean_ms"], 3), "max_ms": round(st["max_ms"], 3),
+            "elapsed_s": round(st["elapsed_s"], 4),
+            "serial_cblas_tflops": round(flop / serial_s / 1e12, 3),
+            "serial_cblas_s": round(serial_s, 4),
+            "serial_fifo_same_arrivals_p50_ms": round(float(np.percentile(fifo, 50)), 3),
+            "serial_fifo_same_arrivals_p99_ms": round(float(np.percentile(fifo, 99)), 3),
+            "throughput_vs_serial": round(st["tflops"] / (flop / serial_s / 1e12), 2),
+            "burst_tflops": round(burst["tflops"], 3),
+            "burst_p50_ms": round(burst["p50_ms"], 3), "burst_p99_ms": round(burst["p99_ms"], 3),
+            "launches": st["launches"], "grouped_launches": st["grouped_launches"],
+            "grouped_requests": st["grouped_requests"], "max_pending": st["max_pending"],
+            "max_inflight_launches": st["max_inflight"],
+            "hbm_budget_gb": round(st["hbm_budget"] / 2 ** 30, 1),
+            "hbm_peak_gb": round(st["hbm_peak"] / 2 ** 30, 2),
+            "max_arrival_lag_ms": round(lag, 3),
+        }
+    finally:
+        srv.close()
 
 
 def run_precision(args, torch, M, lib, dev, g, extra):
@@ -378,6 +504,11 @@ def main():
     dev = torch.device("cuda:0")
     n = args.n
     g = torch.Generator(device=dev).manual_seed(314159)
+    if args.only == "serving":
+        extra = {}
+        run_serving(args, torch, M, lib, extra)
+        print(json.dumps(extra), flush=True)
+        return
     if args.only or args.traffic_probe:
         return profile_mode(args, torch, M, lib, dev, g)
     # Column-major K x M A is the row-major (M, K) tensor; op(A) = A^T.

@@ -121,6 +121,19 @@ int mublis_gfx950_strsm_inverted_dev(int lower, int m, int n, const float *a,
                                      int64_t rs_a, int64_t cs_a, float *b,
                                      int64_t rs_b, int64_t cs_b, void *stream);
 
+/* Left-side TRSM with caller workspace (mublis_gfx950_trsm_ws_bytes(m)
+ * bytes of device memory), fully asynchronous on `stream`: the form the
+ * serving loop uses so its scheduler never waits on a solve. */
+size_t mublis_gfx950_trsm_ws_bytes(int m);
+int mublis_gfx950_dtrsm_ws_dev(int lower, int unit, int m, int n, double alpha,
+                               const double *a, int64_t rs_a, int64_t cs_a, double *b,
+                               int64_t rs_b, int64_t cs_b, void *ws, size_t ws_bytes,
+                               void *stream);
+int mublis_gfx950_strsm_ws_dev(int lower, int unit, int m, int n, float alpha,
+                               const float *a, int64_t rs_a, int64_t cs_a, float *b,
+                               int64_t rs_b, int64_t cs_b, void *ws, size_t ws_bytes,
+                               void *stream);
+
 /* Strided copy of an m x n matrix (dst may have other strides). */
 int mublis_gfx950_dcopy_dev(int m, int n, const double *src, int64_t rs_s,
                             int64_t cs_s, double *dst, int64_t rs_d,
@@ -202,9 +215,11 @@ int mublis_get_sgemm_mode(void);
  * Operands are column-major DEVICE pointers.  GEMM: C := alpha*op(A)*op(B)
  * + beta*C with op from trans_a / trans_b (0 or 1), C m x n.  SYRK:
  * C := alpha*op(A)*op(A)' + beta*C on the `uplo` triangle (MUBLIS_LOWER /
- * MUBLIS_UPPER) of the n x n C, op(A) n x k (trans_a), b unused.
+ * MUBLIS_UPPER) of the n x n C, op(A) n x k (trans_a), b unused.  SYR2K:
+ * C := alpha*(op(A)*op(B)' + op(B)*op(A)') + beta*C on the `uplo` triangle,
+ * op(A) and op(B) n x k (trans_a applies to both).
  */
-enum { MUBLIS_BATCH_GEMM = 0, MUBLIS_BATCH_SYRK = 1 };
+enum { MUBLIS_BATCH_GEMM = 0, MUBLIS_BATCH_SYRK = 1, MUBLIS_BATCH_SYR2K = 2 };
 typedef struct {
   int op;
   int trans_a, trans_b;
@@ -236,6 +251,14 @@ int mublis_grouped_plan_info(mublis_grouped_plan_t plan, double *flops, double *
                              int *nitems, int *tile_hist3);
 /* Release a plan whose executions have completed. */
 int mublis_grouped_plan_destroy(mublis_grouped_plan_t plan);
+/* Without a plan or graph: size of the device workspace (and of the pinned
+ * host staging buffer) a launch of these ops needs, and an asynchronous
+ * launch that stages the work list through `host_stage` (pinned, kept
+ * untouched until the launch completes) into `dev_ws`.  Used by the
+ * serving loop, which cannot stall its scheduler on a plan. */
+size_t mublis_sgrouped_ws_bytes(const mublis_sbatch_op_t *ops, int count);
+int mublis_sgrouped_launch_ws(const mublis_sbatch_op_t *ops, int count, void *host_stage,
+                              void *dev_ws, size_t ws_bytes, void *stream);
 /* One-shot: plan, execute, synchronise, destroy. */
 int mublis_sgemm_grouped(const mublis_sbatch_op_t *ops, int count);
 /* Autotuner: time each tile shape per (log2 m, log2 n, log2 k) bucket over
@@ -245,6 +268,108 @@ int mublis_sgemm_grouped(const mublis_sbatch_op_t *ops, int count);
  * returns the number of buckets read. */
 int mublis_grouped_autotune(const char *path, int lo, int hi, int per_bucket, int reps);
 int mublis_grouped_load_table(const char *path);
+
+/* ------------------------------------------------------------------------
+ * Serving: a request loop over device-resident matrices.
+ *
+ * No counterpart in the reference (its L3 calls are synchronous and
+ * per-thread, frame/l3/l3_driver.c:49-256); the requests carry the same
+ * arguments as the cblas_* calls they stand for (include/cblas.h:37-95),
+ * column-major.  A server owns one HBM budget, reserved once at creation:
+ * matrices (handles) and launch workspace are carved out of it.  Requests
+ * are submitted from any thread, wait for admission (a free launch slot,
+ * workspace within the budget, no conflict with earlier requests on the
+ * same matrices), and a scheduler thread groups every admissible FP32
+ * GEMM/SYRK/SYR2K request into one grouped persistent launch; TRSM and
+ * DGEMM requests run as their own launches.  Launches are spread over
+ * several streams, so new requests start while earlier ones still run.
+ * Requests on one matrix keep their submission order.
+ * ---------------------------------------------------------------------- */
+typedef struct mublis_server_s *mublis_server_t;
+typedef struct mublis_matrix_s *mublis_matrix_t;
+
+enum { MUBLIS_F32 = 0, MUBLIS_F64 = 1 };
+enum {
+  MUBLIS_REQ_SGEMM = 0,  /* C := alpha*op(A)*op(B) + beta*C, C m x n, k inner  */
+  MUBLIS_REQ_SSYRK = 1,  /* C := alpha*op(A)*op(A)' + beta*C, uplo of n x n C  */
+  MUBLIS_REQ_SSYR2K = 2, /* C := alpha*(op(A)op(B)' + op(B)op(A)') + beta*C     */
+  MUBLIS_REQ_DGEMM = 3,
+  MUBLIS_REQ_STRSM = 4,  /* B := alpha*op(A)^-1*B (left) or alpha*B*op(A)^-1    */
+  MUBLIS_REQ_DTRSM = 5
+};
+
+typedef struct {
+  size_t hbm_budget;  /* bytes reserved for matrices and workspace (0: 1 GiB)   */
+  int streams;        /* device streams launches are spread over (0: 4)         */
+  int slots_per_stream; /* launches in flight per stream (0: 2)                */
+  int max_group;      /* requests per grouped launch (0: 256)                   */
+  double split_flop;  /* a GEMM-like request above this many flop gets its own
+                         launch, so it does not hold small ones back (0: 2e10;
+                         negative: never split)                                 */
+} mublis_server_config_t;
+
+typedef struct {
+  int op;              /* MUBLIS_REQ_*                                          */
+  int trans_a;         /* MUBLIS_NO_TRANSPOSE / MUBLIS_TRANSPOSE: op(A)         */
+  int trans_b;         /* GEMM: op(B)                                           */
+  int side;            /* TRSM: MUBLIS_LEFT / MUBLIS_RIGHT                       */
+  int uplo;            /* SYRK/SYR2K: triangle of C; TRSM: triangle of A        */
+  int diag;            /* TRSM: MUBLIS_NON_UNIT_DIAG / MUBLIS_UNIT_DIAG          */
+  int m, n, k;         /* as in the cblas call (SYRK/SYR2K: n, k; TRSM: m, n)    */
+  double alpha, beta;  /* converted to float for the S operations               */
+  mublis_matrix_t a, b, c; /* operands (TRSM: A and B, B is overwritten)        */
+} mublis_request_t;
+
+typedef struct {
+  uint64_t submitted, completed, failed;
+  uint64_t launches;         /* device launches issued                         */
+  uint64_t grouped_launches; /* of which grouped FP32 launches                 */
+  uint64_t grouped_requests; /* requests that ran inside grouped launches      */
+  uint64_t max_pending;      /* most requests waiting for admission at once    */
+  uint64_t max_inflight;     /* most launches in flight at once                */
+  double p50_ms, p99_ms, mean_ms, max_ms; /* submit -> completion              */
+  double flop;               /* algorithmic flop of completed requests         */
+  double elapsed_s;          /* first submit -> last completion                */
+  double tflops;             /* flop / elapsed_s / 1e12                        */
+  size_t hbm_budget, hbm_used, hbm_peak; /* bytes                              */
+} mublis_server_stats_t;
+
+/* Create a server on the current device; reserves cfg->hbm_budget bytes
+ * (NULL cfg: defaults).  MUBLIS_ENOMEM if the reservation fails. */
+int mublis_server_create(const mublis_server_config_t *cfg, mublis_server_t *out);
+/* Complete every submitted request, then release everything, handles
+ * included. */
+int mublis_server_destroy(mublis_server_t srv);
+/* A rows x cols column-major matrix in the budget (leading dimension from
+ * mublis_matrix_ld, padded to 64 bytes).  When it does not fit now: wait
+ * for memory to be released if `wait` is non-zero, else MUBLIS_ENOMEM; a
+ * matrix larger than the budget less the launch-workspace reserve is
+ * MUBLIS_ENOMEM at once.  Contents are undefined until written. */
+int mublis_matrix_create(mublis_server_t srv, int dtype, int rows, int cols, int wait,
+                         mublis_matrix_t *out);
+/* Copy from / to host memory (leading dimension ld >= rows); writes wait
+ * for every submitted request on the matrix, reads for every writer. */
+int mublis_matrix_write(mublis_server_t srv, mublis_matrix_t h, const void *host, int64_t ld);
+int mublis_matrix_read(mublis_server_t srv, mublis_matrix_t h, void *host, int64_t ld);
+/* Device address and leading dimension (elements), for callers that fill or
+ * consume the matrix on the device themselves; such work must be complete
+ * (or on the legacy default stream) before a request on the matrix is
+ * submitted, and a consumer must wait for the request first. */
+void *mublis_matrix_device_ptr(mublis_matrix_t h);
+int64_t mublis_matrix_ld(mublis_matrix_t h);
+/* Release the matrix once the requests submitted on it have completed. */
+int mublis_matrix_destroy(mublis_server_t srv, mublis_matrix_t h);
+/* Queue a request; returns at once with a ticket for mublis_wait.  Invalid
+ * requests (shapes beyond the matrices, wrong precision, C aliasing an
+ * input) are rejected here with MUBLIS_EINVAL. */
+int mublis_submit(mublis_server_t srv, const mublis_request_t *req, uint64_t *ticket);
+/* Block until the request completes; returns its status and (optionally)
+ * its latency from submission to completion.  Each ticket is waited once. */
+int mublis_wait(mublis_server_t srv, uint64_t ticket, double *latency_ms);
+/* Counters and the latency distribution of the requests completed since the
+ * server was created or mublis_server_reset_stats was called. */
+int mublis_server_stats(mublis_server_t srv, mublis_server_stats_t *out);
+int mublis_server_reset_stats(mublis_server_t srv);
 
 #ifdef __cplusplus
 }

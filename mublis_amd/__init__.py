@@ -26,11 +26,11 @@ OK, ECONTEXT, ENOMEM, EINVAL, EHIP = 0, 1, 2, 3, 4
 _lib = None
 
 # include/mublis_gfx950.h batched extensions
-BATCH_GEMM, BATCH_SYRK = 0, 1
+BATCH_GEMM, BATCH_SYRK, BATCH_SYR2K = 0, 1, 2
 
 
 class SbatchOp(ctypes.Structure):
-    """mublis_sbatch_op_t: one FP32 GEMM or SYRK problem on device memory."""
+    """mublis_sbatch_op_t: one FP32 GEMM, SYRK or SYR2K problem on device memory."""
     _fields_ = [("op", _I), ("trans_a", _I), ("trans_b", _I), ("uplo", _I),
                 ("m", _I), ("n", _I), ("k", _I), ("alpha", _F), ("beta", _F),
                 ("a", _VP), ("lda", _I), ("b", _VP), ("ldb", _I), ("c", _VP), ("ldc", _I)]
@@ -119,6 +119,34 @@ def _declare(lib):
     lib.mublis_rt_is_device_ptr.restype = _I
     lib.mublis_get_context.argtypes = [_VP]
     lib.mublis_get_context.restype = _I
+    _SZ = ctypes.c_size_t
+    lib.mublis_sgrouped_ws_bytes.argtypes = [_VP, _I]
+    lib.mublis_sgrouped_ws_bytes.restype = _SZ
+    lib.mublis_sgrouped_launch_ws.argtypes = [_VP, _I, _VP, _VP, _SZ, _VP]
+    lib.mublis_sgrouped_launch_ws.restype = _I
+    lib.mublis_gfx950_trsm_ws_bytes.argtypes = [_I]
+    lib.mublis_gfx950_trsm_ws_bytes.restype = _SZ
+    for name, ct in (("mublis_gfx950_dtrsm_ws_dev", _D), ("mublis_gfx950_strsm_ws_dev", _F)):
+        f = getattr(lib, name)
+        f.argtypes = [_I, _I, _I, _I, ct, _VP, _I64, _I64, _VP, _I64, _I64, _VP, _SZ, _VP]
+        f.restype = _I
+    # serving loop
+    for name, args in (("mublis_server_create", [_VP, _VP]),
+                       ("mublis_server_destroy", [_VP]),
+                       ("mublis_matrix_create", [_VP, _I, _I, _I, _I, _VP]),
+                       ("mublis_matrix_write", [_VP, _VP, _VP, _I64]),
+                       ("mublis_matrix_read", [_VP, _VP, _VP, _I64]),
+                       ("mublis_matrix_destroy", [_VP, _VP]),
+                       ("mublis_submit", [_VP, _VP, _VP]),
+                       ("mublis_wait", [_VP, ctypes.c_uint64, _VP]),
+                       ("mublis_server_stats", [_VP, _VP]),
+                       ("mublis_server_reset_stats", [_VP])):
+        f = getattr(lib, name)
+        f.argtypes, f.restype = args, _I
+    lib.mublis_matrix_device_ptr.argtypes = [_VP]
+    lib.mublis_matrix_device_ptr.restype = _VP
+    lib.mublis_matrix_ld.argtypes = [_VP]
+    lib.mublis_matrix_ld.restype = _I64
     return lib
 
 

@@ -154,25 +154,29 @@ int mublis_grouped_load_table(const char *path) {
   return (int)g_tab.size();
 }
 
-int mublis_sgrouped_plan_create(const mublis_sbatch_op_t *ops, int count, int flags,
-                                mublis_grouped_plan_t *out) {
-  if (!ops || count < 0 || !out) return MUBLIS_EINVAL;
-  int st = mublis_rt_init();
-  if (st) return st;
-  auto *plan = new mublis_grouped_plan_s();
+}  // extern "C"
+
+namespace {
+
+struct Work {
   std::vector<GemmArgs<float>> probs;
   std::vector<GroupItem> items;
+  double flops = 0, bytes = 0;
+  int hist[3] = {0, 0, 0};
+};
+
+// Expand ops into problems and a cost-sorted tile list.
+int build_work(const mublis_sbatch_op_t *ops, int count, int flags, Work &w) {
+  std::vector<GroupItem> items;
   std::vector<double> cost;
-  probs.reserve(count);
+  w.probs.reserve(count);
   const int forced = (flags >> 4) & 3;  // MUBLIS_GROUPED_FORCE_TILE(t) stores t + 1
   for (int i = 0; i < count; ++i) {
     const mublis_sbatch_op_t &o = ops[i];
-    const bool syrk = o.op == MUBLIS_BATCH_SYRK;
-    const int m = syrk ? o.n : o.m, n = o.n, k = o.k;
-    if (m < 0 || n < 0 || k < 0 || (!syrk && o.op != MUBLIS_BATCH_GEMM)) {
-      delete plan;
-      return MUBLIS_EINVAL;
-    }
+    const bool syrk = o.op == MUBLIS_BATCH_SYRK, syr2k = o.op == MUBLIS_BATCH_SYR2K;
+    const bool sym = syrk || syr2k;
+    const int m = sym ? o.n : o.m, n = o.n, k = o.k;
+    if (m < 0 || n < 0 || k < 0 || (!sym && o.op != MUBLIS_BATCH_GEMM)) return MUBLIS_EINVAL;
     GemmArgs<float> p;
     memset(&p, 0, sizeof p);
     // alpha == 0 reads neither A nor B (frame/l3/l3_driver.c:123-124): K = 0
@@ -181,9 +185,17 @@ int mublis_sgrouped_plan_create(const mublis_sbatch_op_t *ops, int count, int fl
     // column-major operands; transposes are stride swaps (frame/l3/gemm.c:14-24)
     int64_t rsa = 1, csa = o.lda, rsb = 1, csb = o.ldb;
     if (o.trans_a) std::swap(rsa, csa);
-    if (syrk) {
+    if (sym) {
+      if (syr2k && o.trans_a) std::swap(rsb, csb);
       p.a = o.a; p.rs_a = rsa; p.cs_a = csa;
-      p.b = o.a; p.rs_b = csa; p.cs_b = rsa;  // B = A' (frame/l3/syrk.c:43-51)
+      if (syrk) {
+        p.b = o.a; p.rs_b = csa; p.cs_b = rsa;  // B = A' (frame/l3/syrk.c:43-51)
+      } else {
+        // A*B' and B*A' as the two K segments (frame/l3/syr2k.c:47-68)
+        p.b = o.b; p.rs_b = csb; p.cs_b = rsb;
+        p.a2 = o.b; p.rs_a2 = rsb; p.cs_a2 = csb;
+        p.b2 = o.a; p.rs_b2 = csa; p.cs_b2 = rsa;
+      }
       p.ji = o.uplo == MUBLIS_UPPER ? 2 : 1;
     } else {
       if (o.trans_b) std::swap(rsb, csb);
@@ -193,30 +205,33 @@ int mublis_sgrouped_plan_create(const mublis_sbatch_op_t *ops, int count, int fl
     }
     p.c = o.c; p.rs_c = 1; p.cs_c = o.ldc;
     const bool akc = p.cs_a == 1, bnc = p.rs_b != 1;
-    if ((p.rs_a != 1 && p.cs_a != 1) || (p.rs_b != 1 && p.cs_b != 1)) {
-      delete plan;
-      return MUBLIS_EINVAL;
-    }
+    if ((p.rs_a != 1 && p.cs_a != 1) || (p.rs_b != 1 && p.cs_b != 1)) return MUBLIS_EINVAL;
     p.vec_a = aligned16f(p.a, akc ? p.rs_a : p.cs_a);
     p.vec_b = aligned16f(p.b, bnc ? p.rs_b : p.cs_b);
+    if (syr2k) {
+      p.vec_a2 = aligned16f(p.a2, akc ? p.rs_a2 : p.cs_a2);
+      p.vec_b2 = aligned16f(p.b2, bnc ? p.rs_b2 : p.cs_b2);
+    }
+    if (p.k == 0) p.a2 = p.b2 = nullptr;
     const int tix = forced ? forced - 1 : choose_tile(m, n, k);
     const int T = kTiles[tix];
-    plan->tile_hist[tix]++;
+    w.hist[tix]++;
     const int layout = (akc ? 1 : 0) | (bnc ? 2 : 0);
     const int tmn = (m + T - 1) / T, tnn = (n + T - 1) / T;
-    const int pid = (int)probs.size();
-    probs.push_back(p);
-    plan->flops += syrk ? (double)n * (n + 1) * k : 2.0 * m * n * k;
-    plan->bytes += 4.0 * ((double)m * k + (syrk ? 0.0 : (double)k * n) +
-                          (o.beta != 0.0f ? 2.0 : 1.0) * (syrk ? n * (n + 1) / 2.0
-                                                               : (double)m * n));
+    const int pid = (int)w.probs.size();
+    w.probs.push_back(p);
+    const double tri = n * (n + 1) / 2.0;
+    w.flops += syrk ? (double)n * (n + 1) * k : syr2k ? 2.0 * n * (n + 1) * k
+                                                      : 2.0 * m * n * k;
+    w.bytes += 4.0 * ((syr2k ? 2.0 : 1.0) * m * k + (sym ? 0.0 : (double)k * n) +
+                      (o.beta != 0.0f ? 2.0 : 1.0) * (sym ? tri : (double)m * n));
     if (m == 0 || n == 0) continue;
     for (int tn = 0; tn < tnn; ++tn)
       for (int tm = 0; tm < tmn; ++tm) {
-        if (syrk && p.ji == 1 && tn > tm) continue;  // tile entirely above
-        if (syrk && p.ji == 2 && tm > tn) continue;  // tile entirely below
+        if (sym && p.ji == 1 && tn > tm) continue;  // tile entirely above
+        if (sym && p.ji == 2 && tm > tn) continue;  // tile entirely below
         items.push_back({pid, tm, tn, tix * 4 + layout});
-        cost.push_back((double)T * T * std::max(k, 1) + 64.0 * T * T);
+        cost.push_back((double)T * T * std::max(k, 1) * (syr2k ? 2 : 1) + 64.0 * T * T);
       }
   }
   // largest items first, so the cyclic walk of the persistent grid ends
@@ -224,14 +239,40 @@ int mublis_sgrouped_plan_create(const mublis_sbatch_op_t *ops, int count, int fl
   std::vector<int> order(items.size());
   for (size_t i = 0; i < order.size(); ++i) order[i] = (int)i;
   std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return cost[x] > cost[y]; });
-  std::vector<GroupItem> sorted(items.size());
-  for (size_t i = 0; i < order.size(); ++i) sorted[i] = items[order[i]];
-  plan->nitems = (int)sorted.size();
+  w.items.resize(items.size());
+  for (size_t i = 0; i < order.size(); ++i) w.items[i] = items[order[i]];
+  return MUBLIS_OK;
+}
+
+int persistent_grid(int nitems) {
   int dev = 0, cus = 256;
   if (hipGetDevice(&dev) != hipSuccess ||
       hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
     cus = 256;
-  plan->grid = std::max(1, std::min(plan->nitems, 2 * cus));
+  return std::max(1, std::min(nitems, 2 * cus));
+}
+
+size_t align256(size_t x) { return (x + 255) / 256 * 256; }
+
+}  // namespace
+
+extern "C" {
+
+int mublis_sgrouped_plan_create(const mublis_sbatch_op_t *ops, int count, int flags,
+                                mublis_grouped_plan_t *out) {
+  if (!ops || count < 0 || !out) return MUBLIS_EINVAL;
+  int st = mublis_rt_init();
+  if (st) return st;
+  Work w;
+  if ((st = build_work(ops, count, flags, w))) return st;
+  auto *plan = new mublis_grouped_plan_s();
+  plan->flops = w.flops;
+  plan->bytes = w.bytes;
+  for (int i = 0; i < 3; ++i) plan->tile_hist[i] = w.hist[i];
+  const std::vector<GemmArgs<float>> &probs = w.probs;
+  const std::vector<GroupItem> &sorted = w.items;
+  plan->nitems = (int)sorted.size();
+  plan->grid = persistent_grid(plan->nitems);
   st = MUBLIS_OK;
   if (!probs.empty() && hipMalloc(&plan->d_probs, sizeof(GemmArgs<float>) * probs.size()))
     st = MUBLIS_ENOMEM;
@@ -264,6 +305,35 @@ int mublis_sgrouped_plan_create(const mublis_sbatch_op_t *ops, int count, int fl
   }
   *out = plan;
   return MUBLIS_OK;
+}
+
+size_t mublis_sgrouped_ws_bytes(const mublis_sbatch_op_t *ops, int count) {
+  Work w;
+  if (!ops || count < 0 || build_work(ops, count, 0, w)) return 0;
+  return align256(sizeof(GemmArgs<float>) * w.probs.size()) +
+         align256(sizeof(GroupItem) * w.items.size()) + 256;
+}
+
+int mublis_sgrouped_launch_ws(const mublis_sbatch_op_t *ops, int count, void *host_stage,
+                              void *dev_ws, size_t ws_bytes, void *stream) {
+  if (!ops || count < 0 || !host_stage || !dev_ws) return MUBLIS_EINVAL;
+  Work w;
+  int st = build_work(ops, count, 0, w);
+  if (st) return st;
+  const size_t pb = align256(sizeof(GemmArgs<float>) * w.probs.size());
+  const size_t ib = align256(sizeof(GroupItem) * w.items.size());
+  if (pb + ib > ws_bytes) return MUBLIS_EINVAL;
+  if (w.items.empty()) return MUBLIS_OK;
+  char *h = (char *)host_stage;
+  memcpy(h, w.probs.data(), sizeof(GemmArgs<float>) * w.probs.size());
+  memcpy(h + pb, w.items.data(), sizeof(GroupItem) * w.items.size());
+  hipStream_t s = (hipStream_t)(stream ? stream : mublis_rt_stream());
+  if (hipMemcpyAsync(dev_ws, h, pb + ib, hipMemcpyHostToDevice, s)) return MUBLIS_EHIP;
+  const int grid = persistent_grid((int)w.items.size());
+  hipLaunchKernelGGL(grouped_kernel, dim3(grid), dim3(NT), 0, s,
+                     (const GemmArgs<float> *)dev_ws,
+                     (const GroupItem *)((char *)dev_ws + pb), (int)w.items.size());
+  return hipGetLastError() == hipSuccess ? MUBLIS_OK : MUBLIS_EHIP;
 }
 
 int mublis_grouped_plan_execute(mublis_grouped_plan_t plan, void *stream) {

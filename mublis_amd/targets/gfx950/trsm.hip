@@ -237,17 +237,26 @@ int trsm_rec(bool lower, const double *ad, int m, int n, T alpha, const T *a,
   return trsm_rec<T>(lower, ad1, m1, n, T(1), a11, rs_a, cs_a, b1, rs_b, cs_b, s);
 }
 
-// Prepare all diagonal blocks, run the recursive schedule, release the
-// workspace once the stream has drained.
+// Prepare all diagonal blocks into `ad` (ws_doubles(m) doubles) and run the
+// recursive schedule; asynchronous on s.
+int64_t ws_doubles(int m) { return (int64_t)((m + NB - 1) / NB) * BLK; }
+
 template <typename T>
-int solve(bool lower, int diag_mode, int m, int n, T alpha, const T *a, int64_t rs_a,
-          int64_t cs_a, T *b, int64_t rs_b, int64_t cs_b, hipStream_t s) {
-  const int blocks = (m + NB - 1) / NB;
-  double *ad = (double *)mublis_rt_alloc(sizeof(double) * (size_t)blocks * BLK);
-  if (!ad) return MUBLIS_ENOMEM;
+int solve_ws(bool lower, int diag_mode, int m, int n, T alpha, const T *a, int64_t rs_a,
+             int64_t cs_a, T *b, int64_t rs_b, int64_t cs_b, double *ad, hipStream_t s) {
   int st = prep<T>(lower, m, a, rs_a, cs_a, diag_mode, ad, s);
   if (st == MUBLIS_OK) st = trsm_rec<T>(lower, ad, m, n, alpha, a, rs_a, cs_a, b, rs_b,
                                         cs_b, s);
+  return st;
+}
+
+// Same with workspace from the arena, released once the stream has drained.
+template <typename T>
+int solve(bool lower, int diag_mode, int m, int n, T alpha, const T *a, int64_t rs_a,
+          int64_t cs_a, T *b, int64_t rs_b, int64_t cs_b, hipStream_t s) {
+  double *ad = (double *)mublis_rt_alloc(sizeof(double) * (size_t)ws_doubles(m));
+  if (!ad) return MUBLIS_ENOMEM;
+  int st = solve_ws<T>(lower, diag_mode, m, n, alpha, a, rs_a, cs_a, b, rs_b, cs_b, ad, s);
   if (hipStreamSynchronize(s) != hipSuccess && st == MUBLIS_OK) st = MUBLIS_EHIP;
   mublis_rt_free(ad);
   return st;
@@ -297,4 +306,36 @@ extern "C" int mublis_gfx950_strsm_dev(int lower, int unit, int m, int n, float 
                                        void *stream) {
   return trsm_dev<float>(lower, unit, m, n, alpha, a, rs_a, cs_a, b, rs_b, cs_b,
                          stream);
+}
+
+extern "C" size_t mublis_gfx950_trsm_ws_bytes(int m) {
+  return m <= 0 ? 0 : sizeof(double) * (size_t)ws_doubles(m);
+}
+
+template <typename T>
+static int trsm_ws_dev(int lower, int unit, int m, int n, T alpha, const T *a, int64_t rs_a,
+                       int64_t cs_a, T *b, int64_t rs_b, int64_t cs_b, void *ws,
+                       size_t ws_bytes, void *stream) {
+  if (m < 0 || n < 0) return MUBLIS_EINVAL;
+  if (m == 0 || n == 0) return MUBLIS_OK;
+  hipStream_t s = (hipStream_t)(stream ? stream : mublis_rt_stream());
+  if (alpha == T(0)) return gfx950_fill<T>(m, n, T(0), b, rs_b, cs_b, s);
+  if (!ws || ws_bytes < mublis_gfx950_trsm_ws_bytes(m)) return MUBLIS_EINVAL;
+  return solve_ws<T>(lower != 0, unit ? 1 : 0, m, n, alpha, a, rs_a, cs_a, b, rs_b, cs_b,
+                     (double *)ws, s);
+}
+
+extern "C" int mublis_gfx950_dtrsm_ws_dev(int lower, int unit, int m, int n, double alpha,
+                                          const double *a, int64_t rs_a, int64_t cs_a,
+                                          double *b, int64_t rs_b, int64_t cs_b, void *ws,
+                                          size_t ws_bytes, void *stream) {
+  return trsm_ws_dev<double>(lower, unit, m, n, alpha, a, rs_a, cs_a, b, rs_b, cs_b, ws,
+                             ws_bytes, stream);
+}
+extern "C" int mublis_gfx950_strsm_ws_dev(int lower, int unit, int m, int n, float alpha,
+                                          const float *a, int64_t rs_a, int64_t cs_a, float *b,
+                                          int64_t rs_b, int64_t cs_b, void *ws, size_t ws_bytes,
+                                          void *stream) {
+  return trsm_ws_dev<float>(lower, unit, m, n, alpha, a, rs_a, cs_a, b, rs_b, cs_b, ws,
+                            ws_bytes, stream);
 }

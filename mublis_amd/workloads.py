@@ -7,6 +7,8 @@ entries uniform[-1, 1], column-major device tensors, plus the
 (shape (cols, ld)) so that a row-major torch tensor is a column-major
 matrix.
 """
+import math
+
 import numpy as np
 
 from . import BATCH_GEMM, BATCH_SYRK, SbatchOp
@@ -58,3 +60,73 @@ def grouped_problems(torch, seed=314159, n_gemm=4096, n_syrk=1024, gemm_range=(1
         keep += [A, C]
         specs.append(dict(op="syrk", n=n, k=k, trans=tr, upper=up, A=A, C=C, lda=A.shape[1]))
     return ops, keep, specs
+
+
+def _log_uniform(rng, lo, hi):
+    return int(round(math.exp(rng.uniform(math.log(lo), math.log(hi)))))
+
+
+def serving_requests(torch, lib, server, seed=2024, count=2000, alpha=1.0, beta=0.5):
+    """The mixed-L3 serving workload on device-resident matrices.
+
+    60% SGEMM (M, N, K log-uniform [64, 4096]), 20% SSYRK/SSYR2K (half each;
+    n log-uniform [128, 4096], k log-uniform [64, 2048], triangle chosen at
+    random), 20% DTRSM (Left, Lower, NoTrans, NonUnit; m log-uniform
+    [256, 4096], nrhs log-uniform [1, 512], A diagonally dominant: diagonal
+    m, off-diagonal U[-1, 1]).  Entries U[-1, 1], filled on the device from a
+    shared random pool.  Returns (requests, matrices): each request is a dict
+    of mublis_submit keyword arguments plus "op" and "flop".
+    """
+    from . import serving as S
+    import random
+    rng = random.Random(seed)
+    gen = torch.Generator(device="cuda").manual_seed(seed)
+    n_pool = 1 << 25  # covers the largest operand (4096 x 4096)
+    pools = {S.F32: torch.rand(n_pool, device="cuda", generator=gen) * 2 - 1,
+             S.F64: torch.rand(n_pool, device="cuda", generator=gen,
+                               dtype=torch.float64) * 2 - 1}
+    copy = {S.F32: lib.mublis_gfx950_scopy_dev, S.F64: lib.mublis_gfx950_dcopy_dev}
+    mats = []
+
+    def matrix(dt, rows, cols):
+        h = server.matrix(dt, rows, cols)
+        pool = pools[dt]
+        off = rng.randrange(0, n_pool - rows * cols + 1)
+        addr = pool.data_ptr() + off * pool.element_size()
+        st = copy[dt](rows, cols, addr, 1, rows, h.ptr, 1, h.ld, None)
+        assert st == 0, st
+        mats.append(h)
+        return h
+
+    reqs, keep = [], []
+    for _ in range(count):
+        u = rng.random()
+        if u < 0.6:
+            m, n, k = (_log_uniform(rng, 64, 4096) for _ in range(3))
+            r = dict(op=S.SGEMM, m=m, n=n, k=k, alpha=alpha, beta=beta,
+                     a=matrix(S.F32, m, k), b=matrix(S.F32, k, n), c=matrix(S.F32, m, n))
+            r["flop"] = 2.0 * m * n * k
+        elif u < 0.8:
+            n, k = _log_uniform(rng, 128, 4096), _log_uniform(rng, 64, 2048)
+            two = rng.random() < 0.5
+            uplo = S.LOWER if rng.random() < 0.5 else S.UPPER
+            r = dict(op=S.SSYR2K if two else S.SSYRK, n=n, k=k, alpha=alpha, beta=beta,
+                     uplo=uplo, a=matrix(S.F32, n, k), c=matrix(S.F32, n, n))
+            if two:
+                r["b"] = matrix(S.F32, n, k)
+            r["flop"] = (2.0 if two else 1.0) * n * (n + 1) * k
+        else:
+            m, nrhs = _log_uniform(rng, 256, 4096), _log_uniform(rng, 1, 512)
+            a = matrix(S.F64, m, m)
+            diag = torch.full((m,), float(m), dtype=torch.float64, device="cuda")
+            st = lib.mublis_gfx950_dcopy_dev(1, m, diag.data_ptr(), 1, 1, a.ptr, 1, a.ld + 1,
+                                             None)
+            assert st == 0, st
+            keep.append(diag)  # alive until the copies have run
+            r = dict(op=S.DTRSM, m=m, n=nrhs, alpha=alpha, uplo=S.LOWER, a=a,
+                     b=matrix(S.F64, m, nrhs))
+            r["flop"] = float(m) * m * nrhs
+        reqs.append(r)
+    lib.mublis_rt_sync(None)
+    torch.cuda.synchronize()
+    return reqs, mats
