@@ -31,7 +31,7 @@ PEAK = {"f64": 78.6, "f32": 157.3}  # dense MFMA TFLOP/s, MI355X
 METRIC = ("TFLOP/s & %MFMA peak, D/SGEMM+DTRSM n=8192; p50/p99 mixed-L3 request "
           "latency")
 COL, ROW, NOTR, TR, UP, LO, NONUNIT, LEFT = 102, 101, 111, 112, 121, 122, 131, 141
-SERVE_RATE = 8000.0  # requests/s of the serving workload's Poisson arrivals
+SERVE_RATE = 16000.0  # requests/s of the serving workload's Poisson arrivals
 
 
 def parse():
@@ -393,9 +393,12 @@ def run_serving(args, torch, M, lib, extra):
         st, lag = serve_run(S, srv, reqs, arrivals)
         fifo = 1e3 * fifo_latencies(arrivals, service)
         burst, _ = serve_run(S, srv, reqs, np.zeros(len(reqs)))
-        mix = {}
-        for r in reqs:
-            mix[S.OP_NAMES[r["op"]]] = mix.get(S.OP_NAMES[r["op"]], 0) + 1
+        mix, by_op = {}, {}
+        for r, d in zip(reqs, service):
+            name = S.OP_NAMES[r["op"]]
+            mix[name] = mix.get(name, 0) + 1
+            f, t = by_op.get(name, (0.0, 0.0))
+            by_op[name] = (f + r["flop"], t + d)
         extra["serving_mixed_l3"] = {
             "requests": len(reqs), "mix": mix, "arrival_rate_per_s": args.serve_rate,
             "tflops": round(st["tflops"], 3),
@@ -404,6 +407,8 @@ def run_serving(args, torch, M, lib, extra):
             "elapsed_s": round(st["elapsed_s"], 4),
             "serial_cblas_tflops": round(flop / serial_s / 1e12, 3),
             "serial_cblas_s": round(serial_s, 4),
+            "serial_s_and_tflops_by_op": {k: [round(t, 4), round(f / t / 1e12, 2)]
+                                          for k, (f, t) in by_op.items()},
             "serial_fifo_same_arrivals_p50_ms": round(float(np.percentile(fifo, 50)), 3),
             "serial_fifo_same_arrivals_p99_ms": round(float(np.percentile(fifo, 99)), 3),
             "throughput_vs_serial": round(st["tflops"] / (flop / serial_s / 1e12), 2),

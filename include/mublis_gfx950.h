@@ -125,6 +125,12 @@ int mublis_gfx950_strsm_inverted_dev(int lower, int m, int n, const float *a,
  * bytes of device memory), fully asynchronous on `stream`: the form the
  * serving loop uses so its scheduler never waits on a solve. */
 size_t mublis_gfx950_trsm_ws_bytes(int m);
+/* Widest B (columns after the side transformation) that the left-side
+ * solves hand to the pipelined GEMMTRSM kernel (one launch, block rows
+ * chained by device-side flags) instead of the recursive schedule; set it
+ * with max_n >= 0 (at most 4096), query with max_n < 0.  Returns the
+ * previous value.  Tuning knob; the frame's calls share it. */
+int mublis_gfx950_trsm_pipe_max_n(int max_n);
 int mublis_gfx950_dtrsm_ws_dev(int lower, int unit, int m, int n, double alpha,
                                const double *a, int64_t rs_a, int64_t cs_a, double *b,
                                int64_t rs_b, int64_t cs_b, void *ws, size_t ws_bytes,

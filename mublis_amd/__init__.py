@@ -126,6 +126,8 @@ def _declare(lib):
     lib.mublis_sgrouped_launch_ws.restype = _I
     lib.mublis_gfx950_trsm_ws_bytes.argtypes = [_I]
     lib.mublis_gfx950_trsm_ws_bytes.restype = _SZ
+    lib.mublis_gfx950_trsm_pipe_max_n.argtypes = [_I]
+    lib.mublis_gfx950_trsm_pipe_max_n.restype = _I
     for name, ct in (("mublis_gfx950_dtrsm_ws_dev", _D), ("mublis_gfx950_strsm_ws_dev", _F)):
         f = getattr(lib, name)
         f.argtypes = [_I, _I, _I, _I, ct, _VP, _I64, _I64, _VP, _I64, _I64, _VP, _SZ, _VP]

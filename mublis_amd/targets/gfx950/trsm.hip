@@ -4,14 +4,20 @@
 //
 // The reference walks kc-row diagonal blocks, solving each with the fused
 // GEMMTRSM micro-kernel and pushing a rank-kc GEMM update through the rest
-// of B (frame/l3/trsm.c:91-431).  On the GPU a rank-64 update of all of B
-// per block would re-stream B from HBM m/64 times, so the schedule is
-// recursive instead: split A in halves, solve the first, update the second
-// half of B with ONE large MFMA GEMM (K = m/2), recurse.  All but O(m * NB * n)
-// of the flops run in big GEMMs; the NB x NB diagonal blocks are solved by
-// a leaf kernel in FP64 (also for float data), one lane per column of B,
-// multiplying by reciprocal diagonals like the reference's GEMMTRSM
-// (targets/reference/gemmtrsm_ukr.inc:29-40, frame/l1m/packm.c:143-144).
+// of B (frame/l3/trsm.c:91-431).  Two GPU schedules replace that loop:
+//
+// * wide B: recursive -- split A in halves, solve the first, update the
+//   second half of B with ONE large MFMA GEMM (K = m/2), recurse.  All but
+//   O(m * NB * n) of the flops run in big GEMMs; a NB-row leaf multiplies
+//   its rows of B by the inverted diagonal block (FP64 MFMA).
+// * narrow B: one pipelined launch over block rows (below).
+//
+// Both start from the same prep launch, which inverts every NB x NB
+// diagonal block in FP64 (also for float data), in parallel.  The
+// reference's micro-kernel multiplies by reciprocal diagonals
+// (targets/reference/gemmtrsm_ukr.inc:29-40, frame/l1m/packm.c:143-144);
+// here the whole diagonal block is inverted once, so the dependent part of
+// every leaf is one small matrix product instead of an NB-step substitution.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -21,190 +27,328 @@
 
 namespace {
 
-constexpr int NB = 64;  // diagonal block (leaf) size = columns per leaf workgroup
+constexpr int NB = 64;  // diagonal block size
+// Workspace per diagonal block, in doubles: its FP64 inverse plus room for
+// the pipeline's per-block flags.
+constexpr int BLK = NB * NB + NB;
 
-// diag_mode: 0 read the diagonal and use its reciprocal, 1 unit (never read),
-// 2 the stored diagonal is already the reciprocal (GEMMTRSM micro-kernel
-// contract, include/mublis_instantiate.h:84-86).
+// diag_mode: 0 read the diagonal, 1 unit (never read), 2 the stored diagonal
+// is already the reciprocal (GEMMTRSM micro-kernel contract,
+// include/mublis_instantiate.h:84-86).
+
+// ---------------------------------------------------------------------------
+// Pipelined GEMMTRSM for narrow right-hand sides.
 //
-// Prep: every NB x NB diagonal block of A in FP64, column-major, with zeros
-// outside the triangle AND on the diagonal, followed by the NB reciprocal
-// diagonal entries (1 for UNIT, never read) -- the device counterpart of
-// packm_trsm_a (frame/l1m/packm.c:114-157), for all blocks in one launch.
-// Only the stored triangle of A is read.
-constexpr int BLK = NB * NB + NB;  // doubles per prepared block
+// With few columns in B the recursive schedule is a chain of ~2 m / NB small
+// launches, each too narrow to fill the GPU.  Here one launch solves the
+// whole system: workgroup (j, c) owns block row j (in solve order) of
+// column group c.  It accumulates the update sum_{i<j} A(j, i) X(i) block by
+// block as soon as X(i) is published, then forms X(j) = inv(A(j, j)) *
+// (alpha B(j) - update) and publishes it through a per-block flag -- the
+// fused GEMM + triangular-solve step of the reference's GEMMTRSM
+// micro-kernel (targets/reference/gemmtrsm_ukr.inc:1-45), pipelined across
+// block rows instead of looped.  A workgroup only waits for flags of
+// workgroups with lower linear ids, which the dispatcher starts first, so
+// every wait ends; the waits are also bounded, and a timed-out wait raises
+// an abort flag that releases all others.  Updates and the inverse product
+// run in FP64 MFMA, also for float data; the inverted diagonal blocks are
+// computed in FP64 by one prep launch that also clears the flags.
+constexpr int PB = NB;  // block rows of the pipeline
 
 template <typename T, bool LOWER>
-__global__ __launch_bounds__(256) void trsm_prep(int m, const T *__restrict__ a,
-                                                 int64_t rs_a, int64_t cs_a,
-                                                 int diag_mode,
-                                                 double *__restrict__ out) {
-  const int blk = blockIdx.x;
-  const int base = blk * NB;
-  const int nb = min(NB, m - base);
-  double *o = out + (int64_t)blk * BLK;
-  for (int t = threadIdx.x; t < NB * NB; t += blockDim.x) {
-    const int p = t / NB, i = t % NB;  // o[p*NB + i] = A(base+i, base+p)
+__global__ __launch_bounds__(64) void trsm_inv_prep(int m, const T *__restrict__ a,
+                                                    int64_t rs_a, int64_t cs_a,
+                                                    int diag_mode,
+                                                    double *__restrict__ inv,
+                                                    int *__restrict__ flags, int nflags) {
+  __shared__ double L[PB][PB + 1];
+  __shared__ double X[PB][PB + 1];
+  __shared__ double rd[PB];
+  const int p = blockIdx.x, c = threadIdx.x;
+  for (int i = p * PB + c; i < nflags; i += gridDim.x * PB) flags[i] = 0;
+  const int base = p * PB;
+  const int nbr = min(PB, m - base);
+  for (int r = 0; r < PB; ++r) {
     double v = 0.0;
-    if (i < nb && p < nb && (LOWER ? (i > p) : (i < p)))
-      v = (double)a[(int64_t)(base + i) * rs_a + (int64_t)(base + p) * cs_a];
-    o[t] = v;
+    if (r < nbr && c < nbr && (LOWER ? r > c : r < c))
+      v = (double)a[(int64_t)(base + r) * rs_a + (int64_t)(base + c) * cs_a];
+    L[r][c] = v;
   }
-  for (int i = threadIdx.x; i < NB; i += blockDim.x) {
-    double v = 0.0;
-    if (i < nb) {
-      if (diag_mode == 1) v = 1.0;
-      else {
-        const int64_t g = base + i;
-        const double d = (double)a[g * rs_a + g * cs_a];
-        v = diag_mode == 2 ? d : 1.0 / d;
-      }
-    }
-    o[NB * NB + i] = v;
-  }
-}
-
-// Broadcast lane `src` of every group of 4 lanes to the whole group
-// (DPP quad permute; `src` must be a constant after unrolling).
-__device__ inline double quad_bcast(double v, int src) {
-  int lo = __double2loint(v), hi = __double2hiint(v);
-  switch (src) {
-    case 0:
-      lo = __builtin_amdgcn_update_dpp(lo, lo, 0x00, 0xF, 0xF, false);
-      hi = __builtin_amdgcn_update_dpp(hi, hi, 0x00, 0xF, 0xF, false);
-      break;
-    case 1:
-      lo = __builtin_amdgcn_update_dpp(lo, lo, 0x55, 0xF, 0xF, false);
-      hi = __builtin_amdgcn_update_dpp(hi, hi, 0x55, 0xF, 0xF, false);
-      break;
-    case 2:
-      lo = __builtin_amdgcn_update_dpp(lo, lo, 0xAA, 0xF, 0xF, false);
-      hi = __builtin_amdgcn_update_dpp(hi, hi, 0xAA, 0xF, 0xF, false);
-      break;
-    default:
-      lo = __builtin_amdgcn_update_dpp(lo, lo, 0xFF, 0xF, 0xF, false);
-      hi = __builtin_amdgcn_update_dpp(hi, hi, 0xFF, 0xF, 0xF, false);
-      break;
-  }
-  return __hiloint2double(hi, lo);
-}
-
-// Leaf: solve one prepared NB x NB block against LC = 16 columns of B per
-// workgroup of 64 lanes, in FP64.  Lane 4c + q owns column c and rows
-// 16q .. 16q + 15.  The substitution runs column by column: x_p is finished
-// by its owner lane (times the reciprocal diagonal), broadcast to the other
-// three lanes of the column with a DPP quad permute, and every lane then
-// updates its 16 rows with 16 independent FMAs (rows the triangle does not
-// reach have a zero coefficient).  The B tile moves through LDS so global
-// accesses run along B's unit stride.
-constexpr int LC = 16;
-
-template <typename T, bool LOWER>
-__global__ __launch_bounds__(64) void trsm_leaf(int nb, int n, T alpha,
-                                                const double *__restrict__ blk,
-                                                T *__restrict__ b, int64_t rs_b,
-                                                int64_t cs_b) {
-  __shared__ __attribute__((aligned(16))) double as[NB * NB];
-  __shared__ double dinv[NB];
-  __shared__ double xs[NB][LC + 1];
-  const int tid = threadIdx.x;
   {
-    // 32 independent 16-byte loads per lane in flight, then the LDS stores
-    typedef double d2 __attribute__((ext_vector_type(2)));
-    const d2 *src = reinterpret_cast<const d2 *>(blk);
-    d2 *dst = reinterpret_cast<d2 *>(as);
-    d2 tmp[NB * NB / 2 / 64];
-#pragma unroll
-    for (int u = 0; u < NB * NB / 2 / 64; ++u) tmp[u] = src[u * 64 + tid];
-#pragma unroll
-    for (int u = 0; u < NB * NB / 2 / 64; ++u) dst[u * 64 + tid] = tmp[u];
-  }
-  dinv[tid] = blk[NB * NB + tid];
-  const int j0 = blockIdx.x * LC;
-  if (rs_b == 1) {
-#pragma unroll
-    for (int c = 0; c < LC; ++c) {
-      double v = 0.0;
-      if (tid < nb && j0 + c < n) v = (double)b[(int64_t)tid + (int64_t)(j0 + c) * cs_b];
-      xs[tid][c] = v;
+    double v = 1.0;  // rows past m: identity
+    if (c < nbr && diag_mode != 1) {
+      const int64_t g = base + c;
+      const double d = (double)a[g * rs_a + g * cs_a];
+      v = diag_mode == 2 ? d : 1.0 / d;
     }
-  } else {
-#pragma unroll
-    for (int it = 0; it < NB * LC / 64; ++it) {
-      const int lin = it * 64 + tid, i = lin / LC, c = lin % LC;
-      double v = 0.0;
-      if (i < nb && j0 + c < n)
-        v = (double)b[(int64_t)i * rs_b + (int64_t)(j0 + c) * cs_b];
-      xs[i][c] = v;
-    }
+    rd[c] = v;
   }
   __syncthreads();
-  const int c = tid >> 2, q = tid & 3;
-  const double al = (double)alpha;
-  double x[16];
-#pragma unroll
-  for (int r = 0; r < 16; ++r) x[r] = al * xs[16 * q + r][c];
-#pragma unroll
-  for (int s = 0; s < NB; ++s) {
-    const int p = LOWER ? s : NB - 1 - s;
-    const int qp = p >> 4, rp = p & 15;
-    const double xp = quad_bcast(x[rp] * dinv[p], qp);
-    x[rp] = (q == qp) ? xp : x[rp];
-    const double *colp = &as[p * NB + 16 * q];
-#pragma unroll
-    for (int r = 0; r < 16; ++r) x[r] = fma(-colp[r], xp, x[r]);
-  }
-#pragma unroll
-  for (int r = 0; r < 16; ++r) xs[16 * q + r][c] = x[r];
-  __syncthreads();
-  if (rs_b == 1) {
-#pragma unroll
-    for (int cc = 0; cc < LC; ++cc)
-      if (tid < nb && j0 + cc < n)
-        b[(int64_t)tid + (int64_t)(j0 + cc) * cs_b] = (T)xs[tid][cc];
+  // Lane c computes column c of the inverse by substitution.  The loops are
+  // uniform across lanes: X[k][c] is exactly 0 outside the triangle, so
+  // every lane reads the same L[r][k] (an LDS broadcast).
+  if (LOWER) {
+    for (int r = 0; r < PB; ++r) {
+      double s = r == c ? 1.0 : 0.0;
+      for (int k = 0; k < r; ++k) s = fma(-L[r][k], X[k][c], s);
+      X[r][c] = s * rd[r];
+    }
   } else {
-#pragma unroll
-    for (int it = 0; it < NB * LC / 64; ++it) {
-      const int lin = it * 64 + tid, i = lin / LC, cc = lin % LC;
-      if (i < nb && j0 + cc < n)
-        b[(int64_t)i * rs_b + (int64_t)(j0 + cc) * cs_b] = (T)xs[i][cc];
+    for (int r = PB - 1; r >= 0; --r) {
+      double s = r == c ? 1.0 : 0.0;
+      for (int k = r + 1; k < PB; ++k) s = fma(-L[r][k], X[k][c], s);
+      X[r][c] = s * rd[r];
+    }
+  }
+  double *o = inv + (int64_t)p * PB * PB;  // row-major
+  for (int r = 0; r < PB; ++r) o[r * PB + c] = X[r][c];
+}
+
+__device__ inline void wait_flag(int *f, int *abort_flag) {
+  int spins = 0;
+  while (__hip_atomic_load(f, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == 0) {
+    if (__hip_atomic_load(abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
+    __builtin_amdgcn_s_sleep(2);
+    if (++spins > (1 << 20)) {
+      __hip_atomic_store(abort_flag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      return;
     }
   }
 }
 
+// Registers <-> block row `blk` x columns c0 .. c0+CW of B, along B's unit
+// stride (FP64; zero outside the matrix).  Element e = it * 256 + tid of the
+// tile is (r, cc) = (e % PB, e / PB) when rs_b == 1, else (e / CW, e % CW).
+template <int CW>
+__device__ inline void tile_rc(int e, bool colmajor, int &r, int &cc) {
+  if (colmajor) { r = e % PB; cc = e / PB; }
+  else { r = e / CW; cc = e % CW; }
+}
+
+template <typename T, int CW>
+__device__ inline void load_tile(double (&v)[PB * CW / 256], const T *b, int64_t rs_b,
+                                 int64_t cs_b, int m, int n, int blk, int c0) {
+  const bool cm = rs_b == 1;
+#pragma unroll
+  for (int it = 0; it < PB * CW / 256; ++it) {
+    int r, cc;
+    tile_rc<CW>(it * 256 + (int)threadIdx.x, cm, r, cc);
+    const int gr = blk * PB + r, gc = c0 + cc;
+    v[it] = (gr < m && gc < n) ? (double)b[(int64_t)gr * rs_b + (int64_t)gc * cs_b] : 0.0;
+  }
+}
+
+template <int CW>
+__device__ inline void tile_to_lds(const double (&v)[PB * CW / 256], double (*xs)[CW + 1],
+                                   bool cm) {
+#pragma unroll
+  for (int it = 0; it < PB * CW / 256; ++it) {
+    int r, cc;
+    tile_rc<CW>(it * 256 + (int)threadIdx.x, cm, r, cc);
+    xs[r][cc] = v[it];
+  }
+}
+
+// This lane's 16 values of A(p rows, block q columns) as the MFMA A operand.
+template <typename T>
+__device__ inline void load_ablk(double (&av)[16], const T *a, int64_t rs_a, int64_t cs_a,
+                                 int m, int arow, int q, int lk) {
+#pragma unroll
+  for (int kk = 0; kk < 16; ++kk) {
+    const int col = q * PB + 4 * kk + lk;
+    av[kk] = (arow < m && col < m) ? (double)a[(int64_t)arow * rs_a + (int64_t)col * cs_a]
+                                   : 0.0;
+  }
+}
+
+template <typename T, bool LOWER, int CW>
+__global__ __launch_bounds__(256) void trsm_pipe(int m, int n, T alpha,
+                                                 const T *__restrict__ a, int64_t rs_a,
+                                                 int64_t cs_a, T *b, int64_t rs_b,
+                                                 int64_t cs_b, const double *__restrict__ inv,
+                                                 int *flags, int cg) {
+  typedef double d4 __attribute__((ext_vector_type(4)));
+  constexpr int TN = CW / 16;          // 16-column MFMA tiles per wave
+  constexpr int TV = PB * CW / 256;    // tile elements per thread
+  __shared__ double xs[2][PB][CW + 1];
+  __shared__ int s_front;
+  const int nbk = (m + PB - 1) / PB;
+  const int j = blockIdx.x / cg, cgi = blockIdx.x % cg;
+  const int p = LOWER ? j : nbk - 1 - j;  // physical block row
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int lr = lane & 15, lk = lane >> 4;
+  const int c0 = cgi * CW;
+  const bool cm = rs_b == 1;
+  const int arow = p * PB + 16 * wave + lr;  // row of A / inv this lane feeds
+  int *abort_flag = flags + nbk * cg;
+  // independent of the other block rows: this block's right-hand side and
+  // inverse rows, fetched before any wait
+  double breg[TV], ireg[16];
+  load_tile<T, CW>(breg, b, rs_b, cs_b, m, n, p, c0);
+  {
+    const double *ip = inv + (int64_t)p * PB * PB + (16 * wave + lr) * PB;
+#pragma unroll
+    for (int kk = 0; kk < 16; ++kk) ireg[kk] = ip[4 * kk + lk];
+  }
+  d4 acc[TN];
+#pragma unroll
+  for (int t = 0; t < TN; ++t) acc[t] = d4{0, 0, 0, 0};
+  double av[16], xr[TV];
+  int i = 0;
+  while (i < j) {
+    // A(p, q(i)) before the wait; then the run of published blocks i .. f-1
+    load_ablk<T>(av, a, rs_a, cs_a, m, arow, LOWER ? i : nbk - 1 - i, lk);
+    if (tid == 0) {
+      wait_flag(&flags[i * cg + cgi], abort_flag);
+      int f = i + 1;
+      while (f < j &&
+             __hip_atomic_load(&flags[f * cg + cgi], __ATOMIC_ACQUIRE,
+                               __HIP_MEMORY_SCOPE_AGENT) != 0)
+        ++f;
+      s_front = f;
+    }
+    __syncthreads();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    const int f = s_front;
+    load_tile<T, CW>(xr, b, rs_b, cs_b, m, n, LOWER ? i : nbk - 1 - i, c0);
+    for (; i < f; ++i) {
+      const int buf = i & 1;
+      tile_to_lds<CW>(xr, xs[buf], cm);
+      double ac[16];
+#pragma unroll
+      for (int kk = 0; kk < 16; ++kk) ac[kk] = av[kk];
+      __syncthreads();
+      if (i + 1 < f) {  // next block of the run, in flight during the MFMAs
+        const int q1 = LOWER ? i + 1 : nbk - 2 - i;
+        load_tile<T, CW>(xr, b, rs_b, cs_b, m, n, q1, c0);
+        load_ablk<T>(av, a, rs_a, cs_a, m, arow, q1, lk);
+      }
+#pragma unroll
+      for (int kk = 0; kk < 16; ++kk)
+#pragma unroll
+        for (int t = 0; t < TN; ++t)
+          acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(ac[kk], xs[buf][4 * kk + lk][16 * t + lr],
+                                                        acc[t], 0, 0, 0);
+    }
+    __syncthreads();  // the next run reuses both buffers
+  }
+  // R = alpha * B(p) - update, in xs[0] (FP64 MFMA output row of register r
+  // in lane group g is g + 4 r)
+  tile_to_lds<CW>(breg, xs[0], cm);
+  __syncthreads();
+  const double al = (double)alpha;
+#pragma unroll
+  for (int t = 0; t < TN; ++t)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      double &x = xs[0][16 * wave + lk + 4 * r][16 * t + lr];
+      x = al * x - acc[t][r];
+    }
+  __syncthreads();
+  d4 out[TN];
+#pragma unroll
+  for (int t = 0; t < TN; ++t) out[t] = d4{0, 0, 0, 0};
+#pragma unroll
+  for (int kk = 0; kk < 16; ++kk)
+#pragma unroll
+    for (int t = 0; t < TN; ++t)
+      out[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(ireg[kk], xs[0][4 * kk + lk][16 * t + lr],
+                                                    out[t], 0, 0, 0);
+#pragma unroll
+  for (int t = 0; t < TN; ++t)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) xs[1][16 * wave + lk + 4 * r][16 * t + lr] = out[t][r];
+  __syncthreads();
+  // store X(p) along B's unit stride, then publish it
+#pragma unroll
+  for (int it = 0; it < TV; ++it) {
+    int r, cc;
+    tile_rc<CW>(it * 256 + tid, cm, r, cc);
+    const int gr = p * PB + r, gc = c0 + cc;
+    if (gr < m && gc < n) b[(int64_t)gr * rs_b + (int64_t)gc * cs_b] = (T)xs[1][r][cc];
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+  __syncthreads();
+  if (tid == 0)
+    __hip_atomic_store(&flags[j * cg + cgi], 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Recursive-schedule leaf: X = inv(A(0,0)) * alpha * B for the nb <= NB
+// rows of a sub-problem, CW = 64 columns of B per workgroup.
+template <typename T>
+__global__ __launch_bounds__(256) void trsm_inv_apply(int nb, int n, T alpha,
+                                                      const double *__restrict__ inv, T *b,
+                                                      int64_t rs_b, int64_t cs_b) {
+  typedef double d4 __attribute__((ext_vector_type(4)));
+  constexpr int CW = 64, TN = CW / 16, TV = PB * CW / 256;
+  __shared__ double xs[PB][CW + 1];
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int lr = lane & 15, lk = lane >> 4;
+  const int c0 = blockIdx.x * CW;
+  const bool cm = rs_b == 1;
+  double ireg[16], v[TV];
+  const double *ip = inv + (16 * wave + lr) * PB;
+#pragma unroll
+  for (int kk = 0; kk < 16; ++kk) ireg[kk] = ip[4 * kk + lk];
+  load_tile<T, CW>(v, b, rs_b, cs_b, nb, n, 0, c0);
+  const double al = (double)alpha;
+#pragma unroll
+  for (int it = 0; it < TV; ++it) v[it] *= al;
+  tile_to_lds<CW>(v, xs, cm);
+  __syncthreads();
+  d4 out[TN];
+#pragma unroll
+  for (int t = 0; t < TN; ++t) out[t] = d4{0, 0, 0, 0};
+#pragma unroll
+  for (int kk = 0; kk < 16; ++kk)
+#pragma unroll
+    for (int t = 0; t < TN; ++t)
+      out[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(ireg[kk], xs[4 * kk + lk][16 * t + lr],
+                                                    out[t], 0, 0, 0);
+  __syncthreads();
+#pragma unroll
+  for (int t = 0; t < TN; ++t)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) xs[16 * wave + lk + 4 * r][16 * t + lr] = out[t][r];
+  __syncthreads();
+#pragma unroll
+  for (int it = 0; it < TV; ++it) {
+    int r, cc;
+    tile_rc<CW>(it * 256 + tid, cm, r, cc);
+    const int gc = c0 + cc;
+    if (r < nb && gc < n) b[(int64_t)r * rs_b + (int64_t)gc * cs_b] = (T)xs[r][cc];
+  }
+}
+
+// Inverted diagonal blocks (NB * NB doubles each, row-major) into `inv`.
 template <typename T>
 int prep(bool lower, int m, const T *a, int64_t rs_a, int64_t cs_a, int diag_mode,
-         double *out, hipStream_t s) {
+         double *inv, hipStream_t s) {
   const int blocks = (m + NB - 1) / NB;
   if (lower)
-    hipLaunchKernelGGL((trsm_prep<T, true>), dim3(blocks), dim3(256), 0, s, m, a, rs_a,
-                       cs_a, diag_mode, out);
+    hipLaunchKernelGGL((trsm_inv_prep<T, true>), dim3(blocks), dim3(NB), 0, s, m, a, rs_a,
+                       cs_a, diag_mode, inv, (int *)nullptr, 0);
   else
-    hipLaunchKernelGGL((trsm_prep<T, false>), dim3(blocks), dim3(256), 0, s, m, a, rs_a,
-                       cs_a, diag_mode, out);
+    hipLaunchKernelGGL((trsm_inv_prep<T, false>), dim3(blocks), dim3(NB), 0, s, m, a, rs_a,
+                       cs_a, diag_mode, inv, (int *)nullptr, 0);
   return hipGetLastError() == hipSuccess ? MUBLIS_OK : MUBLIS_EHIP;
 }
 
 template <typename T>
-int leaf(bool lower, int nb, int n, T alpha, const double *ad, T *b, int64_t rs_b,
-         int64_t cs_b, hipStream_t s) {
-  const int grid = (n + LC - 1) / LC;
-  if (lower)
-    hipLaunchKernelGGL((trsm_leaf<T, true>), dim3(grid), dim3(NB), 0, s, nb, n, alpha, ad,
-                       b, rs_b, cs_b);
-  else
-    hipLaunchKernelGGL((trsm_leaf<T, false>), dim3(grid), dim3(NB), 0, s, nb, n, alpha,
-                       ad, b, rs_b, cs_b);
+int leaf(int nb, int n, T alpha, const double *inv, T *b, int64_t rs_b, int64_t cs_b,
+         hipStream_t s) {
+  hipLaunchKernelGGL((trsm_inv_apply<T>), dim3((n + 63) / 64), dim3(256), 0, s, nb, n, alpha,
+                     inv, b, rs_b, cs_b);
   return hipGetLastError() == hipSuccess ? MUBLIS_OK : MUBLIS_EHIP;
 }
 
-// `ad` holds the prepared diagonal blocks of this sub-problem (block 0 is
+// `ad` holds the inverted diagonal blocks of this sub-problem (block 0 is
 // its top-left NB x NB block).
 template <typename T>
 int trsm_rec(bool lower, const double *ad, int m, int n, T alpha, const T *a,
              int64_t rs_a, int64_t cs_a, T *b, int64_t rs_b, int64_t cs_b,
              hipStream_t s) {
-  if (m <= NB) return leaf<T>(lower, m, n, alpha, ad, b, rs_b, cs_b, s);
+  if (m <= NB) return leaf<T>(m, n, alpha, ad, b, rs_b, cs_b, s);
   // First part: a multiple of NB close to half, so GEMM K and M stay tile
   // friendly at every level.
   int m1 = (m / 2 + NB - 1) / NB * NB;
@@ -215,7 +359,7 @@ int trsm_rec(bool lower, const double *ad, int m, int n, T alpha, const T *a,
   const T *a12 = a + (int64_t)m1 * cs_a;
   const T *a22 = a + (int64_t)m1 * rs_a + (int64_t)m1 * cs_a;
   T *b1 = b, *b2 = b + (int64_t)m1 * rs_b;
-  const double *ad1 = ad, *ad2 = ad + (int64_t)(m1 / NB) * BLK;
+  const double *ad1 = ad, *ad2 = ad + (int64_t)(m1 / NB) * NB * NB;
   int st;
   if (lower) {
     if ((st = trsm_rec<T>(lower, ad1, m1, n, alpha, a11, rs_a, cs_a, b1, rs_b, cs_b,
@@ -241,9 +385,53 @@ int trsm_rec(bool lower, const double *ad, int m, int n, T alpha, const T *a,
 // recursive schedule; asynchronous on s.
 int64_t ws_doubles(int m) { return (int64_t)((m + NB - 1) / NB) * BLK; }
 
+// Widest B solved by the pipelined kernel (at most 4096, so its flags fit in
+// the recursive schedule's workspace).  Below that cap the pipeline takes
+// B up to 128 columns, or up to m * n = 2^20: beyond, the recursive
+// schedule's large GEMMs fill the GPU better than one chain of block rows
+// (measured crossover on MI355X, FP64: n ~ 1024 at m = 1024, 256 at 4096,
+// 128 at 8192).
+int g_pipe_max_n = 4096;
+
+bool use_pipe(int m, int n) {
+  return m > NB && n <= g_pipe_max_n && (n <= 128 || (int64_t)m * n <= (int64_t(1) << 20));
+}
+
+template <typename T, bool LOWER, int CW>
+int pipe_launch(int m, int n, T alpha, const T *a, int64_t rs_a, int64_t cs_a, T *b,
+                int64_t rs_b, int64_t cs_b, const double *inv, int *flags, hipStream_t s) {
+  const int nbk = (m + PB - 1) / PB, cg = (n + CW - 1) / CW;
+  hipLaunchKernelGGL((trsm_pipe<T, LOWER, CW>), dim3(nbk * cg), dim3(256), 0, s, m, n, alpha,
+                     a, rs_a, cs_a, b, rs_b, cs_b, inv, flags, cg);
+  return hipGetLastError() == hipSuccess ? MUBLIS_OK : MUBLIS_EHIP;
+}
+
+template <typename T, bool LOWER>
+int solve_pipe(int diag_mode, int m, int n, T alpha, const T *a, int64_t rs_a, int64_t cs_a,
+               T *b, int64_t rs_b, int64_t cs_b, double *ws, hipStream_t s) {
+  const int nbk = (m + PB - 1) / PB;
+  const int cw = n <= 32 ? 16 : (n <= 128 ? 32 : 64);
+  const int cg = (n + cw - 1) / cw;
+  double *inv = ws;
+  int *flags = reinterpret_cast<int *>(ws + (int64_t)nbk * PB * PB);
+  hipLaunchKernelGGL((trsm_inv_prep<T, LOWER>), dim3(nbk), dim3(PB), 0, s, m, a, rs_a, cs_a,
+                     diag_mode, inv, flags, nbk * cg + 1);
+  if (hipGetLastError() != hipSuccess) return MUBLIS_EHIP;
+  if (cw == 16)
+    return pipe_launch<T, LOWER, 16>(m, n, alpha, a, rs_a, cs_a, b, rs_b, cs_b, inv, flags, s);
+  if (cw == 32)
+    return pipe_launch<T, LOWER, 32>(m, n, alpha, a, rs_a, cs_a, b, rs_b, cs_b, inv, flags, s);
+  return pipe_launch<T, LOWER, 64>(m, n, alpha, a, rs_a, cs_a, b, rs_b, cs_b, inv, flags, s);
+}
+
 template <typename T>
 int solve_ws(bool lower, int diag_mode, int m, int n, T alpha, const T *a, int64_t rs_a,
              int64_t cs_a, T *b, int64_t rs_b, int64_t cs_b, double *ad, hipStream_t s) {
+  if (use_pipe(m, n))
+    return lower ? solve_pipe<T, true>(diag_mode, m, n, alpha, a, rs_a, cs_a, b, rs_b, cs_b,
+                                       ad, s)
+                 : solve_pipe<T, false>(diag_mode, m, n, alpha, a, rs_a, cs_a, b, rs_b, cs_b,
+                                        ad, s);
   int st = prep<T>(lower, m, a, rs_a, cs_a, diag_mode, ad, s);
   if (st == MUBLIS_OK) st = trsm_rec<T>(lower, ad, m, n, alpha, a, rs_a, cs_a, b, rs_b,
                                         cs_b, s);
@@ -338,4 +526,10 @@ extern "C" int mublis_gfx950_strsm_ws_dev(int lower, int unit, int m, int n, flo
                                           void *stream) {
   return trsm_ws_dev<float>(lower, unit, m, n, alpha, a, rs_a, cs_a, b, rs_b, cs_b, ws,
                             ws_bytes, stream);
+}
+
+extern "C" int mublis_gfx950_trsm_pipe_max_n(int max_n) {
+  const int old = g_pipe_max_n;
+  if (max_n >= 0) g_pipe_max_n = std::min(max_n, 4096);
+  return old;
 }

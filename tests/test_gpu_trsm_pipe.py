@@ -1,0 +1,145 @@
+"""GPU checks of the pipelined GEMMTRSM schedule (narrow B: one launch, block
+rows chained by device-side flags) against the recursive schedule and the
+oracle, through the cblas_?trsm C ABI.
+
+Bars:
+* residual probe in long double (oracle): for a random vector v,
+  ||op(A) (X v) - alpha B v|| <= 2 s eps ||A||_F ||X||_F ||v|| (Left), and
+  ||X (op(A) v) - alpha B v|| likewise (Right), s the order of A and eps of
+  the data type;
+* the pipelined and the recursive schedule agree to a relative forward
+  difference of 50 s eps (both solve the same well-conditioned system);
+* ragged shapes: m not a multiple of the 64-row block, n not a multiple of
+  the column group (16, 32 or 64 columns), a single right-hand side; every
+  shape here is one the default dispatch sends to the pipeline (n <= 128 or
+  m n <= 2^20).
+"""
+import ctypes
+import itertools
+
+import numpy as np
+import pytest
+
+import l3cases as L
+import mublis_amd as M
+
+pytestmark = pytest.mark.gpu
+
+EPS = {np.dtype(np.float32): np.finfo(np.float32).eps,
+       np.dtype(np.float64): np.finfo(np.float64).eps}
+
+
+@pytest.fixture(scope="module")
+def lib():
+    lib = M.load()
+    assert lib.mublis_rt_init() == 0
+    return lib
+
+
+@pytest.fixture(scope="module")
+def prod(lib):
+    return L.Cblas(lib)
+
+
+def _orc(name, argtypes):
+    f = getattr(ctypes.CDLL(L.ORACLE_LIB), name)
+    f.argtypes = argtypes
+    return f
+
+
+_VP, _I = ctypes.c_void_p, ctypes.c_int
+GEMV = _orc("orc_dgemv_ld", [_I, _I, _VP, _I, _VP, _VP])
+TRMV = _orc("orc_dtrmv_ld", [_I, _I, _I, _I, _VP, _I, _VP, _VP])
+
+
+def residual(a, x, b0, alpha, right, lower, trans, unit, seed=0):
+    """||probe|| and its bound, in long double through the oracle."""
+    m, n = x.shape
+    s = a.shape[0]
+    A = np.asfortranarray(a, dtype=np.float64)
+    X = np.asfortranarray(x, dtype=np.float64)
+    B = np.asfortranarray(b0, dtype=np.float64)
+    uplo, ta, dg = (L.LO if lower else L.UP), (L.TR if trans else L.NOTR), \
+        (L.UNIT if unit else L.NONUNIT)
+    v = np.random.default_rng(seed).standard_normal(n)
+    bv = np.zeros(m)
+    GEMV(m, n, B.ctypes.data, m, v.ctypes.data, bv.ctypes.data)
+    lhs = np.zeros(m)
+    if right:
+        w = np.zeros(n)
+        TRMV(uplo, ta, dg, n, A.ctypes.data, s, v.ctypes.data, w.ctypes.data)
+        GEMV(m, n, X.ctypes.data, m, w.ctypes.data, lhs.ctypes.data)
+    else:
+        xv = np.zeros(m)
+        GEMV(m, n, X.ctypes.data, m, v.ctypes.data, xv.ctypes.data)
+        TRMV(uplo, ta, dg, m, A.ctypes.data, s, xv.ctypes.data, lhs.ctypes.data)
+    res = np.linalg.norm(lhs - alpha * bv)
+    T = L.tri_dense(a, lower, unit)
+    lim = 2 * s * EPS[np.dtype(x.dtype)] * np.linalg.norm(T) * np.linalg.norm(X) * \
+        np.linalg.norm(v)
+    return res, lim
+
+
+SHAPES = [(65, 1), (200, 16), (257, 17), (1000, 33), (640, 128), (1536, 200), (3000, 300)]
+
+
+@pytest.mark.parametrize("dt", L.DTYPES)
+@pytest.mark.parametrize("m,n", SHAPES)
+def test_pipelined_vs_recursive_left(lib, prod, dt, m, n):
+    rng = np.random.default_rng(m * 1000 + n)
+    failures = []
+    for lower, unit in itertools.product((True, False), repeat=2):
+        trans = (m + n + lower) % 2 == 1
+        a = L.conditioned_triangular(rng, m, dt, "F", lower, unit)
+        b0 = L.rand(rng, (m, n), dt, "F")
+        alpha = 0.75
+        outs = []
+        for max_n in (-1, 0):  # default (pipelined for this n), then recursive only
+            old = lib.mublis_gfx950_trsm_pipe_max_n(max_n)
+            try:
+                x = np.array(b0, copy=True, order="F")
+                prod.call(dt, "trsm", L.COL, L.LEFT, L.LO if lower else L.UP,
+                          L.TR if trans else L.NOTR, L.UNIT if unit else L.NONUNIT, m, n,
+                          alpha, a, m, x, m)
+            finally:
+                lib.mublis_gfx950_trsm_pipe_max_n(old)
+            outs.append(x)
+        pipe, rec = outs
+        res, lim = residual(a, pipe, b0, alpha, False, lower, trans, unit)
+        diff = np.linalg.norm(pipe.astype(np.float64) - rec) / np.linalg.norm(rec)
+        tag = f"l{int(lower)} u{int(unit)} t{int(trans)}"
+        if not np.isfinite(pipe).all() or not res <= lim:
+            failures.append(f"{tag}: residual {res:.3e} > {lim:.3e}")
+        if not diff <= 50 * m * EPS[np.dtype(dt)]:
+            failures.append(f"{tag}: pipelined vs recursive {diff:.3e}")
+    assert not failures, "\n".join(failures)
+
+
+@pytest.mark.parametrize("dt", L.DTYPES)
+def test_pipelined_right_side(lib, prod, dt):
+    """Right side: the frame transposes B's view (unit stride along the
+    columns of the solve), which the pipeline stages along that stride."""
+    rng = np.random.default_rng(4)
+    m, n = 96, 700
+    assert lib.mublis_gfx950_trsm_pipe_max_n(-1) >= m
+    failures = []
+    for lower, trans, unit in itertools.product((True, False), repeat=3):
+        a = L.conditioned_triangular(rng, n, dt, "F", lower, unit)
+        b0 = L.rand(rng, (m, n), dt, "F")
+        x = np.array(b0, copy=True, order="F")
+        prod.call(dt, "trsm", L.COL, L.RIGHT, L.LO if lower else L.UP,
+                  L.TR if trans else L.NOTR, L.UNIT if unit else L.NONUNIT, m, n, -1.25, a,
+                  n, x, m)
+        res, lim = residual(a, x, b0, -1.25, True, lower, trans, unit)
+        if not np.isfinite(x).all() or not res <= lim:
+            failures.append(f"l{lower} t{trans} u{unit}: {res:.3e} > {lim:.3e}")
+    assert not failures, "\n".join(failures)
+
+
+def test_pipe_threshold_knob(lib):
+    old = lib.mublis_gfx950_trsm_pipe_max_n(100000)
+    try:
+        assert lib.mublis_gfx950_trsm_pipe_max_n(-1) == 4096  # clamped
+    finally:
+        lib.mublis_gfx950_trsm_pipe_max_n(old)
+    assert lib.mublis_gfx950_trsm_pipe_max_n(-1) == old
