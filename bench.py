@@ -52,7 +52,7 @@ def parse():
     p.add_argument("--cpu-slice", type=int, default=2048,
                    help="M = N of the CPU-baseline DGEMM slice (K stays 8192)")
     p.add_argument("--only", choices=["dgemm", "dtrsm", "dsyrk", "sgemm", "sgemm_bf16x3",
-                                      "serving"],
+                                      "batched", "serving"],
                    help="profiling aid: run just this operation (3 calls) and exit")
     p.add_argument("--traffic-probe", action="store_true",
                    help="profiling aid: one 512 MiB device copy (counter "
@@ -152,6 +152,14 @@ def profile_mode(args, torch, M, lib, dev, g):
             Bt = B0.clone()
             lib.cblas_dtrsm(COL, LEFT, LO, NOTR, NONUNIT, m, nrhs, 1.0, M.ptr(At), m,
                             M.ptr(Bt), m)
+    elif what == "batched":
+        from mublis_amd.workloads import grouped_problems
+        ops, keep, specs = grouped_problems(torch)
+        plan = M.GroupedPlan(ops)
+        for _ in range(calls):
+            plan.execute(lib.mublis_rt_stream())
+        lib.mublis_rt_sync(None)
+        plan.close()
     elif what == "dsyrk":
         k = n // 4
         As = torch.randn((k, n), dtype=torch.float64, device=dev, generator=g)

@@ -61,52 +61,59 @@ __global__ __launch_bounds__(64) void trsm_inv_prep(int m, const T *__restrict__
                                                     int diag_mode,
                                                     double *__restrict__ inv,
                                                     int *__restrict__ flags, int nflags) {
-  __shared__ double L[PB][PB + 1];
-  __shared__ double X[PB][PB + 1];
+  __shared__ double LT[PB][PB + 1];  // LT[k][r] = A(base + r, base + k) in the triangle
   __shared__ double rd[PB];
-  const int p = blockIdx.x, c = threadIdx.x;
-  for (int i = p * PB + c; i < nflags; i += gridDim.x * PB) flags[i] = 0;
+  const int p = blockIdx.x, lane = threadIdx.x;
+  for (int i = p * PB + lane; i < nflags; i += gridDim.x * PB) flags[i] = 0;
   const int base = p * PB;
   const int nbr = min(PB, m - base);
-  for (int r = 0; r < PB; ++r) {
+  // lanes run along A's unit stride
+  const bool cm = rs_a == 1;
+  for (int u = 0; u < PB; ++u) {
+    const int r = cm ? lane : u, c = cm ? u : lane;
     double v = 0.0;
     if (r < nbr && c < nbr && (LOWER ? r > c : r < c))
       v = (double)a[(int64_t)(base + r) * rs_a + (int64_t)(base + c) * cs_a];
-    L[r][c] = v;
+    LT[c][r] = v;
   }
   {
     double v = 1.0;  // rows past m: identity
-    if (c < nbr && diag_mode != 1) {
-      const int64_t g = base + c;
+    if (lane < nbr && diag_mode != 1) {
+      const int64_t g = base + lane;
       const double d = (double)a[g * rs_a + g * cs_a];
       v = diag_mode == 2 ? d : 1.0 / d;
     }
-    rd[c] = v;
+    rd[lane] = v;
   }
   __syncthreads();
-  // Lane c computes column c of the inverse by substitution.  The loops are
-  // uniform across lanes: X[k][c] is exactly 0 outside the triangle, so
-  // every lane reads the same L[r][k] (an LDS broadcast).
-  if (LOWER) {
-    for (int r = 0; r < PB; ++r) {
-      double s = r == c ? 1.0 : 0.0;
-      for (int k = 0; k < r; ++k) s = fma(-L[r][k], X[k][c], s);
-      X[r][c] = s * rd[r];
-    }
-  } else {
-    for (int r = PB - 1; r >= 0; --r) {
-      double s = r == c ? 1.0 : 0.0;
-      for (int k = r + 1; k < PB; ++k) s = fma(-L[r][k], X[k][c], s);
-      X[r][c] = s * rd[r];
+  // Lane c computes column c of the inverse, right-looking: once x_k is
+  // final it is pushed into every later row, so each step is up to 63
+  // independent FMAs on registers (the column lives in s[]).  Entries above
+  // (below) the diagonal of a lower (upper) inverse stay exactly 0; every
+  // lane reads the same LT[k][r] (an LDS broadcast).
+  const int c = lane;
+  double s[PB];
+#pragma unroll
+  for (int r = 0; r < PB; ++r) s[r] = r == c ? 1.0 : 0.0;
+#pragma unroll
+  for (int t = 0; t < PB; ++t) {
+    const int k = LOWER ? t : PB - 1 - t;
+    const double xk = s[k] * rd[k];
+    s[k] = xk;
+#pragma unroll
+    for (int u = t + 1; u < PB; ++u) {
+      const int r = LOWER ? u : PB - 1 - u;
+      s[r] = fma(-LT[k][r], xk, s[r]);
     }
   }
   double *o = inv + (int64_t)p * PB * PB;  // row-major
-  for (int r = 0; r < PB; ++r) o[r * PB + c] = X[r][c];
+#pragma unroll
+  for (int r = 0; r < PB; ++r) o[r * PB + c] = s[r];
 }
 
-__device__ inline void wait_flag(int *f, int *abort_flag) {
+__device__ inline void wait_flag(int *f, int epoch, int *abort_flag) {
   int spins = 0;
-  while (__hip_atomic_load(f, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == 0) {
+  while (__hip_atomic_load(f, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) != epoch) {
     if (__hip_atomic_load(abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
     __builtin_amdgcn_s_sleep(2);
     if (++spins > (1 << 20)) {
@@ -166,7 +173,7 @@ __global__ __launch_bounds__(256) void trsm_pipe(int m, int n, T alpha,
                                                  const T *__restrict__ a, int64_t rs_a,
                                                  int64_t cs_a, T *b, int64_t rs_b,
                                                  int64_t cs_b, const double *__restrict__ inv,
-                                                 int *flags, int cg) {
+                                                 int *flags, int cg, int epoch) {
   typedef double d4 __attribute__((ext_vector_type(4)));
   constexpr int TN = CW / 16;          // 16-column MFMA tiles per wave
   constexpr int TV = PB * CW / 256;    // tile elements per thread
@@ -180,7 +187,8 @@ __global__ __launch_bounds__(256) void trsm_pipe(int m, int n, T alpha,
   const int c0 = cgi * CW;
   const bool cm = rs_b == 1;
   const int arow = p * PB + 16 * wave + lr;  // row of A / inv this lane feeds
-  int *abort_flag = flags + nbk * cg;
+  int *abort_flag = flags;  // flags[0]; block (j, c) publishes flags[1 + j * cg + c]
+  flags += 1;
   // independent of the other block rows: this block's right-hand side and
   // inverse rows, fetched before any wait
   double breg[TV], ireg[16];
@@ -199,11 +207,11 @@ __global__ __launch_bounds__(256) void trsm_pipe(int m, int n, T alpha,
     // A(p, q(i)) before the wait; then the run of published blocks i .. f-1
     load_ablk<T>(av, a, rs_a, cs_a, m, arow, LOWER ? i : nbk - 1 - i, lk);
     if (tid == 0) {
-      wait_flag(&flags[i * cg + cgi], abort_flag);
+      wait_flag(&flags[i * cg + cgi], epoch, abort_flag);
       int f = i + 1;
       while (f < j &&
              __hip_atomic_load(&flags[f * cg + cgi], __ATOMIC_ACQUIRE,
-                               __HIP_MEMORY_SCOPE_AGENT) != 0)
+                               __HIP_MEMORY_SCOPE_AGENT) == epoch)
         ++f;
       s_front = f;
     }
@@ -270,7 +278,8 @@ __global__ __launch_bounds__(256) void trsm_pipe(int m, int n, T alpha,
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
   __syncthreads();
   if (tid == 0)
-    __hip_atomic_store(&flags[j * cg + cgi], 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(&flags[j * cg + cgi], epoch, __ATOMIC_RELEASE,
+                       __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // Recursive-schedule leaf: X = inv(A(0,0)) * alpha * B for the nb <= NB
@@ -320,17 +329,44 @@ __global__ __launch_bounds__(256) void trsm_inv_apply(int nb, int n, T alpha,
   }
 }
 
-// Inverted diagonal blocks (NB * NB doubles each, row-major) into `inv`.
+// Widest B solved by the pipelined kernel, at most 4096; 0 disables it.
+// Below that cap a whole solve takes the pipeline for B up to 128 columns or
+// up to m * n = 2^20; beyond, the recursive schedule's large GEMMs fill the
+// GPU better than one chain of block rows (measured crossover on MI355X,
+// FP64: n ~ 1024 at m = 1024, 256 at 4096, 128 at 8192).  The recursive
+// schedule in turn hands its diagonal sub-problems of at most kPipeLeaf
+// rows to the pipeline, which replaces the narrowest GEMM launches.
+int g_pipe_max_n = 4096;
+constexpr int kPipeLeaf = 512;
+
+bool use_pipe(int m, int n) {
+  return m > NB && n <= g_pipe_max_n && (n <= 128 || (int64_t)m * n <= (int64_t(1) << 20));
+}
+
+int pipe_cw(int n) { return n <= 32 ? 16 : (n <= 128 ? 32 : 64); }
+
+// Workspace of a solve: the inverted diagonal blocks (NB * NB doubles each,
+// row-major), then the pipeline's flags: flags[0] is the abort flag,
+// flags[1 + j * cg + c] the publication epoch of block row j, column group c.
+// The flags fit in the NB doubles per block that follow the inverses while
+// 1 + nbk * cg <= nbk * 2 * NB.
+int64_t ws_doubles(int m) { return (int64_t)((m + NB - 1) / NB) * BLK; }
+
+bool flags_fit(int m, int n) {
+  const int64_t nbk = (m + NB - 1) / NB, cg = (n + pipe_cw(n) - 1) / pipe_cw(n);
+  return 1 + nbk * cg <= nbk * 2 * NB;
+}
+
 template <typename T>
 int prep(bool lower, int m, const T *a, int64_t rs_a, int64_t cs_a, int diag_mode,
-         double *inv, hipStream_t s) {
+         double *inv, int *flags, int nflags, hipStream_t s) {
   const int blocks = (m + NB - 1) / NB;
   if (lower)
     hipLaunchKernelGGL((trsm_inv_prep<T, true>), dim3(blocks), dim3(NB), 0, s, m, a, rs_a,
-                       cs_a, diag_mode, inv, (int *)nullptr, 0);
+                       cs_a, diag_mode, inv, flags, nflags);
   else
     hipLaunchKernelGGL((trsm_inv_prep<T, false>), dim3(blocks), dim3(NB), 0, s, m, a, rs_a,
-                       cs_a, diag_mode, inv, (int *)nullptr, 0);
+                       cs_a, diag_mode, inv, flags, nflags);
   return hipGetLastError() == hipSuccess ? MUBLIS_OK : MUBLIS_EHIP;
 }
 
@@ -342,13 +378,47 @@ int leaf(int nb, int n, T alpha, const double *inv, T *b, int64_t rs_b, int64_t 
   return hipGetLastError() == hipSuccess ? MUBLIS_OK : MUBLIS_EHIP;
 }
 
-// `ad` holds the inverted diagonal blocks of this sub-problem (block 0 is
-// its top-left NB x NB block).
+template <typename T, bool LOWER, int CW>
+int pipe_launch(int m, int n, T alpha, const T *a, int64_t rs_a, int64_t cs_a, T *b,
+                int64_t rs_b, int64_t cs_b, const double *inv, int *flags, int epoch,
+                hipStream_t s) {
+  const int nbk = (m + PB - 1) / PB, cg = (n + CW - 1) / CW;
+  hipLaunchKernelGGL((trsm_pipe<T, LOWER, CW>), dim3(nbk * cg), dim3(256), 0, s, m, n, alpha,
+                     a, rs_a, cs_a, b, rs_b, cs_b, inv, flags, cg, epoch);
+  return hipGetLastError() == hipSuccess ? MUBLIS_OK : MUBLIS_EHIP;
+}
+
+// One pipelined solve of an m-row (sub-)problem whose inverted blocks start
+// at `inv`; flags must hold `epoch - 1` or less (prep clears them to 0).
+template <typename T>
+int pipe(bool lower, int m, int n, T alpha, const T *a, int64_t rs_a, int64_t cs_a, T *b,
+         int64_t rs_b, int64_t cs_b, const double *inv, int *flags, int epoch,
+         hipStream_t s) {
+  const int cw = pipe_cw(n);
+#define MUBLIS_PIPE(L, W)                                                                   \
+  return pipe_launch<T, L, W>(m, n, alpha, a, rs_a, cs_a, b, rs_b, cs_b, inv, flags, epoch, \
+                              s)
+  if (lower) {
+    if (cw == 16) MUBLIS_PIPE(true, 16);
+    if (cw == 32) MUBLIS_PIPE(true, 32);
+    MUBLIS_PIPE(true, 64);
+  }
+  if (cw == 16) MUBLIS_PIPE(false, 16);
+  if (cw == 32) MUBLIS_PIPE(false, 32);
+  MUBLIS_PIPE(false, 64);
+#undef MUBLIS_PIPE
+}
+
+// Recursive schedule.  `ad` holds the inverted diagonal blocks of this
+// sub-problem (block 0 is its top-left NB x NB block); `flags` is null when
+// the pipeline is not used for leaves.
 template <typename T>
 int trsm_rec(bool lower, const double *ad, int m, int n, T alpha, const T *a,
-             int64_t rs_a, int64_t cs_a, T *b, int64_t rs_b, int64_t cs_b,
-             hipStream_t s) {
+             int64_t rs_a, int64_t cs_a, T *b, int64_t rs_b, int64_t cs_b, int *flags,
+             int &epoch, hipStream_t s) {
   if (m <= NB) return leaf<T>(m, n, alpha, ad, b, rs_b, cs_b, s);
+  if (flags && m <= kPipeLeaf)
+    return pipe<T>(lower, m, n, alpha, a, rs_a, cs_a, b, rs_b, cs_b, ad, flags, ++epoch, s);
   // First part: a multiple of NB close to half, so GEMM K and M stay tile
   // friendly at every level.
   int m1 = (m / 2 + NB - 1) / NB * NB;
@@ -362,80 +432,42 @@ int trsm_rec(bool lower, const double *ad, int m, int n, T alpha, const T *a,
   const double *ad1 = ad, *ad2 = ad + (int64_t)(m1 / NB) * NB * NB;
   int st;
   if (lower) {
-    if ((st = trsm_rec<T>(lower, ad1, m1, n, alpha, a11, rs_a, cs_a, b1, rs_b, cs_b,
-                          s)))
+    if ((st = trsm_rec<T>(lower, ad1, m1, n, alpha, a11, rs_a, cs_a, b1, rs_b, cs_b, flags,
+                          epoch, s)))
       return st;
     // B2 := alpha*B2 - A21 * X1 (the reference's GEMM with alpha -1 and
     // beta = alpha_eff, frame/l3/trsm.c:165-197, 241-249)
     if ((st = gfx950_gemm<T>(m2, n, m1, T(-1), a21, rs_a, cs_a, b1, rs_b, cs_b, alpha,
                              b2, rs_b, cs_b, s)))
       return st;
-    return trsm_rec<T>(lower, ad2, m2, n, T(1), a22, rs_a, cs_a, b2, rs_b, cs_b, s);
+    return trsm_rec<T>(lower, ad2, m2, n, T(1), a22, rs_a, cs_a, b2, rs_b, cs_b, flags,
+                       epoch, s);
   }
-  if ((st = trsm_rec<T>(lower, ad2, m2, n, alpha, a22, rs_a, cs_a, b2, rs_b, cs_b,
-                        s)))
+  if ((st = trsm_rec<T>(lower, ad2, m2, n, alpha, a22, rs_a, cs_a, b2, rs_b, cs_b, flags,
+                        epoch, s)))
     return st;
   if ((st = gfx950_gemm<T>(m1, n, m2, T(-1), a12, rs_a, cs_a, b2, rs_b, cs_b, alpha,
                            b1, rs_b, cs_b, s)))
     return st;
-  return trsm_rec<T>(lower, ad1, m1, n, T(1), a11, rs_a, cs_a, b1, rs_b, cs_b, s);
+  return trsm_rec<T>(lower, ad1, m1, n, T(1), a11, rs_a, cs_a, b1, rs_b, cs_b, flags, epoch,
+                     s);
 }
 
-// Prepare all diagonal blocks into `ad` (ws_doubles(m) doubles) and run the
-// recursive schedule; asynchronous on s.
-int64_t ws_doubles(int m) { return (int64_t)((m + NB - 1) / NB) * BLK; }
-
-// Widest B solved by the pipelined kernel (at most 4096, so its flags fit in
-// the recursive schedule's workspace).  Below that cap the pipeline takes
-// B up to 128 columns, or up to m * n = 2^20: beyond, the recursive
-// schedule's large GEMMs fill the GPU better than one chain of block rows
-// (measured crossover on MI355X, FP64: n ~ 1024 at m = 1024, 256 at 4096,
-// 128 at 8192).
-int g_pipe_max_n = 4096;
-
-bool use_pipe(int m, int n) {
-  return m > NB && n <= g_pipe_max_n && (n <= 128 || (int64_t)m * n <= (int64_t(1) << 20));
-}
-
-template <typename T, bool LOWER, int CW>
-int pipe_launch(int m, int n, T alpha, const T *a, int64_t rs_a, int64_t cs_a, T *b,
-                int64_t rs_b, int64_t cs_b, const double *inv, int *flags, hipStream_t s) {
-  const int nbk = (m + PB - 1) / PB, cg = (n + CW - 1) / CW;
-  hipLaunchKernelGGL((trsm_pipe<T, LOWER, CW>), dim3(nbk * cg), dim3(256), 0, s, m, n, alpha,
-                     a, rs_a, cs_a, b, rs_b, cs_b, inv, flags, cg);
-  return hipGetLastError() == hipSuccess ? MUBLIS_OK : MUBLIS_EHIP;
-}
-
-template <typename T, bool LOWER>
-int solve_pipe(int diag_mode, int m, int n, T alpha, const T *a, int64_t rs_a, int64_t cs_a,
-               T *b, int64_t rs_b, int64_t cs_b, double *ws, hipStream_t s) {
-  const int nbk = (m + PB - 1) / PB;
-  const int cw = n <= 32 ? 16 : (n <= 128 ? 32 : 64);
-  const int cg = (n + cw - 1) / cw;
-  double *inv = ws;
-  int *flags = reinterpret_cast<int *>(ws + (int64_t)nbk * PB * PB);
-  hipLaunchKernelGGL((trsm_inv_prep<T, LOWER>), dim3(nbk), dim3(PB), 0, s, m, a, rs_a, cs_a,
-                     diag_mode, inv, flags, nbk * cg + 1);
-  if (hipGetLastError() != hipSuccess) return MUBLIS_EHIP;
-  if (cw == 16)
-    return pipe_launch<T, LOWER, 16>(m, n, alpha, a, rs_a, cs_a, b, rs_b, cs_b, inv, flags, s);
-  if (cw == 32)
-    return pipe_launch<T, LOWER, 32>(m, n, alpha, a, rs_a, cs_a, b, rs_b, cs_b, inv, flags, s);
-  return pipe_launch<T, LOWER, 64>(m, n, alpha, a, rs_a, cs_a, b, rs_b, cs_b, inv, flags, s);
-}
-
+// Invert the diagonal blocks into `ad` (ws_doubles(m) doubles) and run the
+// pipelined or the recursive schedule; asynchronous on s.
 template <typename T>
 int solve_ws(bool lower, int diag_mode, int m, int n, T alpha, const T *a, int64_t rs_a,
              int64_t cs_a, T *b, int64_t rs_b, int64_t cs_b, double *ad, hipStream_t s) {
-  if (use_pipe(m, n))
-    return lower ? solve_pipe<T, true>(diag_mode, m, n, alpha, a, rs_a, cs_a, b, rs_b, cs_b,
-                                       ad, s)
-                 : solve_pipe<T, false>(diag_mode, m, n, alpha, a, rs_a, cs_a, b, rs_b, cs_b,
-                                        ad, s);
-  int st = prep<T>(lower, m, a, rs_a, cs_a, diag_mode, ad, s);
-  if (st == MUBLIS_OK) st = trsm_rec<T>(lower, ad, m, n, alpha, a, rs_a, cs_a, b, rs_b,
-                                        cs_b, s);
-  return st;
+  const int nbk = (m + NB - 1) / NB;
+  const bool piped = m > NB && n <= g_pipe_max_n && flags_fit(m, n);
+  int *flags = piped ? reinterpret_cast<int *>(ad + (int64_t)nbk * NB * NB) : nullptr;
+  const int cg = (n + pipe_cw(n) - 1) / pipe_cw(n);
+  int st = prep<T>(lower, m, a, rs_a, cs_a, diag_mode, ad, flags, piped ? 1 + nbk * cg : 0, s);
+  if (st) return st;
+  int epoch = 0;
+  if (piped && use_pipe(m, n))
+    return pipe<T>(lower, m, n, alpha, a, rs_a, cs_a, b, rs_b, cs_b, ad, flags, ++epoch, s);
+  return trsm_rec<T>(lower, ad, m, n, alpha, a, rs_a, cs_a, b, rs_b, cs_b, flags, epoch, s);
 }
 
 // Same with workspace from the arena, released once the stream has drained.

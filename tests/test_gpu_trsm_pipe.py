@@ -136,6 +136,37 @@ def test_pipelined_right_side(lib, prod, dt):
     assert not failures, "\n".join(failures)
 
 
+@pytest.mark.parametrize("dt", L.DTYPES)
+def test_recursive_schedule_with_pipelined_leaves(lib, prod, dt):
+    """Wide B: the recursive schedule splits A and hands its diagonal
+    sub-problems of at most 512 rows to the pipeline (2000 -> 1024 + 976 ->
+    512-row pieces); compared with the recursive schedule down to 64-row
+    leaves (pipeline disabled) and checked by the residual probe."""
+    rng = np.random.default_rng(21)
+    m, n = 2000, 1000
+    failures = []
+    for lower, trans in ((True, False), (False, True)):
+        a = L.conditioned_triangular(rng, m, dt, "F", lower, False)
+        b0 = L.rand(rng, (m, n), dt, "F")
+        outs = []
+        for max_n in (-1, 0):
+            old = lib.mublis_gfx950_trsm_pipe_max_n(max_n)
+            try:
+                x = np.array(b0, copy=True, order="F")
+                prod.call(dt, "trsm", L.COL, L.LEFT, L.LO if lower else L.UP,
+                          L.TR if trans else L.NOTR, L.NONUNIT, m, n, 2.0, a, m, x, m)
+            finally:
+                lib.mublis_gfx950_trsm_pipe_max_n(old)
+            outs.append(x)
+        res, lim = residual(a, outs[0], b0, 2.0, False, lower, trans, False)
+        diff = np.linalg.norm(outs[0].astype(np.float64) - outs[1]) / np.linalg.norm(outs[1])
+        if not np.isfinite(outs[0]).all() or not res <= lim:
+            failures.append(f"l{lower} t{trans}: residual {res:.3e} > {lim:.3e}")
+        if not diff <= 50 * m * EPS[np.dtype(dt)]:
+            failures.append(f"l{lower} t{trans}: vs 64-row leaves {diff:.3e}")
+    assert not failures, "\n".join(failures)
+
+
 def test_pipe_threshold_knob(lib):
     old = lib.mublis_gfx950_trsm_pipe_max_n(100000)
     try:
