@@ -69,12 +69,22 @@ __global__ __launch_bounds__(64) void trsm_inv_prep(int m, const T *__restrict__
   const int nbr = min(PB, m - base);
   // lanes run along A's unit stride
   const bool cm = rs_a == 1;
-  for (int u = 0; u < PB; ++u) {
-    const int r = cm ? lane : u, c = cm ? u : lane;
-    double v = 0.0;
-    if (r < nbr && c < nbr && (LOWER ? r > c : r < c))
-      v = (double)a[(int64_t)(base + r) * rs_a + (int64_t)(base + c) * cs_a];
-    LT[c][r] = v;
+  // 16 loads in flight per batch, then the LDS stores
+#pragma unroll
+  for (int u0 = 0; u0 < PB; u0 += 16) {
+    double v[16];
+#pragma unroll
+    for (int d = 0; d < 16; ++d) {
+      const int u = u0 + d, r = cm ? lane : u, c = cm ? u : lane;
+      v[d] = (r < nbr && c < nbr && (LOWER ? r > c : r < c))
+                 ? (double)a[(int64_t)(base + r) * rs_a + (int64_t)(base + c) * cs_a]
+                 : 0.0;
+    }
+#pragma unroll
+    for (int d = 0; d < 16; ++d) {
+      const int u = u0 + d, r = cm ? lane : u, c = cm ? u : lane;
+      LT[c][r] = v[d];
+    }
   }
   {
     double v = 1.0;  // rows past m: identity
@@ -329,15 +339,99 @@ __global__ __launch_bounds__(256) void trsm_inv_apply(int nb, int n, T alpha,
   }
 }
 
+// Recursive-schedule leaf for wide B: the sub-problem's (at most
+// kPanelRows) rows times PW = 16 columns of B per workgroup.  The workgroup
+// walks the block rows in solve order itself, keeping the solved rows in
+// LDS: X(j) = inv(A(j,j)) * (alpha B(j) - sum_{i<j} A(j,i) X(i)), the
+// reference's GEMMTRSM step (targets/reference/gemmtrsm_ukr.inc:1-45) with
+// a panel of B per workgroup.  No workgroup waits for another; parallelism
+// comes from the n / 16 column panels.
+constexpr int kPanelRows = 512;
+constexpr int PW = 16;
+
+template <typename T, bool LOWER>
+__global__ __launch_bounds__(256) void trsm_panel(int m, int n, T alpha,
+                                                  const T *__restrict__ a, int64_t rs_a,
+                                                  int64_t cs_a, T *b, int64_t rs_b,
+                                                  int64_t cs_b,
+                                                  const double *__restrict__ inv) {
+  typedef double d4 __attribute__((ext_vector_type(4)));
+  constexpr int TV = PB * PW / 256;
+  __shared__ double xs[kPanelRows][PW + 1];
+  const int nbk = (m + PB - 1) / PB;
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int lr = lane & 15, lk = lane >> 4;
+  const int c0 = blockIdx.x * PW;
+  const bool cm = rs_b == 1;
+  const double al = (double)alpha;
+  for (int j = 0; j < nbk; ++j) {
+    const int p = LOWER ? j : nbk - 1 - j;
+    const int arow = p * PB + 16 * wave + lr;
+    double breg[TV], ireg[16];
+    load_tile<T, PW>(breg, b, rs_b, cs_b, m, n, p, c0);
+    {
+      const double *ip = inv + (int64_t)p * PB * PB + (16 * wave + lr) * PB;
+#pragma unroll
+      for (int kk = 0; kk < 16; ++kk) ireg[kk] = ip[4 * kk + lk];
+    }
+    d4 acc = d4{0, 0, 0, 0};
+    double av[16], an[16];
+    if (j > 0) load_ablk<T>(av, a, rs_a, cs_a, m, arow, LOWER ? 0 : nbk - 1, lk);
+    for (int i = 0; i < j; ++i) {
+      const int q = LOWER ? i : nbk - 1 - i;
+      if (i + 1 < j) load_ablk<T>(an, a, rs_a, cs_a, m, arow, LOWER ? i + 1 : nbk - 2 - i, lk);
+#pragma unroll
+      for (int kk = 0; kk < 16; ++kk)
+        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av[kk], xs[q * PB + 4 * kk + lk][lr], acc,
+                                                   0, 0, 0);
+#pragma unroll
+      for (int kk = 0; kk < 16; ++kk) av[kk] = an[kk];
+    }
+    // R = alpha B(p) - update, staged in block p's rows of xs
+#pragma unroll
+    for (int it = 0; it < TV; ++it) {
+      int r, cc;
+      tile_rc<PW>(it * 256 + tid, cm, r, cc);
+      xs[p * PB + r][cc] = al * breg[it];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < 4; ++r) xs[p * PB + 16 * wave + lk + 4 * r][lr] -= acc[r];
+    __syncthreads();
+    d4 out = d4{0, 0, 0, 0};
+#pragma unroll
+    for (int kk = 0; kk < 16; ++kk)
+      out = __builtin_amdgcn_mfma_f64_16x16x4f64(ireg[kk], xs[p * PB + 4 * kk + lk][lr], out,
+                                                 0, 0, 0);
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < 4; ++r) xs[p * PB + 16 * wave + lk + 4 * r][lr] = out[r];
+    __syncthreads();
+  }
+  for (int j = 0; j < nbk; ++j) {
+#pragma unroll
+    for (int it = 0; it < TV; ++it) {
+      int r, cc;
+      tile_rc<PW>(it * 256 + tid, cm, r, cc);
+      const int gr = j * PB + r, gc = c0 + cc;
+      if (gr < m && gc < n) b[(int64_t)gr * rs_b + (int64_t)gc * cs_b] = (T)xs[j * PB + r][cc];
+    }
+  }
+}
+
 // Widest B solved by the pipelined kernel, at most 4096; 0 disables it.
 // Below that cap a whole solve takes the pipeline for B up to 128 columns or
 // up to m * n = 2^20; beyond, the recursive schedule's large GEMMs fill the
 // GPU better than one chain of block rows (measured crossover on MI355X,
 // FP64: n ~ 1024 at m = 1024, 256 at 4096, 128 at 8192).  The recursive
 // schedule in turn hands its diagonal sub-problems of at most kPipeLeaf
-// rows to the pipeline, which replaces the narrowest GEMM launches.
+// rows to the pipeline (narrower B) or to column-panel workgroups (wider B),
+// which replaces the narrowest GEMM launches.
 int g_pipe_max_n = 4096;
 constexpr int kPipeLeaf = 512;
+// From this many columns up, the recursion's leaves of at most kPanelRows
+// rows are column panels (n / 16 independent workgroups) instead.
+constexpr int kPanelMinN = 2048;
 
 bool use_pipe(int m, int n) {
   return m > NB && n <= g_pipe_max_n && (n <= 128 || (int64_t)m * n <= (int64_t(1) << 20));
@@ -417,6 +511,15 @@ int trsm_rec(bool lower, const double *ad, int m, int n, T alpha, const T *a,
              int64_t rs_a, int64_t cs_a, T *b, int64_t rs_b, int64_t cs_b, int *flags,
              int &epoch, hipStream_t s) {
   if (m <= NB) return leaf<T>(m, n, alpha, ad, b, rs_b, cs_b, s);
+  if (flags && m <= kPanelRows && n >= kPanelMinN) {
+    if (lower)
+      hipLaunchKernelGGL((trsm_panel<T, true>), dim3((n + PW - 1) / PW), dim3(256), 0, s, m, n,
+                         alpha, a, rs_a, cs_a, b, rs_b, cs_b, ad);
+    else
+      hipLaunchKernelGGL((trsm_panel<T, false>), dim3((n + PW - 1) / PW), dim3(256), 0, s, m,
+                         n, alpha, a, rs_a, cs_a, b, rs_b, cs_b, ad);
+    return hipGetLastError() == hipSuccess ? MUBLIS_OK : MUBLIS_EHIP;
+  }
   if (flags && m <= kPipeLeaf)
     return pipe<T>(lower, m, n, alpha, a, rs_a, cs_a, b, rs_b, cs_b, ad, flags, ++epoch, s);
   // First part: a multiple of NB close to half, so GEMM K and M stay tile

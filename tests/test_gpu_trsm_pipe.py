@@ -137,13 +137,15 @@ def test_pipelined_right_side(lib, prod, dt):
 
 
 @pytest.mark.parametrize("dt", L.DTYPES)
-def test_recursive_schedule_with_pipelined_leaves(lib, prod, dt):
+@pytest.mark.parametrize("m,n", [(2000, 1000), (1100, 2100)])
+def test_recursive_schedule_with_pipelined_leaves(lib, prod, dt, m, n):
     """Wide B: the recursive schedule splits A and hands its diagonal
-    sub-problems of at most 512 rows to the pipeline (2000 -> 1024 + 976 ->
-    512-row pieces); compared with the recursive schedule down to 64-row
-    leaves (pipeline disabled) and checked by the residual probe."""
+    sub-problems of at most 512 rows to the pipeline (n < 2048: 2000 ->
+    1024 + 976 -> 512-row pieces) or to column-panel workgroups (n >= 2048:
+    1100 -> 576 + 524 -> pieces of 256 to 320 rows); compared with the
+    recursive schedule down to 64-row leaves (pipeline disabled) and checked
+    by the residual probe."""
     rng = np.random.default_rng(21)
-    m, n = 2000, 1000
     failures = []
     for lower, trans in ((True, False), (False, True)):
         a = L.conditioned_triangular(rng, m, dt, "F", lower, False)
