@@ -11,11 +11,12 @@
 // the result rounds like a native FP32 GEMM.
 //
 // Layout: a split pass writes each operand as three K-contiguous BF16 planes
-// ([row][kp], rows padded to the 128 tile, K to 32, zero filled), so the GEMM
+// ([row][kp], rows padded to the tile, K to 32, zero filled), so the GEMM
 // kernel streams aligned 16-byte vectors with no bounds checks.  The GEMM
-// kernel: 128 x 128 output tile per workgroup, 4 waves of 64 x 64, each a
-// 2 x 2 grid of v_mfma_f32_32x32x16_bf16 tiles (6 MFMAs per tile per k=16),
-// LDS double-buffered, the XCD-aware tile order of the FP64/FP32 kernel.
+// kernel: 256 x 128 output tile per workgroup, one workgroup per CU, 4 waves
+// of 128 x 64, each a 4 x 2 grid of v_mfma_f32_32x32x16_bf16 tiles (6 MFMAs
+// per tile per k = 16: 48 MFMAs for 18 LDS fragment reads per lane), LDS
+// double-buffered, the XCD-aware tile order of the FP64/FP32 kernel.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -31,7 +32,8 @@ typedef float f16v __attribute__((ext_vector_type(16)));
 typedef unsigned short u16;
 typedef u16 u16x8 __attribute__((ext_vector_type(8)));
 
-constexpr int XT = 128;    // output tile
+constexpr int XTM = 256;   // output tile rows
+constexpr int XTN = 128;   // output tile columns
 constexpr int XBK = 16;    // k per stage (one MFMA k step)
 constexpr int XLD = 24;    // LDS row stride in BF16 (48 B: conflict-free b128 reads)
 constexpr int XNT = 256;
@@ -48,6 +50,7 @@ __device__ inline float bf16_to_f(u16 h) { return __uint_as_float((uint32_t)h <<
 // along the source's unit stride and writes along k.
 __global__ __launch_bounds__(256) void split3_kernel(int m, int k, const float *__restrict__ x,
                                                      int64_t rs, int64_t cs, int mp, int kp,
+                                                     int ldp,
                                                      u16 *__restrict__ hi, u16 *__restrict__ mid,
                                                      u16 *__restrict__ lo) {
   __shared__ float t[64][65];
@@ -80,7 +83,7 @@ __global__ __launch_bounds__(256) void split3_kernel(int m, int k, const float *
     vm[e / 8][e % 8] = md;
     vl[e / 8][e % 8] = l;
   }
-  const int64_t off = (int64_t)(r0 + r) * kp + k0 + c0;
+  const int64_t off = (int64_t)(r0 + r) * ldp + k0 + c0;
 #pragma unroll
   for (int h = 0; h < 2; ++h) {
     if (k0 + c0 + 8 * h < kp) {
@@ -92,10 +95,10 @@ __global__ __launch_bounds__(256) void split3_kernel(int m, int k, const float *
 }
 
 struct XArgs {
-  int m, n, kp, mp, np;
+  int m, n, kp, ldp, mp, np;
   float alpha, beta;
-  const u16 *a[3];  // [mp][kp]
-  const u16 *b[3];  // [np][kp]
+  const u16 *a[3];  // [mp][ldp], k < kp used
+  const u16 *b[3];  // [np][ldp]
   float *c;
   int64_t rs_c, cs_c;
   int tiles_m, tiles_n, ntiles;
@@ -112,20 +115,21 @@ __device__ inline void map_tile_x(int w, int ntiles, int tiles_m, int tiles_n, i
   tn = in / gs;
 }
 
-__global__ __launch_bounds__(XNT, 2) void gemm_bf16x3_kernel(XArgs p) {
-  __shared__ __attribute__((aligned(16))) u16 As[2][3][XT * XLD];
-  __shared__ __attribute__((aligned(16))) u16 Bs[2][3][XT * XLD];
+__global__ __launch_bounds__(XNT, 1) void gemm_bf16x3_kernel(XArgs p) {
+  __shared__ __attribute__((aligned(16))) u16 As[2][3][XTM * XLD];
+  __shared__ __attribute__((aligned(16))) u16 Bs[2][3][XTN * XLD];
   int tm, tn;
   map_tile_x(blockIdx.x, p.ntiles, p.tiles_m, p.tiles_n, tm, tn);
-  const int m0 = tm * XT, n0 = tn * XT;
+  const int m0 = tm * XTM, n0 = tn * XTN;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int wm = (wave & 1) * 64, wn = (wave >> 1) * 64;
-  // loader: thread -> (row, 8-element half of the 16-k stage)
-  const int lrow = tid >> 1, lhalf = (tid & 1) * 8;
+  const int wm = (wave & 1) * 128, wn = (wave >> 1) * 64;
+  // loaders: A row tid (both 8-element halves of the 16-k stage, 3 planes);
+  // B row tid / 2, half tid % 2 (3 planes)
+  const int brow = tid >> 1, bhalf = (tid & 1) * 8;
 
-  f16v big[2][2], small[2][2];
+  f16v big[4][2], small[4][2];
 #pragma unroll
-  for (int i = 0; i < 2; ++i)
+  for (int i = 0; i < 4; ++i)
 #pragma unroll
     for (int j = 0; j < 2; ++j) {
       big[i][j] = f16v{};
@@ -135,9 +139,9 @@ __global__ __launch_bounds__(XNT, 2) void gemm_bf16x3_kernel(XArgs p) {
   // Two register sets: the loads of stage st + 2 are issued while stage st
   // is computed, and land in LDS only at the end of iteration st + 1, so
   // every load has two compute phases to arrive.
-  u16x8 ra[2][3], rb[2][3];
-  const int64_t aoff = (int64_t)(m0 + lrow) * p.kp + lhalf;
-  const int64_t boff = (int64_t)(n0 + lrow) * p.kp + lhalf;
+  u16x8 ra[2][3][2], rb[2][3];
+  const int64_t aoff = (int64_t)(m0 + tid) * p.ldp;
+  const int64_t boff = (int64_t)(n0 + brow) * p.ldp + bhalf;
   const int stages = p.kp / XBK;
   // Loads are issued unconditionally (a stage past the end re-reads the last
   // one, in bounds and never used): conditional loads would make the
@@ -147,14 +151,16 @@ __global__ __launch_bounds__(XNT, 2) void gemm_bf16x3_kernel(XArgs p) {
     const int64_t k0 = (int64_t)min(t, stages - 1) * XBK;
 #pragma unroll
     for (int s = 0; s < 3; ++s) {
-      ra[t][s] = *reinterpret_cast<const u16x8 *>(p.a[s] + aoff + k0);
+      ra[t][s][0] = *reinterpret_cast<const u16x8 *>(p.a[s] + aoff + k0);
+      ra[t][s][1] = *reinterpret_cast<const u16x8 *>(p.a[s] + aoff + k0 + 8);
       rb[t][s] = *reinterpret_cast<const u16x8 *>(p.b[s] + boff + k0);
     }
   }
 #pragma unroll
   for (int s = 0; s < 3; ++s) {
-    *reinterpret_cast<u16x8 *>(&As[0][s][lrow * XLD + lhalf]) = ra[0][s];
-    *reinterpret_cast<u16x8 *>(&Bs[0][s][lrow * XLD + lhalf]) = rb[0][s];
+    *reinterpret_cast<u16x8 *>(&As[0][s][tid * XLD]) = ra[0][s][0];
+    *reinterpret_cast<u16x8 *>(&As[0][s][tid * XLD + 8]) = ra[0][s][1];
+    *reinterpret_cast<u16x8 *>(&Bs[0][s][brow * XLD + bhalf]) = rb[0][s];
   }
   __syncthreads();
 
@@ -173,7 +179,8 @@ __global__ __launch_bounds__(XNT, 2) void gemm_bf16x3_kernel(XArgs p) {
         const int64_t k0 = (int64_t)min(cur_st + 2, stages - 1) * XBK;
 #pragma unroll
         for (int s = 0; s < 3; ++s) {
-          ra[h][s] = *reinterpret_cast<const u16x8 *>(p.a[s] + aoff + k0);
+          ra[h][s][0] = *reinterpret_cast<const u16x8 *>(p.a[s] + aoff + k0);
+          ra[h][s][1] = *reinterpret_cast<const u16x8 *>(p.a[s] + aoff + k0 + 8);
           rb[h][s] = *reinterpret_cast<const u16x8 *>(p.b[s] + boff + k0);
         }
       }
@@ -181,16 +188,18 @@ __global__ __launch_bounds__(XNT, 2) void gemm_bf16x3_kernel(XArgs p) {
       // otherwise hoist the stores and sink the loads, shortening the time
       // each load has to arrive.
       __builtin_amdgcn_sched_barrier(0);
-      bf8 af[3][2], bf[3][2];
+      bf8 af[3][4], bf[3][2];
 #pragma unroll
-      for (int s = 0; s < 3; ++s)
+      for (int s = 0; s < 3; ++s) {
 #pragma unroll
-        for (int i = 0; i < 2; ++i) {
+        for (int i = 0; i < 4; ++i)
           af[s][i] = __builtin_bit_cast(
               bf8, *reinterpret_cast<const u16x8 *>(&As[cur][s][(wm + i * 32 + fr) * XLD + fk]));
-          bf[s][i] = __builtin_bit_cast(
-              bf8, *reinterpret_cast<const u16x8 *>(&Bs[cur][s][(wn + i * 32 + fr) * XLD + fk]));
-        }
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+          bf[s][j] = __builtin_bit_cast(
+              bf8, *reinterpret_cast<const u16x8 *>(&Bs[cur][s][(wn + j * 32 + fr) * XLD + fk]));
+      }
       // Six products per tile, issued product-major so that consecutive
       // MFMAs update independent accumulators: (mid, mid), (lo, hi),
       // (hi, lo), (mid, hi), (hi, mid) into `small`, then (hi, hi) into `big`.
@@ -199,7 +208,7 @@ __global__ __launch_bounds__(XNT, 2) void gemm_bf16x3_kernel(XArgs p) {
         const int pa = t == 0 ? 1 : t == 1 ? 2 : t == 2 ? 0 : t == 3 ? 1 : 0;
         const int pb = t == 0 ? 1 : t == 1 ? 0 : t == 2 ? 2 : t == 3 ? 0 : t == 4 ? 1 : 0;
 #pragma unroll
-        for (int i = 0; i < 2; ++i)
+        for (int i = 0; i < 4; ++i)
 #pragma unroll
           for (int j = 0; j < 2; ++j) {
             if (t < 5)
@@ -214,8 +223,9 @@ __global__ __launch_bounds__(XNT, 2) void gemm_bf16x3_kernel(XArgs p) {
       // stage cur_st + 1 (loaded one iteration earlier) -> the other slot
 #pragma unroll
       for (int s = 0; s < 3; ++s) {
-        *reinterpret_cast<u16x8 *>(&As[cur ^ 1][s][lrow * XLD + lhalf]) = ra[h ^ 1][s];
-        *reinterpret_cast<u16x8 *>(&Bs[cur ^ 1][s][lrow * XLD + lhalf]) = rb[h ^ 1][s];
+        *reinterpret_cast<u16x8 *>(&As[cur ^ 1][s][tid * XLD]) = ra[h ^ 1][s][0];
+        *reinterpret_cast<u16x8 *>(&As[cur ^ 1][s][tid * XLD + 8]) = ra[h ^ 1][s][1];
+        *reinterpret_cast<u16x8 *>(&Bs[cur ^ 1][s][brow * XLD + bhalf]) = rb[h ^ 1][s];
       }
       __syncthreads();
     }
@@ -225,7 +235,7 @@ __global__ __launch_bounds__(XNT, 2) void gemm_bf16x3_kernel(XArgs p) {
   // 8*(r/4) + 4*(l/32) + r%4 of its tile (measured on gfx950).
   const float alpha = p.alpha, beta = p.beta;
 #pragma unroll
-  for (int i = 0; i < 2; ++i)
+  for (int i = 0; i < 4; ++i)
 #pragma unroll
     for (int j = 0; j < 2; ++j) {
       const int col = n0 + wn + j * 32 + (lane & 31);
@@ -242,11 +252,11 @@ __global__ __launch_bounds__(XNT, 2) void gemm_bf16x3_kernel(XArgs p) {
     }
 }
 
-int split3(int rows, int k, const float *x, int64_t rs, int64_t cs, int rp, int kp,
+int split3(int rows, int k, const float *x, int64_t rs, int64_t cs, int rp, int kp, int ldp,
            u16 *planes, hipStream_t s) {
-  const size_t plane = (size_t)rp * kp;
+  const size_t plane = (size_t)rp * ldp;
   dim3 grid((kp + 63) / 64, (rp + 63) / 64);
-  hipLaunchKernelGGL(split3_kernel, grid, dim3(256), 0, s, rows, k, x, rs, cs, rp, kp,
+  hipLaunchKernelGGL(split3_kernel, grid, dim3(256), 0, s, rows, k, x, rs, cs, rp, kp, ldp,
                      planes, planes + plane, planes + 2 * plane);
   return hipGetLastError() == hipSuccess ? MUBLIS_OK : MUBLIS_EHIP;
 }
@@ -264,25 +274,30 @@ extern "C" int mublis_gfx950_sgemm_bf16x3_dev(int m, int n, int k, float alpha,
   if (alpha == 0.0f || k == 0)
     return mublis_gfx950_sgemm_dev(m, n, 0, 0.0f, a, rs_a, cs_a, b, rs_b, cs_b, nullptr, 0,
                                    0, nullptr, 0, 0, beta, c, rs_c, cs_c, 0, s);
-  const int mp = (m + XT - 1) / XT * XT, np = (n + XT - 1) / XT * XT;
+  const int mp = (m + XTM - 1) / XTM * XTM, np = (n + XTN - 1) / XTN * XTN;
   const int kp = (k + 2 * XBK - 1) / (2 * XBK) * (2 * XBK);  // whole stage pairs
-  u16 *pa = (u16 *)mublis_rt_alloc(sizeof(u16) * 3 * (size_t)mp * kp);
-  u16 *pb = (u16 *)mublis_rt_alloc(sizeof(u16) * 3 * (size_t)np * kp);
+  // Plane rows are 128 bytes longer than K: with a power-of-two row stride
+  // the rows a workgroup streams at one k offset fall into a few L2 sets and
+  // evict each other (at K = 16384 the L2 hit rate halved and HBM traffic
+  // grew twentyfold).
+  const int ldp = kp + 64;
+  u16 *pa = (u16 *)mublis_rt_alloc(sizeof(u16) * 3 * (size_t)mp * ldp);
+  u16 *pb = (u16 *)mublis_rt_alloc(sizeof(u16) * 3 * (size_t)np * ldp);
   int st = (pa && pb) ? MUBLIS_OK : MUBLIS_ENOMEM;
-  if (st == MUBLIS_OK) st = split3(m, k, a, rs_a, cs_a, mp, kp, pa, s);
+  if (st == MUBLIS_OK) st = split3(m, k, a, rs_a, cs_a, mp, kp, ldp, pa, s);
   // B is split as its transpose: row j of the plane holds B(:, j).
-  if (st == MUBLIS_OK) st = split3(n, k, b, cs_b, rs_b, np, kp, pb, s);
+  if (st == MUBLIS_OK) st = split3(n, k, b, cs_b, rs_b, np, kp, ldp, pb, s);
   if (st == MUBLIS_OK) {
     XArgs p;
-    p.m = m; p.n = n; p.kp = kp; p.mp = mp; p.np = np;
+    p.m = m; p.n = n; p.kp = kp; p.ldp = ldp; p.mp = mp; p.np = np;
     p.alpha = alpha; p.beta = beta;
-    const size_t pa_plane = (size_t)mp * kp, pb_plane = (size_t)np * kp;
+    const size_t pa_plane = (size_t)mp * ldp, pb_plane = (size_t)np * ldp;
     for (int q = 0; q < 3; ++q) {
       p.a[q] = pa + q * pa_plane;
       p.b[q] = pb + q * pb_plane;
     }
     p.c = c; p.rs_c = rs_c; p.cs_c = cs_c;
-    p.tiles_m = mp / XT; p.tiles_n = np / XT; p.ntiles = p.tiles_m * p.tiles_n;
+    p.tiles_m = mp / XTM; p.tiles_n = np / XTN; p.ntiles = p.tiles_m * p.tiles_n;
     hipLaunchKernelGGL(gemm_bf16x3_kernel, dim3(p.ntiles), dim3(XNT), 0, s, p);
     if (hipGetLastError() != hipSuccess) st = MUBLIS_EHIP;
   }
