@@ -25,6 +25,8 @@ template <> struct Mfma<double> {
   // Output row of register r in lane group g (= lane / 16): the FP64 MFMA
   // interleaves rows across lane groups (measured on gfx950).
   __device__ static inline int row(int g, int r) { return g + 4 * r; }
+  // LDS column swizzle of k row k for K-contiguous operands (see Slice).
+  __device__ static inline int swz(int k) { return k & 15; }
 };
 template <> struct Mfma<float> {
   typedef f4 acc_t;
@@ -35,6 +37,7 @@ template <> struct Mfma<float> {
   }
   // The FP32 MFMA keeps 4 consecutive rows per lane group.
   __device__ static inline int row(int g, int r) { return 4 * g + r; }
+  __device__ static inline int swz(int k) { return k & 28; }
 };
 
 constexpr int NT = 256;       // threads per workgroup (4 waves, 2 x 2)
@@ -135,7 +138,14 @@ struct Slice {
   }
 
   // LDS tile is [k][mn] with row stride S.  When the unit-stride dimension
-  // is k (CK), element (c, o) goes to [c][o]; otherwise to [o][c].
+  // is k (CK), element (c, o) goes to [c][o ^ swz(c)]; otherwise to [o][c].
+  // Without the swizzle the lanes of one CK store instruction write the
+  // same column o of 8 different k rows, and with S = TILE + 16 those rows
+  // start in the same bank: an 8-way conflict on every store (measured: 19
+  // conflict cycles per LDS instruction in FP64).  XOR-ing o by the k row
+  // spreads them over the banks; readers apply the same XOR, which only
+  // permutes words inside each 16 (FP64) or 32 (FP32) column group, so
+  // their 16 consecutive columns stay conflict-free.
   template <bool CK>
   __device__ inline void store(T *lds, int tid) const {
 #pragma unroll
@@ -145,7 +155,7 @@ struct Slice {
       const int o = lin / (LC / V);
 #pragma unroll
       for (int e = 0; e < V; ++e) {
-        if (CK) lds[(c + e) * S + o] = v[i][e];
+        if (CK) lds[(c + e) * S + (o ^ Mfma<T>::swz(c + e))] = v[i][e];
         else lds[o * S + c + e] = v[i][e];
       }
     }
@@ -245,11 +255,12 @@ __device__ inline void gemm_tile(const GemmArgs<T> &p, int tm, int tn, T *lds) {
 #pragma unroll
       for (int kk = 0; kk < BK / 4; ++kk) {
         T af[TI], bf[TI];
-        const int row = (kk * 4 + lk) * S;
+        const int kr = kk * 4 + lk, row = kr * S;
+        const int swa = A_KC ? MF::swz(kr) : 0, swb = B_NC ? 0 : MF::swz(kr);
 #pragma unroll
-        for (int i = 0; i < TI; ++i) af[i] = as[row + wm + i * 16 + lr];
+        for (int i = 0; i < TI; ++i) af[i] = as[row + ((wm + i * 16 + lr) ^ swa)];
 #pragma unroll
-        for (int j = 0; j < TI; ++j) bf[j] = bs[row + wn + j * 16 + lr];
+        for (int j = 0; j < TI; ++j) bf[j] = bs[row + ((wn + j * 16 + lr) ^ swb)];
 #pragma unroll
         for (int i = 0; i < TI; ++i)
 #pragma unroll
