@@ -136,96 +136,87 @@ __global__ __launch_bounds__(XNT, 1) void gemm_bf16x3_kernel(XArgs p) {
       small[i][j] = f16v{};
     }
 
-  // Two register sets: the loads of stage st + 2 are issued while stage st
-  // is computed, and land in LDS only at the end of iteration st + 1, so
-  // every load has two compute phases to arrive.
-  u16x8 ra[2][3][2], rb[2][3];
+  // The loads of stage st + 1 are issued while stage st is computed and
+  // land in the other LDS slot at the end of the iteration.  (One register
+  // set: with the 256 accumulators in AGPRs the architectural VGPRs are the
+  // scarce resource, and fragments read early hide LDS latency better than
+  // a second prefetch set.)
+  u16x8 ra[3][2], rb[3];
   const int64_t aoff = (int64_t)(m0 + tid) * p.ldp;
   const int64_t boff = (int64_t)(n0 + brow) * p.ldp + bhalf;
   const int stages = p.kp / XBK;
-  // Loads are issued unconditionally (a stage past the end re-reads the last
-  // one, in bounds and never used): conditional loads would make the
-  // compiler's wait counts conservative and serialise the pipeline.
 #pragma unroll
-  for (int t = 0; t < 2; ++t) {
-    const int64_t k0 = (int64_t)min(t, stages - 1) * XBK;
-#pragma unroll
-    for (int s = 0; s < 3; ++s) {
-      ra[t][s][0] = *reinterpret_cast<const u16x8 *>(p.a[s] + aoff + k0);
-      ra[t][s][1] = *reinterpret_cast<const u16x8 *>(p.a[s] + aoff + k0 + 8);
-      rb[t][s] = *reinterpret_cast<const u16x8 *>(p.b[s] + boff + k0);
-    }
+  for (int s = 0; s < 3; ++s) {
+    ra[s][0] = *reinterpret_cast<const u16x8 *>(p.a[s] + aoff);
+    ra[s][1] = *reinterpret_cast<const u16x8 *>(p.a[s] + aoff + 8);
+    rb[s] = *reinterpret_cast<const u16x8 *>(p.b[s] + boff);
   }
 #pragma unroll
   for (int s = 0; s < 3; ++s) {
-    *reinterpret_cast<u16x8 *>(&As[0][s][tid * XLD]) = ra[0][s][0];
-    *reinterpret_cast<u16x8 *>(&As[0][s][tid * XLD + 8]) = ra[0][s][1];
-    *reinterpret_cast<u16x8 *>(&Bs[0][s][brow * XLD + bhalf]) = rb[0][s];
+    *reinterpret_cast<u16x8 *>(&As[0][s][tid * XLD]) = ra[s][0];
+    *reinterpret_cast<u16x8 *>(&As[0][s][tid * XLD + 8]) = ra[s][1];
+    *reinterpret_cast<u16x8 *>(&Bs[0][s][brow * XLD + bhalf]) = rb[s];
   }
   __syncthreads();
 
   const int fr = lane & 31, fk = (lane >> 5) * 8;
-  // kp is a multiple of 2 * XBK, so stages come in pairs and no branch
-  // interrupts the pipeline (branches make the compiler's wait counts
-  // conservative).  The store after the last stage writes a slot that is
-  // never read again.
+  // kp is a multiple of 2 * XBK, so stages come in pairs (the LDS slot is a
+  // compile-time constant) and no branch interrupts the pipeline (branches
+  // make the compiler's wait counts conservative).  Loads are issued
+  // unconditionally: past the end they re-read the last stage, in bounds,
+  // into a slot that is never read again.
 #pragma unroll 1
   for (int st = 0; st < stages; st += 2) {
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
-      const int cur_st = st + h;
-      const int cur = h;  // LDS slot and register set of stage cur_st
+      const int cur = h;
       {
-        const int64_t k0 = (int64_t)min(cur_st + 2, stages - 1) * XBK;
+        const int64_t k0 = (int64_t)min(st + h + 1, stages - 1) * XBK;
 #pragma unroll
         for (int s = 0; s < 3; ++s) {
-          ra[h][s][0] = *reinterpret_cast<const u16x8 *>(p.a[s] + aoff + k0);
-          ra[h][s][1] = *reinterpret_cast<const u16x8 *>(p.a[s] + aoff + k0 + 8);
-          rb[h][s] = *reinterpret_cast<const u16x8 *>(p.b[s] + boff + k0);
+          ra[s][0] = *reinterpret_cast<const u16x8 *>(p.a[s] + aoff + k0);
+          ra[s][1] = *reinterpret_cast<const u16x8 *>(p.a[s] + aoff + k0 + 8);
+          rb[s] = *reinterpret_cast<const u16x8 *>(p.b[s] + boff + k0);
         }
       }
-      // Keep the order loads -> MFMAs -> LDS stores: the scheduler would
-      // otherwise hoist the stores and sink the loads, shortening the time
-      // each load has to arrive.
       __builtin_amdgcn_sched_barrier(0);
-      bf8 af[3][4], bf[3][2];
+      bf8 bf[3][2];
 #pragma unroll
-      for (int s = 0; s < 3; ++s) {
-#pragma unroll
-        for (int i = 0; i < 4; ++i)
-          af[s][i] = __builtin_bit_cast(
-              bf8, *reinterpret_cast<const u16x8 *>(&As[cur][s][(wm + i * 32 + fr) * XLD + fk]));
+      for (int s = 0; s < 3; ++s)
 #pragma unroll
         for (int j = 0; j < 2; ++j)
           bf[s][j] = __builtin_bit_cast(
               bf8, *reinterpret_cast<const u16x8 *>(&Bs[cur][s][(wn + j * 32 + fr) * XLD + fk]));
-      }
-      // Six products per tile, issued product-major so that consecutive
-      // MFMAs update independent accumulators: (mid, mid), (lo, hi),
-      // (hi, lo), (mid, hi), (hi, mid) into `small`, then (hi, hi) into `big`.
+      // Products grouped by A plane, so one plane of A fragments is live at
+      // a time: hi x (hi -> big; mid, lo -> small), mid x (hi, mid), lo x hi.
+      // Consecutive MFMAs update independent accumulators.
 #pragma unroll
-      for (int t = 0; t < 6; ++t) {
-        const int pa = t == 0 ? 1 : t == 1 ? 2 : t == 2 ? 0 : t == 3 ? 1 : 0;
-        const int pb = t == 0 ? 1 : t == 1 ? 0 : t == 2 ? 2 : t == 3 ? 0 : t == 4 ? 1 : 0;
+      for (int pa = 0; pa < 3; ++pa) {
+        bf8 af[4];
 #pragma unroll
         for (int i = 0; i < 4; ++i)
+          af[i] = __builtin_bit_cast(
+              bf8, *reinterpret_cast<const u16x8 *>(&As[cur][pa][(wm + i * 32 + fr) * XLD + fk]));
 #pragma unroll
-          for (int j = 0; j < 2; ++j) {
-            if (t < 5)
-              small[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(af[pa][i], bf[pb][j],
-                                                                    small[i][j], 0, 0, 0);
-            else
-              big[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(af[0][i], bf[0][j],
-                                                                  big[i][j], 0, 0, 0);
-          }
+        for (int pb = 0; pb < 3 - pa; ++pb)
+#pragma unroll
+          for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int j = 0; j < 2; ++j) {
+              if (pa == 0 && pb == 0)
+                big[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(af[i], bf[0][j], big[i][j],
+                                                                    0, 0, 0);
+              else
+                small[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(af[i], bf[pb][j],
+                                                                      small[i][j], 0, 0, 0);
+            }
       }
       __builtin_amdgcn_sched_barrier(0);
-      // stage cur_st + 1 (loaded one iteration earlier) -> the other slot
 #pragma unroll
       for (int s = 0; s < 3; ++s) {
-        *reinterpret_cast<u16x8 *>(&As[cur ^ 1][s][tid * XLD]) = ra[h ^ 1][s][0];
-        *reinterpret_cast<u16x8 *>(&As[cur ^ 1][s][tid * XLD + 8]) = ra[h ^ 1][s][1];
-        *reinterpret_cast<u16x8 *>(&Bs[cur ^ 1][s][brow * XLD + bhalf]) = rb[h ^ 1][s];
+        *reinterpret_cast<u16x8 *>(&As[cur ^ 1][s][tid * XLD]) = ra[s][0];
+        *reinterpret_cast<u16x8 *>(&As[cur ^ 1][s][tid * XLD + 8]) = ra[s][1];
+        *reinterpret_cast<u16x8 *>(&Bs[cur ^ 1][s][brow * XLD + bhalf]) = rb[s];
       }
       __syncthreads();
     }
