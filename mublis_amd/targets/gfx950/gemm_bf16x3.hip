@@ -11,12 +11,13 @@
 // the result rounds like a native FP32 GEMM.
 //
 // Layout: a split pass writes each operand as three K-contiguous BF16 planes
-// ([row][kp], rows padded to the tile, K to 32, zero filled), so the GEMM
-// kernel streams aligned 16-byte vectors with no bounds checks.  The GEMM
-// kernel: 256 x 128 output tile per workgroup, one workgroup per CU, 4 waves
-// of 128 x 64, each a 4 x 2 grid of v_mfma_f32_32x32x16_bf16 tiles (6 MFMAs
-// per tile per k = 16: 48 MFMAs for 18 LDS fragment reads per lane), LDS
-// double-buffered, the XCD-aware tile order of the FP64/FP32 kernel.
+// ([row][ldp], rows padded to the tile, K to 64, zero filled; ldp = K + 64
+// so that rows never share L2 sets at one k offset), so the GEMM kernel
+// streams aligned 16-byte vectors with no bounds checks.  The GEMM kernel:
+// 256 x 128 output tile per workgroup, one workgroup per CU, 4 waves of
+// 128 x 64, each a 4 x 2 grid of v_mfma_f32_32x32x16_bf16 tiles (6 MFMAs
+// per tile per k = 16), operands streamed by direct-to-LDS loads into a
+// 4-stage LDS ring, the XCD-aware tile order of the FP64/FP32 kernel.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -35,7 +36,6 @@ typedef u16 u16x8 __attribute__((ext_vector_type(8)));
 constexpr int XTM = 256;   // output tile rows
 constexpr int XTN = 128;   // output tile columns
 constexpr int XBK = 16;    // k per stage (one MFMA k step)
-constexpr int XLD = 24;    // LDS row stride in BF16 (48 B: conflict-free b128 reads)
 constexpr int XNT = 256;
 
 __device__ inline u16 bf16_rne(float x) {
@@ -115,17 +115,71 @@ __device__ inline void map_tile_x(int w, int ntiles, int tiles_m, int tiles_n, i
   tn = in / gs;
 }
 
+// The operand stages stream from global memory
+// straight into a 4-slot LDS ring (global_load_lds_dwordx4: lane i of a
+// wave instruction writes bytes [16 i, 16 i + 16) of a 1 KiB block at the
+// M0 base), three stages ahead of the MFMAs and with no registers held for
+// them.  A stage is 36 blocks of 32 rows x 16 k (A: 3 planes x 8 blocks, B:
+// 3 planes x 4); wave w issues blocks w, w + 4, ...  Inside a block, row r's
+// two 16-byte halves are stored swapped when bit 3 of r is set, so the 16
+// lanes of a fragment-read pass (16 rows, one half) hit 16 distinct 16-byte
+// bank groups.  The four slots are separate LDS arrays and the loop is
+// unrolled by four, so every slot index is a compile-time constant.
+constexpr int XRING = 4;
+constexpr int XSTAGE_U16 = 36 * 512;  // one stage: 36 KiB
+
+template <int SLOT>
+__device__ inline u16 *ring_slot(u16 *r0, u16 *r1, u16 *r2, u16 *r3) {
+  return SLOT == 0 ? r0 : SLOT == 1 ? r1 : SLOT == 2 ? r2 : r3;
+}
+
+// One 16-byte-per-lane global -> LDS transfer to the wave-uniform LDS byte
+// address `lds` (lane i lands at lds + 16 i).  Issued from asm so that the
+// compiler, which cannot tell which ring slot a DMA writes, does not drain
+// every DMA in flight before each LDS read; the kernel orders them itself
+// (s_waitcnt vmcnt + barrier).  The instruction has no destination
+// registers.
+#pragma clang diagnostic push
+#pragma clang diagnostic ignored "-Winline-asm"  // m0 is reserved; the clobber is still honoured
+__device__ inline void dma16(const u16 *g, uint32_t lds) {
+  asm volatile("s_mov_b32 m0, %1\n\tglobal_load_lds_dwordx4 %0, off"
+               :
+               : "v"(g), "s"(lds)
+               : "memory", "m0");
+}
+#pragma clang diagnostic pop
+
 __global__ __launch_bounds__(XNT, 1) void gemm_bf16x3_kernel(XArgs p) {
-  __shared__ __attribute__((aligned(16))) u16 As[2][3][XTM * XLD];
-  __shared__ __attribute__((aligned(16))) u16 Bs[2][3][XTN * XLD];
+  __shared__ __attribute__((aligned(1024))) u16 R0[XSTAGE_U16];
+  __shared__ __attribute__((aligned(1024))) u16 R1[XSTAGE_U16];
+  __shared__ __attribute__((aligned(1024))) u16 R2[XSTAGE_U16];
+  __shared__ __attribute__((aligned(1024))) u16 R3[XSTAGE_U16];
   int tm, tn;
   map_tile_x(blockIdx.x, p.ntiles, p.tiles_m, p.tiles_n, tm, tn);
   const int m0 = tm * XTM, n0 = tn * XTN;
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int wm = (wave & 1) * 128, wn = (wave >> 1) * 64;
-  // loaders: A row tid (both 8-element halves of the 16-k stage, 3 planes);
-  // B row tid / 2, half tid % 2 (3 planes)
-  const int brow = tid >> 1, bhalf = (tid & 1) * 8;
+
+  // This lane's source element offset (u16) for each of the wave's 9 blocks
+  // of a stage, at k = 0; block b = wave + 4 j.
+  const int lrow = lane >> 1, lhalf = (lane & 1) ^ ((lane >> 4) & 1);
+  const u16 *src[9];
+#pragma unroll
+  for (int j = 0; j < 9; ++j) {
+    const int b = wave + 4 * j;
+    if (j < 6) {  // A plane b / 8, rows 32 (b % 8) ..
+      src[j] = p.a[b >> 3] + (int64_t)(m0 + 32 * (b & 7) + lrow) * p.ldp + 8 * lhalf;
+    } else {      // B plane j - 6, rows 32 wave ..
+      src[j] = p.b[j - 6] + (int64_t)(n0 + 32 * wave + lrow) * p.ldp + 8 * lhalf;
+    }
+  }
+  auto issue = [&](u16 *slot, int stage) {
+    const int64_t k0 = (int64_t)stage * XBK;
+    const uint32_t base = (uint32_t)(uintptr_t)(__attribute__((address_space(3))) u16 *)slot;
+#pragma unroll
+    for (int j = 0; j < 9; ++j) dma16(src[j] + k0, base + (wave + 4 * j) * 1024);
+  };
 
   f16v big[4][2], small[4][2];
 #pragma unroll
@@ -136,94 +190,68 @@ __global__ __launch_bounds__(XNT, 1) void gemm_bf16x3_kernel(XArgs p) {
       small[i][j] = f16v{};
     }
 
-  // The loads of stage st + 1 are issued while stage st is computed and
-  // land in the other LDS slot at the end of the iteration.  (One register
-  // set: with the 256 accumulators in AGPRs the architectural VGPRs are the
-  // scarce resource, and fragments read early hide LDS latency better than
-  // a second prefetch set.)
-  u16x8 ra[3][2], rb[3];
-  const int64_t aoff = (int64_t)(m0 + tid) * p.ldp;
-  const int64_t boff = (int64_t)(n0 + brow) * p.ldp + bhalf;
-  const int stages = p.kp / XBK;
-#pragma unroll
-  for (int s = 0; s < 3; ++s) {
-    ra[s][0] = *reinterpret_cast<const u16x8 *>(p.a[s] + aoff);
-    ra[s][1] = *reinterpret_cast<const u16x8 *>(p.a[s] + aoff + 8);
-    rb[s] = *reinterpret_cast<const u16x8 *>(p.b[s] + boff);
-  }
-#pragma unroll
-  for (int s = 0; s < 3; ++s) {
-    *reinterpret_cast<u16x8 *>(&As[0][s][tid * XLD]) = ra[s][0];
-    *reinterpret_cast<u16x8 *>(&As[0][s][tid * XLD + 8]) = ra[s][1];
-    *reinterpret_cast<u16x8 *>(&Bs[0][s][brow * XLD + bhalf]) = rb[s];
-  }
-  __syncthreads();
+  const int stages = p.kp / XBK;  // a multiple of XRING
+  issue(R0, 0);
+  issue(R1, min(1, stages - 1));
+  issue(R2, min(2, stages - 1));
 
-  const int fr = lane & 31, fk = (lane >> 5) * 8;
-  // kp is a multiple of 2 * XBK, so stages come in pairs (the LDS slot is a
-  // compile-time constant) and no branch interrupts the pipeline (branches
-  // make the compiler's wait counts conservative).  Loads are issued
-  // unconditionally: past the end they re-read the last stage, in bounds,
-  // into a slot that is never read again.
-#pragma unroll 1
-  for (int st = 0; st < stages; st += 2) {
+  // fragment addresses (u16) inside a slot: row r, half h at
+  // r * 16 + 8 * (h ^ ((r >> 3) & 1)); B starts after the 24 A blocks
+  const int fr = lane & 31, fh = lane >> 5;
+  const int fsw = 8 * (fh ^ ((fr >> 3) & 1));
+  auto stage_compute = [&](const u16 *slot) {
+    bf8 bf[3][2];
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {
-      const int cur = h;
-      {
-        const int64_t k0 = (int64_t)min(st + h + 1, stages - 1) * XBK;
+    for (int s = 0; s < 3; ++s)
 #pragma unroll
-        for (int s = 0; s < 3; ++s) {
-          ra[s][0] = *reinterpret_cast<const u16x8 *>(p.a[s] + aoff + k0);
-          ra[s][1] = *reinterpret_cast<const u16x8 *>(p.a[s] + aoff + k0 + 8);
-          rb[s] = *reinterpret_cast<const u16x8 *>(p.b[s] + boff + k0);
-        }
-      }
-      __builtin_amdgcn_sched_barrier(0);
-      bf8 bf[3][2];
+      for (int j = 0; j < 2; ++j)
+        bf[s][j] = __builtin_bit_cast(bf8, *reinterpret_cast<const u16x8 *>(
+                                               slot + 24 * 512 + s * 2048 +
+                                               (wn + j * 32 + fr) * 16 + fsw));
 #pragma unroll
-      for (int s = 0; s < 3; ++s)
+    for (int pa = 0; pa < 3; ++pa) {
+      bf8 af[4];
 #pragma unroll
-        for (int j = 0; j < 2; ++j)
-          bf[s][j] = __builtin_bit_cast(
-              bf8, *reinterpret_cast<const u16x8 *>(&Bs[cur][s][(wn + j * 32 + fr) * XLD + fk]));
-      // Products grouped by A plane, so one plane of A fragments is live at
-      // a time: hi x (hi -> big; mid, lo -> small), mid x (hi, mid), lo x hi.
-      // Consecutive MFMAs update independent accumulators.
+      for (int i = 0; i < 4; ++i)
+        af[i] = __builtin_bit_cast(bf8, *reinterpret_cast<const u16x8 *>(
+                                            slot + pa * 4096 + (wm + i * 32 + fr) * 16 + fsw));
 #pragma unroll
-      for (int pa = 0; pa < 3; ++pa) {
-        bf8 af[4];
+      for (int pb = 0; pb < 3 - pa; ++pb)
 #pragma unroll
         for (int i = 0; i < 4; ++i)
-          af[i] = __builtin_bit_cast(
-              bf8, *reinterpret_cast<const u16x8 *>(&As[cur][pa][(wm + i * 32 + fr) * XLD + fk]));
 #pragma unroll
-        for (int pb = 0; pb < 3 - pa; ++pb)
-#pragma unroll
-          for (int i = 0; i < 4; ++i)
-#pragma unroll
-            for (int j = 0; j < 2; ++j) {
-              if (pa == 0 && pb == 0)
-                big[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(af[i], bf[0][j], big[i][j],
-                                                                    0, 0, 0);
-              else
-                small[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(af[i], bf[pb][j],
-                                                                      small[i][j], 0, 0, 0);
-            }
-      }
-      __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-      for (int s = 0; s < 3; ++s) {
-        *reinterpret_cast<u16x8 *>(&As[cur ^ 1][s][tid * XLD]) = ra[s][0];
-        *reinterpret_cast<u16x8 *>(&As[cur ^ 1][s][tid * XLD + 8]) = ra[s][1];
-        *reinterpret_cast<u16x8 *>(&Bs[cur ^ 1][s][brow * XLD + bhalf]) = rb[s];
-      }
-      __syncthreads();
+          for (int j = 0; j < 2; ++j) {
+            if (pa == 0 && pb == 0)
+              big[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(af[i], bf[0][j], big[i][j], 0,
+                                                                  0, 0);
+            else
+              small[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(af[i], bf[pb][j],
+                                                                    small[i][j], 0, 0, 0);
+          }
     }
+  };
+  // Stage s: wait for this wave's blocks of s (the 18 of s + 1 and s + 2 may
+  // still be in flight), barrier (every wave's blocks of s have landed, and
+  // every wave is done with slot s - 1), refill slot s - 1 with stage s + 3
+  // (clamped: past the end it re-reads the last stage into a slot that is
+  // never read again), compute stage s.
+#define X3_STEP(U)                                                              \
+  {                                                                             \
+    asm volatile("s_waitcnt vmcnt(18)" ::: "memory");                           \
+    __builtin_amdgcn_s_barrier();                                               \
+    issue(ring_slot<(U + 3) % XRING>(R0, R1, R2, R3), min(s4 + U + 3, stages - 1)); \
+    stage_compute(ring_slot<U>(R0, R1, R2, R3));                                \
   }
+#pragma unroll 1
+  for (int s4 = 0; s4 < stages; s4 += XRING) {
+    X3_STEP(0)
+    X3_STEP(1)
+    X3_STEP(2)
+    X3_STEP(3)
+  }
+#undef X3_STEP
+  asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
 
-  // 32 x 32 MFMA output: lane l, register r holds column l%32 and row
-  // 8*(r/4) + 4*(l/32) + r%4 of its tile (measured on gfx950).
   const float alpha = p.alpha, beta = p.beta;
 #pragma unroll
   for (int i = 0; i < 4; ++i)
@@ -266,7 +294,7 @@ extern "C" int mublis_gfx950_sgemm_bf16x3_dev(int m, int n, int k, float alpha,
     return mublis_gfx950_sgemm_dev(m, n, 0, 0.0f, a, rs_a, cs_a, b, rs_b, cs_b, nullptr, 0,
                                    0, nullptr, 0, 0, beta, c, rs_c, cs_c, 0, s);
   const int mp = (m + XTM - 1) / XTM * XTM, np = (n + XTN - 1) / XTN * XTN;
-  const int kp = (k + 2 * XBK - 1) / (2 * XBK) * (2 * XBK);  // whole stage pairs
+  const int kp = (k + 4 * XBK - 1) / (4 * XBK) * (4 * XBK);  // whole rounds of the ring
   // Plane rows are 128 bytes longer than K: with a power-of-two row stride
   // the rows a workgroup streams at one k offset fall into a few L2 sets and
   // evict each other (at K = 16384 the L2 hit rate halved and HBM traffic
