@@ -47,6 +47,10 @@ def parse():
     p.add_argument("--serve-requests", type=int, default=2000)
     p.add_argument("--serve-rate", type=float, default=SERVE_RATE,
                    help="Poisson arrival rate of the serving workload, requests/s")
+    p.add_argument("--serve-streams", type=int, default=0,
+                   help="device streams of the serving loop (0: the library default)")
+    p.add_argument("--serve-slots", type=int, default=0,
+                   help="launches in flight per stream (0: the library default)")
     p.add_argument("--precision-n", type=int, default=16384,
                    help="SGEMM size of the native-vs-BF16x3 comparison (0: skip)")
     p.add_argument("--cpu-slice", type=int, default=2048,
@@ -382,7 +386,8 @@ def run_serving(args, torch, M, lib, extra):
     synchronous cblas_* calls."""
     import mublis_amd.serving as S
     from mublis_amd.workloads import serving_requests
-    srv = S.Server(hbm_budget=64 << 30)
+    srv = S.Server(hbm_budget=64 << 30, streams=args.serve_streams,
+                   slots_per_stream=args.serve_slots)
     try:
         reqs, _ = serving_requests(torch, lib, srv, count=args.serve_requests)
         flop = sum(r["flop"] for r in reqs)

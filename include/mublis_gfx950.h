@@ -307,7 +307,7 @@ enum {
 typedef struct {
   size_t hbm_budget;  /* bytes reserved for matrices and workspace (0: 1 GiB)   */
   int streams;        /* device streams launches are spread over (0: 4)         */
-  int slots_per_stream; /* launches in flight per stream (0: 2)                */
+  int slots_per_stream; /* launches in flight per stream (0: 4)                */
   int max_group;      /* requests per grouped launch (0: 256)                   */
   double split_flop;  /* a GEMM-like request above this many flop gets its own
                          launch, so it does not hold small ones back (0: 2e10;

@@ -522,7 +522,7 @@ int mublis_server_create(const mublis_server_config_t *cfg, mublis_server_t *out
   if (cfg) s->cfg = *cfg;
   if (!s->cfg.hbm_budget) s->cfg.hbm_budget = size_t(1) << 30;
   if (s->cfg.streams <= 0) s->cfg.streams = 4;
-  if (s->cfg.slots_per_stream <= 0) s->cfg.slots_per_stream = 2;
+  if (s->cfg.slots_per_stream <= 0) s->cfg.slots_per_stream = 4;
   if (s->cfg.max_group <= 0) s->cfg.max_group = 256;
   if (s->cfg.split_flop == 0) s->cfg.split_flop = 2e10;
   (void)hipGetDevice(&s->device);

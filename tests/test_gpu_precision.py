@@ -89,6 +89,35 @@ def test_bf16x3_vs_oracle_and_native(lib, orc, ta, tb):
     assert es <= 1.5 * en + 1e-30
 
 
+RAGGED = [(1, 1, 1), (1, 300, 70), (257, 1, 33), (129, 131, 1), (255, 129, 63),
+          (256, 128, 64), (257, 127, 65), (513, 200, 129), (64, 700, 1000)]
+
+
+@pytest.mark.parametrize("m,n,k", RAGGED)
+def test_bf16x3_ragged_shapes(lib, orc, m, n, k):
+    """Shapes around the 256 x 128 workgroup tile and the 64-deep K padding
+    of the split planes, single rows, columns and K = 1, beta != 0, and a
+    row-major C (C strides swapped) for the odd cases."""
+    rng = np.random.default_rng(m * 7 + n * 13 + k)
+    ta, tb = (m + k) % 2, (n + k) % 2
+    alpha, beta = -0.6, 0.4
+    a = col(rng.uniform(-1, 1, (k, m) if ta else (m, k)).astype(np.float32))
+    b = col(rng.uniform(-1, 1, (n, k) if tb else (k, n)).astype(np.float32))
+    c0 = col(rng.uniform(-1, 1, (m, n)).astype(np.float32))
+    want = c0.copy(order="F")
+    assert orc.call(np.float32, "gemm", 102, 112 if ta else 111, 112 if tb else 111, m, n, k,
+                    alpha, a, a.shape[0], b, b.shape[0], beta, want, m) == 0
+    rowc = (m + n) % 2 == 1
+    got = np.ascontiguousarray(c0) if rowc else c0.copy(order="F")
+    rs_c, cs_c = (n, 1) if rowc else (1, m)
+    st = lib.mublis_sgemm_bf16x3(int(ta), int(tb), m, n, k, alpha, M.ptr(a), 1, a.shape[0],
+                                 M.ptr(b), 1, b.shape[0], beta, M.ptr(got), rs_c, cs_c)
+    assert st == 0
+    scale = abs(alpha) * np.linalg.norm(a) * np.linalg.norm(b) + abs(beta) * np.linalg.norm(c0)
+    err = np.linalg.norm(np.asarray(got, np.float64) - want)
+    assert err <= 2 * max(k, 1) * EPS32 * scale, (err, scale)
+
+
 def test_bf16x3_row_scaled_inputs(lib):
     """Entries N(0,1) * 10^U(-3,3) per row (the precision workload's
     distribution) at 512^3."""
