@@ -52,10 +52,10 @@ __device__ inline void run_tile(const GemmArgs<float> &p, int tm, int tn, int la
   }
 }
 
-// MAXT: largest tile the launch runs, which sizes LDS (and registers) and
-// so the workgroups per CU: the 128-tile items run 2 per CU, the 32/64-tile
-// items -- most of a batch of small problems -- 4 per CU in their own
-// launch, which hides more of their short K loops' latency.
+// MAXT: the tile shape of every item of the launch, which sizes LDS (and
+// registers) and so the workgroups per CU: 128-tile items run 2 per CU,
+// 64-tile items 4 and 32-tile items 6, so the many small items of a batch
+// of small problems hide more of their short K loops' latency.
 template <int MAXT, int WG_PER_CU>
 __global__ __launch_bounds__(NT, WG_PER_CU) void grouped_kernel(
     const GemmArgs<float> *__restrict__ probs, const GroupItem *__restrict__ items,
@@ -65,12 +65,8 @@ __global__ __launch_bounds__(NT, WG_PER_CU) void grouped_kernel(
     const GroupItem w = items[it];
     const GemmArgs<float> p = probs[w.prob];
     const int tile = w.kind >> 2, layout = w.kind & 3;
-    if (MAXT == 64) {  // the launch of 32/64-tile items
-      if (tile == 0) run_tile<32>(p, w.tm, w.tn, layout, lds);
-      else run_tile<64>(p, w.tm, w.tn, layout, lds);
-    } else {  // the launch of 128-tile items only
-      run_tile<128>(p, w.tm, w.tn, layout, lds);
-    }
+    (void)tile;  // every item of a launch has the launch's tile shape
+    run_tile<MAXT>(p, w.tm, w.tn, layout, lds);
     __syncthreads();  // the next item restages lds
   }
 }
@@ -145,7 +141,7 @@ bool aligned16f(const float *x, int64_t ld) {
 struct mublis_grouped_plan_s {
   GemmArgs<float> *d_probs = nullptr;
   GroupItem *d_items = nullptr;
-  int nitems = 0, nbig = 0;
+  int nitems = 0, ncls[3] = {0, 0, 0};
   hipStream_t cap = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -169,8 +165,8 @@ namespace {
 
 struct Work {
   std::vector<GemmArgs<float>> probs;
-  std::vector<GroupItem> items;  // the 128-tile items first, then the others
-  int nbig = 0;                  // number of 128-tile items
+  std::vector<GroupItem> items;  // by tile shape (128, 64, 32), each by cost
+  int ncls[3] = {0, 0, 0};       // items per tile shape (index 0: 32, 1: 64, 2: 128)
   double flops = 0, bytes = 0;
   int hist[3] = {0, 0, 0};
 };
@@ -244,20 +240,20 @@ int build_work(const mublis_sbatch_op_t *ops, int count, int flags, Work &w) {
         cost.push_back((double)T * T * std::max(k, 1) * (syr2k ? 2 : 1) + 64.0 * T * T);
       }
   }
-  // 128-tile items first (their own launch), then by cost, largest first,
-  // so each launch's cyclic walk ends with small items (LPT order)
+  // one launch per tile shape, largest shape first; within a launch the
+  // largest items first, so its cyclic walk ends with small items (LPT)
   std::vector<int> order(items.size());
   for (size_t i = 0; i < order.size(); ++i) order[i] = (int)i;
-  auto big = [&](int x) { return (items[x].kind >> 2) == 2; };
+  auto cls = [&](int x) { return items[x].kind >> 2; };
   std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
-    if (big(x) != big(y)) return big(x);
+    if (cls(x) != cls(y)) return cls(x) > cls(y);
     return cost[x] > cost[y];
   });
   w.items.resize(items.size());
-  w.nbig = 0;
+  for (int c = 0; c < 3; ++c) w.ncls[c] = 0;
   for (size_t i = 0; i < order.size(); ++i) {
     w.items[i] = items[order[i]];
-    w.nbig += big(order[i]);
+    w.ncls[cls(order[i])]++;
   }
   return MUBLIS_OK;
 }
@@ -270,19 +266,23 @@ int persistent_grid(int nitems, int per_cu) {
   return std::max(1, std::min(nitems, per_cu * cus));
 }
 
-// The two launches of a work list, back to back on one stream (either may
-// be empty): the 128-tile items, then the 32/64-tile items.  Measured on the
-// batched workload, this beats running the two as parallel graph branches
-// (1.86 vs 2.69 ms) and one launch for all items at 2 workgroups per CU
-// (2.27 ms).
-int launch_items(const GemmArgs<float> *probs, const GroupItem *items, int nitems, int nbig,
+// The launches of a work list, back to back on one stream (any may be
+// empty): the 128-tile items, the 64-tile items, the 32-tile items.
+// Measured on the batched workload, back-to-back launches beat parallel
+// graph branches (1.86 vs 2.69 ms for two classes) and one launch for all
+// items at 2 workgroups per CU (2.27 ms).
+int launch_items(const GemmArgs<float> *probs, const GroupItem *items, const int *ncls,
                  hipStream_t s) {
-  if (nbig > 0)
-    hipLaunchKernelGGL((grouped_kernel<128, 2>), dim3(persistent_grid(nbig, 2)), dim3(NT), 0,
-                       s, probs, items, nbig);
-  if (nitems > nbig)
-    hipLaunchKernelGGL((grouped_kernel<64, 4>), dim3(persistent_grid(nitems - nbig, 4)),
-                       dim3(NT), 0, s, probs, items + nbig, nitems - nbig);
+  const int n128 = ncls[2], n64 = ncls[1], n32 = ncls[0];
+  if (n128 > 0)
+    hipLaunchKernelGGL((grouped_kernel<128, 2>), dim3(persistent_grid(n128, 2)), dim3(NT), 0,
+                       s, probs, items, n128);
+  if (n64 > 0)
+    hipLaunchKernelGGL((grouped_kernel<64, 4>), dim3(persistent_grid(n64, 4)), dim3(NT), 0, s,
+                       probs, items + n128, n64);
+  if (n32 > 0)
+    hipLaunchKernelGGL((grouped_kernel<32, 6>), dim3(persistent_grid(n32, 6)), dim3(NT), 0, s,
+                       probs, items + n128 + n64, n32);
   return hipGetLastError() == hipSuccess ? MUBLIS_OK : MUBLIS_EHIP;
 }
 
@@ -306,7 +306,7 @@ int mublis_sgrouped_plan_create(const mublis_sbatch_op_t *ops, int count, int fl
   const std::vector<GemmArgs<float>> &probs = w.probs;
   const std::vector<GroupItem> &sorted = w.items;
   plan->nitems = (int)sorted.size();
-  plan->nbig = w.nbig;
+  for (int c = 0; c < 3; ++c) plan->ncls[c] = w.ncls[c];
   st = MUBLIS_OK;
   if (!probs.empty() && hipMalloc(&plan->d_probs, sizeof(GemmArgs<float>) * probs.size()))
     st = MUBLIS_ENOMEM;
@@ -324,8 +324,7 @@ int mublis_sgrouped_plan_create(const mublis_sbatch_op_t *ops, int count, int fl
   if (!st && hipStreamCreateWithFlags(&plan->cap, hipStreamNonBlocking)) st = MUBLIS_EHIP;
   if (!st && hipStreamBeginCapture(plan->cap, hipStreamCaptureModeThreadLocal)) st = MUBLIS_EHIP;
   if (!st) {
-    const int l = launch_items(plan->d_probs, plan->d_items, plan->nitems, plan->nbig,
-                               plan->cap);
+    const int l = launch_items(plan->d_probs, plan->d_items, plan->ncls, plan->cap);
     hipError_t e2 = hipStreamEndCapture(plan->cap, &plan->graph);
     if (l || e2) st = MUBLIS_EHIP;
   }
@@ -362,7 +361,7 @@ int mublis_sgrouped_launch_ws(const mublis_sbatch_op_t *ops, int count, void *ho
   hipStream_t s = (hipStream_t)(stream ? stream : mublis_rt_stream());
   if (hipMemcpyAsync(dev_ws, h, pb + ib, hipMemcpyHostToDevice, s)) return MUBLIS_EHIP;
   return launch_items((const GemmArgs<float> *)dev_ws, (const GroupItem *)((char *)dev_ws + pb),
-                      (int)w.items.size(), w.nbig, s);
+                      w.ncls, s);
 }
 
 int mublis_grouped_plan_execute(mublis_grouped_plan_t plan, void *stream) {
