@@ -99,10 +99,12 @@ int mublis_gfx950_spack_dev(int m, int n, const float *src, int64_t rs,
  * lower (lower = 1) or upper triangular as viewed through (rs_a, cs_a); unit
  * = 1 means the diagonal is 1 and is never read.  Right-side and transposed
  * problems are mapped onto this form by the caller exactly as the reference
- * does (frame/l3/trsm.c:35-54).  Diagonal blocks are solved in FP64 (also for
- * float data) with reciprocal diagonals; the rest is MFMA GEMM.  Replaces the
- * TRSM loop nest and the GEMMTRSM micro-kernels (frame/l3/trsm.c:87-431,
- * include/mublis_instantiate.h:95-111).
+ * does (frame/l3/trsm.c:35-54).  The 64 x 64 diagonal blocks are inverted in
+ * FP64 (also for float data); narrow B is solved by one pipelined GEMMTRSM
+ * launch, wide B by a recursive schedule of MFMA GEMMs with GEMMTRSM leaves.
+ * Synchronises the stream before returning (it releases its workspace).
+ * Replaces the TRSM loop nest and the GEMMTRSM micro-kernels
+ * (frame/l3/trsm.c:87-431, include/mublis_instantiate.h:95-111).
  */
 int mublis_gfx950_dtrsm_dev(int lower, int unit, int m, int n, double alpha,
                             const double *a, int64_t rs_a, int64_t cs_a,

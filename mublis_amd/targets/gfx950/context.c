@@ -4,13 +4,14 @@
  * Block sizes describe the device tiling:
  *   mr x nr = 64 x 64   one wavefront's MFMA tile (4 x 4 tiles of 16 x 16);
  *   mc x nc = 128 x 128 one workgroup's output tile (2 x 2 wavefronts);
- *   kc      = 64        the TRSM diagonal block solved by one leaf launch.
+ *   kc      = 64        the TRSM diagonal block (inverted in FP64 as a whole).
  *
  * The micro-kernels honour the reference's packed-panel contract
  * (include/mublis_instantiate.h:34-111) by running the tile on the GPU: the
  * packed panels are staged to device memory if needed, the MR x NR tile is
  * computed by the same MFMA GEMM kernel the driver launches, and the
- * GEMMTRSM solve by the FP64 diagonal-block kernel of the TRSM schedule.
+ * GEMMTRSM solve by the TRSM schedule's FP64 diagonal-block inversion and
+ * inverse multiply.
  * The frame itself never calls them per tile; they exist so that code
  * written against the plugin seam works with this target.
  */
