@@ -240,30 +240,66 @@ int build_work(const mublis_sbatch_op_t *ops, int count, int flags, Work &w) {
         cost.push_back((double)T * T * std::max(k, 1) * (syr2k ? 2 : 1) + 64.0 * T * T);
       }
   }
-  // one launch per tile shape, largest shape first; within a launch the
-  // largest items first, so its cyclic walk ends with small items (LPT)
-  std::vector<int> order(items.size());
-  for (size_t i = 0; i < order.size(); ++i) order[i] = (int)i;
-  auto cls = [&](int x) { return items[x].kind >> 2; };
-  std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
-    if (cls(x) != cls(y)) return cls(x) > cls(y);
-    return cost[x] > cost[y];
-  });
-  w.items.resize(items.size());
-  for (int c = 0; c < 3; ++c) w.ncls[c] = 0;
-  for (size_t i = 0; i < order.size(); ++i) {
-    w.items[i] = items[order[i]];
-    w.ncls[cls(order[i])]++;
+  // One launch per tile shape, largest shape first.  Within a launch:
+  // whole problems are dealt to the 8 XCDs, largest first onto the least
+  // loaded one (LPT), and the per-XCD lists are interleaved so that list
+  // position p runs on XCD p % 8 (workgroup p % grid, grids being multiples
+  // of 8, and hardware workgroup ids dealt to XCDs round robin): the tiles of
+  // one problem then share that XCD's L2 instead of re-reading their A and B
+  // panels on eight.  Each list runs largest items first, so the cyclic walk
+  // ends with small items.
+  w.items.clear();
+  w.items.reserve(items.size());
+  for (int c = 2; c >= 0; --c) {
+    std::vector<int> probs_c;  // problems of this class, by cost of their tiles
+    std::vector<double> pcost(w.probs.size(), 0.0);
+    std::vector<std::vector<int>> ptiles(w.probs.size());
+    for (size_t i = 0; i < items.size(); ++i) {
+      if ((items[i].kind >> 2) != c) continue;
+      const int pr = items[i].prob;
+      if (ptiles[pr].empty()) probs_c.push_back(pr);
+      ptiles[pr].push_back((int)i);
+      pcost[pr] += cost[i];
+    }
+    std::stable_sort(probs_c.begin(), probs_c.end(), [&](int x, int y) {
+      return cost[ptiles[x][0]] > cost[ptiles[y][0]];
+    });
+    std::vector<int> lists[8];
+    double load[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (int pr : probs_c) {
+      const int x = (int)(std::min_element(load, load + 8) - load);
+      load[x] += pcost[pr];
+      lists[x].insert(lists[x].end(), ptiles[pr].begin(), ptiles[pr].end());
+    }
+    size_t pos[8] = {0, 0, 0, 0, 0, 0, 0, 0}, left = 0;
+    for (int x = 0; x < 8; ++x) left += lists[x].size();
+    w.ncls[c] = (int)left;
+    for (size_t p = 0; left > 0; ++p) {
+      int x = (int)(p % 8);
+      if (pos[x] == lists[x].size()) {  // this XCD's list is done: take from the longest
+        size_t best = 0;
+        for (int y = 0; y < 8; ++y)
+          if (lists[y].size() - pos[y] > best) {
+            best = lists[y].size() - pos[y];
+            x = y;
+          }
+      }
+      w.items.push_back(items[lists[x][pos[x]++]]);
+      --left;
+    }
   }
   return MUBLIS_OK;
 }
 
+// Persistent grid: per_cu workgroups per CU, at most one per item, rounded
+// up to a multiple of the 8 XCDs (the work lists are laid out for that).
 int persistent_grid(int nitems, int per_cu) {
   int dev = 0, cus = 256;
   if (hipGetDevice(&dev) != hipSuccess ||
       hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
     cus = 256;
-  return std::max(1, std::min(nitems, per_cu * cus));
+  const int g = std::max(1, std::min(nitems, per_cu * cus));
+  return (g + 7) / 8 * 8;
 }
 
 // The launches of a work list, back to back on one stream (any may be
