@@ -275,24 +275,42 @@ __device__ inline void gemm_tile(const GemmArgs<T> &p, int tm, int tn, T *lds) {
   }
 
   // Epilogue.  MFMA 16x16 output layout: lane l, register r holds column
-  // l%16 and row MF::row(l/16, r) of its 16 x 16 tile.
+  // l%16 and row MF::row(l/16, r) of its 16 x 16 tile.  C is read and written
+  // through the same pointer, so the compiler must assume a store may alias
+  // the next load: reading each element just before writing it would make
+  // every element a load -> wait -> store round trip.  The C values of a row
+  // of tiles are therefore all loaded before any of them is written (one
+  // memory latency per row of tiles instead of one per element).
   const T alpha = p.alpha, beta = p.beta;
+  auto c_ok = [&](int row, int col) {
+    return row < p.m && col < p.n && !(p.ji == 1 && col > row) && !(p.ji == 2 && col < row);
+  };
 #pragma unroll
   for (int i = 0; i < TI; ++i) {
+    T cv[TI][4];
+    if (beta != T(0)) {
+#pragma unroll
+      for (int j = 0; j < TI; ++j) {
+        const int col = n0 + wn + j * 16 + lr;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int row = m0 + wm + i * 16 + MF::row(lk, r);
+          cv[j][r] = c_ok(row, col) ? p.c[(int64_t)row * p.rs_c + (int64_t)col * p.cs_c] : T(0);
+        }
+      }
+    }
 #pragma unroll
     for (int j = 0; j < TI; ++j) {
       const int col = n0 + wn + j * 16 + lr;
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int row = m0 + wm + i * 16 + MF::row(lk, r);
-        if (row >= p.m || col >= p.n) continue;
-        if (p.ji == 1 && col > row) continue;
-        if (p.ji == 2 && col < row) continue;
+        if (!c_ok(row, col)) continue;
         T *cp = p.c + (int64_t)row * p.rs_c + (int64_t)col * p.cs_c;
         const T prod = alpha * acc[i][j][r];
         if (beta == T(0)) *cp = prod;
-        else if (beta == T(1)) *cp = *cp + prod;
-        else *cp = prod + beta * *cp;
+        else if (beta == T(1)) *cp = cv[j][r] + prod;
+        else *cp = prod + beta * cv[j][r];
       }
     }
   }
