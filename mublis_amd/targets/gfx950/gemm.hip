@@ -36,7 +36,11 @@ template <typename T, int TILE, bool A_KC, bool B_NC>
 __global__ __launch_bounds__(NT, Tile<TILE>::MIN_WG_PER_CU) void gemm_kernel(GemmArgs<T> p) {
   __shared__ T lds[LdsSize<T, TILE>::value];
   int tm, tn;
-  map_tile(blockIdx.x, p.ntiles, p.tiles_m, p.tiles_n, p.tri, p.ji, tm, tn);
+  if (p.lpt) {
+    if (!map_tile_lpt(blockIdx.x, p.tiles_m, p.tiles_n, p.lpt, p.lpt_g, tm, tn)) return;
+  } else {
+    map_tile(blockIdx.x, p.ntiles, p.tiles_m, p.tiles_n, p.tri, p.ji, tm, tn);
+  }
   gemm_tile<T, TILE, A_KC, B_NC>(p, tm, tn, lds);
 }
 
@@ -67,7 +71,15 @@ hipError_t launch(GemmArgs<T> p, bool akc, bool bnc, hipStream_t s) {
   p.tiles_m = (p.m + TILE - 1) / TILE;
   p.tiles_n = (p.n + TILE - 1) / TILE;
   p.ntiles = p.tri ? p.tiles_m * (p.tiles_m + 1) / 2 : p.tiles_m * p.tiles_n;
-  const dim3 grid(p.ntiles), block(NT);
+  int wgs = p.ntiles;
+  if (p.lpt) {
+    const int lines = p.lpt <= 2 ? p.tiles_m : p.tiles_n;
+    const int per = p.lpt <= 2 ? p.tiles_n : p.tiles_m;
+    p.lpt_g = lpt_unit(lines, per);
+    const int64_t units = (int64_t)lines * ((per + p.lpt_g - 1) / p.lpt_g);
+    wgs = (int)(8 * ((units + 7) / 8) * p.lpt_g);
+  }
+  const dim3 grid(wgs), block(NT);
   if (akc && bnc) hipLaunchKernelGGL((gemm_kernel<T, TILE, true, true>), grid, block, 0, s, p);
   else if (akc) hipLaunchKernelGGL((gemm_kernel<T, TILE, true, false>), grid, block, 0, s, p);
   else if (bnc) hipLaunchKernelGGL((gemm_kernel<T, TILE, false, true>), grid, block, 0, s, p);
@@ -138,16 +150,27 @@ int gemm_dev(int m, int n, int k, T alpha, const T *a, int64_t rs_a, int64_t cs_
     p.tri = (ji != 0 && m == n) ? 1 : 0;
     p.tri_a = a2 ? 0 : tri_a;
     p.tri_b = a2 ? 0 : tri_b;
+    // A triangular operand makes tile costs uneven (K from BK to k): walk
+    // the tiles long-first, balanced over the XCDs.
+    p.lpt = 0;
+    p.lpt_g = 1;
+    if (!p.tri) {
+      if (p.tri_a) p.lpt = p.tri_a == 1 ? 1 : 2;
+      else if (p.tri_b) p.lpt = p.tri_b == 2 ? 3 : 4;
+    }
     p.vec_a = aligned16(a, akc ? rs_a : cs_a);
     p.vec_b = aligned16(b, bnc ? rs_b : cs_b);
     p.vec_a2 = a2 ? aligned16(a2, akc ? rs_a2 : cs_a2) : 0;
     p.vec_b2 = b2 ? aligned16(b2, bnc ? rs_b2 : cs_b2) : 0;
     // 128 x 128 tiles unless that grid would give fewer than two
-    // workgroups per CU (256 CUs).
+    // workgroups per CU (256 CUs).  With uneven tiles (long-first walk) a
+    // single round of 128 tiles lasts as long as its longest tile, so 128
+    // tiles need at least four rounds there.
     const int64_t t128 = (int64_t)((m + 127) / 128) * ((n + 127) / 128);
     const int64_t n128 = p.tri ? ((m + 127) / 128) * ((m + 127) / 128 + 1) / 2 : t128;
-    hipError_t e = n128 >= 512 ? launch<T, 128>(p, akc, bnc, s)
-                               : launch<T, 64>(p, akc, bnc, s);
+    const int64_t need = p.lpt ? 2048 : 512;
+    hipError_t e = n128 >= need ? launch<T, 128>(p, akc, bnc, s)
+                                : launch<T, 64>(p, akc, bnc, s);
     if (e != hipSuccess) st = MUBLIS_EHIP;
     if (st == MUBLIS_OK && a2 && !same2) {
       // Layouts differ between the two products: add the second one in a
@@ -211,6 +234,20 @@ template int gfx950_gemm<float>(int, int, int, float, const float *, int64_t, in
 template int gfx950_gemm<double>(int, int, int, double, const double *, int64_t,
                                  int64_t, const double *, int64_t, int64_t, double,
                                  double *, int64_t, int64_t, hipStream_t);
+
+template <typename T>
+int gfx950_gemm_tri_a(int m, int n, int k, T alpha, const T *a, int64_t rs_a, int64_t cs_a,
+                      const T *b, int64_t rs_b, int64_t cs_b, T beta, T *c, int64_t rs_c,
+                      int64_t cs_c, int tri_a, hipStream_t s) {
+  return gemm_dev<T>(m, n, k, alpha, a, rs_a, cs_a, b, rs_b, cs_b, nullptr, 0, 0, nullptr, 0,
+                     0, beta, c, rs_c, cs_c, 0, (void *)s, tri_a, 0);
+}
+template int gfx950_gemm_tri_a<float>(int, int, int, float, const float *, int64_t, int64_t,
+                                      const float *, int64_t, int64_t, float, float *,
+                                      int64_t, int64_t, int, hipStream_t);
+template int gfx950_gemm_tri_a<double>(int, int, int, double, const double *, int64_t,
+                                       int64_t, const double *, int64_t, int64_t, double,
+                                       double *, int64_t, int64_t, int, hipStream_t);
 
 extern "C" int mublis_gfx950_dgemm_tri_dev(int m, int n, int k, double alpha,
                                            const double *a, int64_t rs_a, int64_t cs_a,

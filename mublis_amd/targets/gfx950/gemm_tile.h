@@ -72,7 +72,40 @@ struct GemmArgs {
   // p < j, 2: for p > j.  Values are still read as stored (the caller has
   // materialised the zeros), so rounding K bounds outward is harmless.
   int tri_a, tri_b;
+  // Long-first walk for a triangular operand (map_tile_lpt): 0 off; 1 / 2
+  // tile rows are the lines, longest at the high / low end; 3 / 4 the same
+  // for tile columns.  lpt_g: tiles per dealt unit (a whole line, or 1).
+  int lpt, lpt_g;
 };
+
+// Long-first walk: with a triangular operand, a tile's K extent grows
+// linearly along one axis (a "line" of tiles shares one K), so one launch
+// mixes tiles of K = BK .. k.  Units of g tiles -- whole lines when there
+// are enough of them to balance (the tiles of a line then share A or B in
+// one L2), else single tiles -- are dealt to the 8 XCDs in snake order of
+// decreasing K (XCD x gets units x, 15 - x, 16 + x, ...), so every XCD
+// receives the same total K, and each XCD starts its longest units first,
+// which lets the short tiles fill the slots the long ones leave (longest
+// processing time first).  The grid is 8 * ceil(units / 8) * g workgroups;
+// returns false for the padding workgroups.
+__device__ inline bool map_tile_lpt(int w, int tiles_m, int tiles_n, int lpt, int g, int &tm,
+                                    int &tn) {
+  const bool rows = lpt <= 2;
+  const int lines = rows ? tiles_m : tiles_n, per = rows ? tiles_n : tiles_m;
+  const int upl = (per + g - 1) / g;  // units per line
+  const int xcd = w % 8, local = w / 8;
+  const int k = local / g, oi = local - k * g;
+  const int d = 8 * k + ((k & 1) ? 7 - xcd : xcd);  // unit rank in decreasing K
+  const int dl = d / upl, o = (d - dl * upl) * g + oi;
+  if (dl >= lines || o >= per) return false;
+  const int line = (lpt == 1 || lpt == 3) ? lines - 1 - dl : dl;
+  tm = rows ? line : o;
+  tn = rows ? o : line;
+  return true;
+}
+
+// Unit size for map_tile_lpt: whole lines once every XCD gets four or more.
+__host__ __device__ inline int lpt_unit(int lines, int per) { return lines >= 32 ? per : 1; }
 
 // Map a hardware workgroup id to an output tile.  Consecutive hardware ids
 // land on different XCDs (round robin over 8), so ids are first regrouped so

@@ -28,6 +28,12 @@ template <typename T>
 int gfx950_gemm(int m, int n, int k, T alpha, const T *a, int64_t rs_a, int64_t cs_a,
                 const T *b, int64_t rs_b, int64_t cs_b, T beta, T *c, int64_t rs_c,
                 int64_t cs_c, hipStream_t s);
+// Same with a triangular A whose zero triangle's K slices are skipped
+// (tri_a 1: A(i,p) = 0 for p > i, 2: for p < i).
+template <typename T>
+int gfx950_gemm_tri_a(int m, int n, int k, T alpha, const T *a, int64_t rs_a, int64_t cs_a,
+                      const T *b, int64_t rs_b, int64_t cs_b, T beta, T *c, int64_t rs_c,
+                      int64_t cs_c, int tri_a, hipStream_t s);
 
 // Precision helpers (refined STRSM): FP32 <-> FP64 copies, FP32 triangular
 // view materialised in FP64, and x64 += d; out := (float) x64.

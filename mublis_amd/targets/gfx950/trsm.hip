@@ -8,8 +8,10 @@
 //
 // * wide B: recursive -- split A in halves, solve the first, update the
 //   second half of B with ONE large MFMA GEMM (K = m/2), recurse.  All but
-//   O(m * NB * n) of the flops run in big GEMMs; a NB-row leaf multiplies
-//   its rows of B by the inverted diagonal block (FP64 MFMA).
+//   O(m * 512 * n) of the flops run in big GEMMs.  The 512-row diagonal
+//   blocks of A are inverted up front (one launch of independent pipelined
+//   solves against the identity), so each leaf is one more GEMM,
+//   X_i = alpha * inv(A_ii) * B_i, with no chain of dependent block rows.
 // * narrow B: one pipelined launch over block rows (below).
 //
 // Both start from the same prep launch, which inverts every NB x NB
@@ -121,9 +123,13 @@ __global__ __launch_bounds__(64) void trsm_inv_prep(int m, const T *__restrict__
   for (int r = 0; r < PB; ++r) o[r * PB + c] = s[r];
 }
 
+// Spin until *f == epoch (or the abort flag is up).  The loads are relaxed:
+// an acquiring load invalidates the L1 and the XCD's L2 on every iteration,
+// which would evict everyone's operands while hundreds of workgroups wait.
+// Callers order their data loads after the wait with one acquire fence.
 __device__ inline void wait_flag(int *f, int epoch, int *abort_flag) {
   int spins = 0;
-  while (__hip_atomic_load(f, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) != epoch) {
+  while (__hip_atomic_load(f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != epoch) {
     if (__hip_atomic_load(abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
     __builtin_amdgcn_s_sleep(2);
     if (++spins > (1 << 20)) {
@@ -178,19 +184,40 @@ __device__ inline void load_ablk(double (&av)[16], const T *a, int64_t rs_a, int
   }
 }
 
+// Independent solves in one launch (blockIdx.y = solve z): solve z covers
+// rows z * m_blk .. of the m-row problem, its A, B, inverted blocks and
+// flags start `a`, `b`, `inv`, `flags` elements after solve z - 1's.  A
+// single solve is m_blk = m with zero strides.
+struct PipeBatch {
+  int m_blk;
+  int64_t a, b, inv;
+  int flags;
+};
+
 template <typename T, bool LOWER, int CW>
 __global__ __launch_bounds__(256) void trsm_pipe(int m, int n, T alpha,
                                                  const T *__restrict__ a, int64_t rs_a,
                                                  int64_t cs_a, T *b, int64_t rs_b,
                                                  int64_t cs_b, const double *__restrict__ inv,
-                                                 int *flags, int cg, int epoch) {
+                                                 int *flags, int cg, int epoch, PipeBatch bt) {
   typedef double d4 __attribute__((ext_vector_type(4)));
   constexpr int TN = CW / 16;          // 16-column MFMA tiles per wave
   constexpr int TV = PB * CW / 256;    // tile elements per thread
   __shared__ double xs[2][PB][CW + 1];
   __shared__ int s_front;
+  {
+    const int z = blockIdx.y;
+    m = min(bt.m_blk, m - z * bt.m_blk);
+    a += z * bt.a;
+    b += z * bt.b;
+    inv += z * bt.inv;
+    flags += z * bt.flags;
+  }
   const int nbk = (m + PB - 1) / PB;
   const int j = blockIdx.x / cg, cgi = blockIdx.x % cg;
+  // Block rows past a short last solve: nothing to do, and no workgroup
+  // waits for them (waits go to lower block rows only).
+  if (j >= nbk) return;
   const int p = LOWER ? j : nbk - 1 - j;  // physical block row
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   const int lr = lane & 15, lk = lane >> 4;
@@ -220,12 +247,13 @@ __global__ __launch_bounds__(256) void trsm_pipe(int m, int n, T alpha,
       wait_flag(&flags[i * cg + cgi], epoch, abort_flag);
       int f = i + 1;
       while (f < j &&
-             __hip_atomic_load(&flags[f * cg + cgi], __ATOMIC_ACQUIRE,
+             __hip_atomic_load(&flags[f * cg + cgi], __ATOMIC_RELAXED,
                                __HIP_MEMORY_SCOPE_AGENT) == epoch)
         ++f;
       s_front = f;
     }
     __syncthreads();
+    // acquire: the flags read above were published with release stores
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     const int f = s_front;
     load_tile<T, CW>(xr, b, rs_b, cs_b, m, n, LOWER ? i : nbk - 1 - i, c0);
@@ -285,7 +313,9 @@ __global__ __launch_bounds__(256) void trsm_pipe(int m, int n, T alpha,
     const int gr = p * PB + r, gc = c0 + cc;
     if (gr < m && gc < n) b[(int64_t)gr * rs_b + (int64_t)gc * cs_b] = (T)xs[1][r][cc];
   }
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+  // Every wave's stores are complete at the barrier; thread 0's release
+  // store then publishes them all (release is cumulative over the
+  // workgroup's synchronisation), one L2 write-back per block.
   __syncthreads();
   if (tid == 0)
     __hip_atomic_store(&flags[j * cg + cgi], epoch, __ATOMIC_RELEASE,
@@ -424,12 +454,13 @@ __global__ __launch_bounds__(256) void trsm_panel(int m, int n, T alpha,
 // up to m * n = 2^20; beyond, the recursive schedule's large GEMMs fill the
 // GPU better than one chain of block rows (measured crossover on MI355X,
 // FP64: n ~ 1024 at m = 1024, 256 at 4096, 128 at 8192).  The recursive
-// schedule in turn hands its diagonal sub-problems of at most kPipeLeaf
-// rows to the pipeline (narrower B) or to column-panel workgroups (wider B),
-// which replaces the narrowest GEMM launches.
+// schedule in turn ends in GEMM leaves (B from 256 columns, see
+// use_inv_leaves), else hands its diagonal sub-problems of at most kPipeLeaf
+// rows to the pipeline (narrower B) or to column-panel workgroups (wider B,
+// when the solve brings no room for the GEMM leaves' inverses).
 int g_pipe_max_n = 4096;
 constexpr int kPipeLeaf = 512;
-// From this many columns up, the recursion's leaves of at most kPanelRows
+// From this many columns up, the pipelined leaves of at most kPanelRows
 // rows are column panels (n / 16 independent workgroups) instead.
 constexpr int kPanelMinN = 2048;
 
@@ -475,23 +506,26 @@ int leaf(int nb, int n, T alpha, const double *inv, T *b, int64_t rs_b, int64_t 
 template <typename T, bool LOWER, int CW>
 int pipe_launch(int m, int n, T alpha, const T *a, int64_t rs_a, int64_t cs_a, T *b,
                 int64_t rs_b, int64_t cs_b, const double *inv, int *flags, int epoch,
-                hipStream_t s) {
-  const int nbk = (m + PB - 1) / PB, cg = (n + CW - 1) / CW;
-  hipLaunchKernelGGL((trsm_pipe<T, LOWER, CW>), dim3(nbk * cg), dim3(256), 0, s, m, n, alpha,
-                     a, rs_a, cs_a, b, rs_b, cs_b, inv, flags, cg, epoch);
+                const PipeBatch &bt, hipStream_t s) {
+  const int nbk = (bt.m_blk + PB - 1) / PB, cg = (n + CW - 1) / CW;
+  const int nz = (m + bt.m_blk - 1) / bt.m_blk;
+  hipLaunchKernelGGL((trsm_pipe<T, LOWER, CW>), dim3(nbk * cg, nz), dim3(256), 0, s, m, n,
+                     alpha, a, rs_a, cs_a, b, rs_b, cs_b, inv, flags, cg, epoch, bt);
   return hipGetLastError() == hipSuccess ? MUBLIS_OK : MUBLIS_EHIP;
 }
 
 // One pipelined solve of an m-row (sub-)problem whose inverted blocks start
 // at `inv`; flags must hold `epoch - 1` or less (prep clears them to 0).
+// With `bt` given, the independent solves it describes instead.
 template <typename T>
 int pipe(bool lower, int m, int n, T alpha, const T *a, int64_t rs_a, int64_t cs_a, T *b,
          int64_t rs_b, int64_t cs_b, const double *inv, int *flags, int epoch,
-         hipStream_t s) {
+         hipStream_t s, const PipeBatch *batch = nullptr) {
   const int cw = pipe_cw(n);
+  const PipeBatch bt = batch ? *batch : PipeBatch{m, 0, 0, 0, 0};
 #define MUBLIS_PIPE(L, W)                                                                   \
   return pipe_launch<T, L, W>(m, n, alpha, a, rs_a, cs_a, b, rs_b, cs_b, inv, flags, epoch, \
-                              s)
+                              bt, s)
   if (lower) {
     if (cw == 16) MUBLIS_PIPE(true, 16);
     if (cw == 32) MUBLIS_PIPE(true, 32);
@@ -503,15 +537,87 @@ int pipe(bool lower, int m, int n, T alpha, const T *a, int64_t rs_a, int64_t cs
 #undef MUBLIS_PIPE
 }
 
-// Recursive schedule.  `ad` holds the inverted diagonal blocks of this
-// sub-problem (block 0 is its top-left NB x NB block); `flags` is null when
-// the pipeline is not used for leaves.
+// Wide right-hand sides: every kInvLeaf-row diagonal block of A is
+// inverted whole (one launch of independent pipelined solves against the
+// identity), and the recursion, split at multiples of kInvLeaf, ends in
+// leaves X = alpha * inv(A_ii) * B_i -- a copy of B_i and one GEMM that
+// skips the inverse's zero triangle.  No leaf then waits on a chain of block
+// rows.  Explicitly inverted diagonal blocks are the usual GPU TRSM form;
+// their forward error grows with the blocks' conditioning, not with m.
+constexpr int kInvLeaf = 512;
+constexpr int kInvMinN = 256;
+
+bool use_inv_leaves(int m, int n) {
+  return m > kInvLeaf && n >= kInvMinN && g_pipe_max_n >= kInvLeaf;
+}
+
+int64_t inv_leaf_elems(int m) {
+  return (int64_t)((m + kInvLeaf - 1) / kInvLeaf) * kInvLeaf * kInvLeaf;
+}
+
+// nz = ceil(m / kInvLeaf) identity blocks, kInvLeaf x kInvLeaf column-major
+// each; rows past m stay zero.
 template <typename T>
-int trsm_rec(bool lower, const double *ad, int m, int n, T alpha, const T *a,
-             int64_t rs_a, int64_t cs_a, T *b, int64_t rs_b, int64_t cs_b, int *flags,
-             int &epoch, hipStream_t s) {
+__global__ void eye_blocks(int m, T *x, int64_t total) {
+  constexpr int64_t KK = (int64_t)kInvLeaf * kInvLeaf;
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t z = t / KK, e = t - z * KK;
+    const int i = (int)(e % kInvLeaf), j = (int)(e / kInvLeaf);
+    x[t] = (i == j && z * kInvLeaf + i < m) ? T(1) : T(0);
+  }
+}
+
+// iv := the inverses of A's kInvLeaf-row diagonal blocks (block z at
+// iv + z * kInvLeaf^2, ld kInvLeaf), from the NB-block inverses in `ad`.
+// Uses pipeline epoch 1 of `flags`.
+template <typename T>
+int inv_blocks(bool lower, int m, const T *a, int64_t rs_a, int64_t cs_a, const double *ad,
+               int *flags, T *iv, hipStream_t s) {
+  const int64_t total = inv_leaf_elems(m);
+  const int grid = (int)std::min<int64_t>((total + 255) / 256, 1 << 16);
+  hipLaunchKernelGGL(eye_blocks<T>, dim3(grid), dim3(256), 0, s, m, iv, total);
+  if (hipGetLastError() != hipSuccess) return MUBLIS_EHIP;
+  const int cg = (kInvLeaf + pipe_cw(kInvLeaf) - 1) / pipe_cw(kInvLeaf);
+  const PipeBatch bt{kInvLeaf, (int64_t)kInvLeaf * (rs_a + cs_a),
+                     (int64_t)kInvLeaf * kInvLeaf, (int64_t)(kInvLeaf / NB) * NB * NB,
+                     1 + (kInvLeaf / NB) * cg};
+  return pipe<T>(lower, m, kInvLeaf, T(1), a, rs_a, cs_a, iv, 1, kInvLeaf, ad, flags, 1, s,
+                 &bt);
+}
+
+// Flags for inv_blocks: one abort flag and (kInvLeaf / NB) x cg block
+// flags per diagonal block.
+int64_t inv_flag_count(int m) {
+  const int cg = (kInvLeaf + pipe_cw(kInvLeaf) - 1) / pipe_cw(kInvLeaf);
+  return (int64_t)((m + kInvLeaf - 1) / kInvLeaf) * (1 + (kInvLeaf / NB) * cg);
+}
+
+// How the recursion's diagonal sub-problems end: GEMM leaves when `w` (a
+// kInvLeaf x n copy buffer) is set, else pipelined or column-panel leaves
+// when `flags` is set, else NB-row inverse products.
+template <typename T>
+struct Leaves {
+  int *flags;
+  int epoch;
+  T *w;
+};
+
+// Recursive schedule.  `ad` holds the NB-block inverses of this sub-problem
+// (block 0 is its top-left NB x NB block), `iv` its kInvLeaf-block inverses
+// for GEMM leaves.
+template <typename T>
+int trsm_rec(bool lower, const double *ad, const T *iv, int m, int n, T alpha, const T *a,
+             int64_t rs_a, int64_t cs_a, T *b, int64_t rs_b, int64_t cs_b, Leaves<T> &lv,
+             hipStream_t s) {
+  if (lv.w && m <= kInvLeaf) {
+    int st = gfx950_copy<T>(m, n, b, rs_b, cs_b, lv.w, 1, m, s);
+    if (st) return st;
+    return gfx950_gemm_tri_a<T>(m, n, m, alpha, iv, 1, kInvLeaf, lv.w, 1, m, T(0), b, rs_b,
+                                cs_b, lower ? 1 : 2, s);
+  }
   if (m <= NB) return leaf<T>(m, n, alpha, ad, b, rs_b, cs_b, s);
-  if (flags && m <= kPanelRows && n >= kPanelMinN) {
+  if (!lv.w && lv.flags && m <= kPanelRows && n >= kPanelMinN) {
     if (lower)
       hipLaunchKernelGGL((trsm_panel<T, true>), dim3((n + PW - 1) / PW), dim3(256), 0, s, m, n,
                          alpha, a, rs_a, cs_a, b, rs_b, cs_b, ad);
@@ -520,11 +626,13 @@ int trsm_rec(bool lower, const double *ad, int m, int n, T alpha, const T *a,
                          n, alpha, a, rs_a, cs_a, b, rs_b, cs_b, ad);
     return hipGetLastError() == hipSuccess ? MUBLIS_OK : MUBLIS_EHIP;
   }
-  if (flags && m <= kPipeLeaf)
-    return pipe<T>(lower, m, n, alpha, a, rs_a, cs_a, b, rs_b, cs_b, ad, flags, ++epoch, s);
-  // First part: a multiple of NB close to half, so GEMM K and M stay tile
-  // friendly at every level.
-  int m1 = (m / 2 + NB - 1) / NB * NB;
+  if (!lv.w && lv.flags && m <= kPipeLeaf)
+    return pipe<T>(lower, m, n, alpha, a, rs_a, cs_a, b, rs_b, cs_b, ad, lv.flags, ++lv.epoch,
+                   s);
+  // First part: a multiple of NB (of kInvLeaf with GEMM leaves) close to
+  // half, so GEMM K and M stay tile friendly at every level.
+  const int unit = lv.w ? kInvLeaf : NB;
+  int m1 = (m / 2 + unit - 1) / unit * unit;
   if (m1 >= m) m1 = m - NB;
   const int m2 = m - m1;
   const T *a11 = a;
@@ -533,44 +641,56 @@ int trsm_rec(bool lower, const double *ad, int m, int n, T alpha, const T *a,
   const T *a22 = a + (int64_t)m1 * rs_a + (int64_t)m1 * cs_a;
   T *b1 = b, *b2 = b + (int64_t)m1 * rs_b;
   const double *ad1 = ad, *ad2 = ad + (int64_t)(m1 / NB) * NB * NB;
+  const T *iv1 = iv, *iv2 = iv ? iv + (int64_t)(m1 / kInvLeaf) * kInvLeaf * kInvLeaf : iv;
   int st;
   if (lower) {
-    if ((st = trsm_rec<T>(lower, ad1, m1, n, alpha, a11, rs_a, cs_a, b1, rs_b, cs_b, flags,
-                          epoch, s)))
+    if ((st = trsm_rec<T>(lower, ad1, iv1, m1, n, alpha, a11, rs_a, cs_a, b1, rs_b, cs_b, lv,
+                          s)))
       return st;
     // B2 := alpha*B2 - A21 * X1 (the reference's GEMM with alpha -1 and
     // beta = alpha_eff, frame/l3/trsm.c:165-197, 241-249)
     if ((st = gfx950_gemm<T>(m2, n, m1, T(-1), a21, rs_a, cs_a, b1, rs_b, cs_b, alpha,
                              b2, rs_b, cs_b, s)))
       return st;
-    return trsm_rec<T>(lower, ad2, m2, n, T(1), a22, rs_a, cs_a, b2, rs_b, cs_b, flags,
-                       epoch, s);
+    return trsm_rec<T>(lower, ad2, iv2, m2, n, T(1), a22, rs_a, cs_a, b2, rs_b, cs_b, lv, s);
   }
-  if ((st = trsm_rec<T>(lower, ad2, m2, n, alpha, a22, rs_a, cs_a, b2, rs_b, cs_b, flags,
-                        epoch, s)))
+  if ((st = trsm_rec<T>(lower, ad2, iv2, m2, n, alpha, a22, rs_a, cs_a, b2, rs_b, cs_b, lv,
+                        s)))
     return st;
   if ((st = gfx950_gemm<T>(m1, n, m2, T(-1), a12, rs_a, cs_a, b2, rs_b, cs_b, alpha,
                            b1, rs_b, cs_b, s)))
     return st;
-  return trsm_rec<T>(lower, ad1, m1, n, T(1), a11, rs_a, cs_a, b1, rs_b, cs_b, flags, epoch,
-                     s);
+  return trsm_rec<T>(lower, ad1, iv1, m1, n, T(1), a11, rs_a, cs_a, b1, rs_b, cs_b, lv, s);
 }
 
 // Invert the diagonal blocks into `ad` (ws_doubles(m) doubles) and run the
-// pipelined or the recursive schedule; asynchronous on s.
+// pipelined or the recursive schedule; asynchronous on s.  With `iv`
+// (inv_leaf_elems(m) elements) and `w` (kInvLeaf * n elements) given and
+// use_inv_leaves(m, n), the recursion ends in GEMM leaves.
 template <typename T>
 int solve_ws(bool lower, int diag_mode, int m, int n, T alpha, const T *a, int64_t rs_a,
-             int64_t cs_a, T *b, int64_t rs_b, int64_t cs_b, double *ad, hipStream_t s) {
+             int64_t cs_a, T *b, int64_t rs_b, int64_t cs_b, double *ad, T *iv, T *w,
+             hipStream_t s) {
   const int nbk = (m + NB - 1) / NB;
   const bool piped = m > NB && n <= g_pipe_max_n && flags_fit(m, n);
-  int *flags = piped ? reinterpret_cast<int *>(ad + (int64_t)nbk * NB * NB) : nullptr;
+  const bool whole = piped && use_pipe(m, n);
+  const bool gemm_leaves = !whole && iv && w && use_inv_leaves(m, n);
+  int *flags = (piped || gemm_leaves) ? reinterpret_cast<int *>(ad + (int64_t)nbk * NB * NB)
+                                      : nullptr;
   const int cg = (n + pipe_cw(n) - 1) / pipe_cw(n);
-  int st = prep<T>(lower, m, a, rs_a, cs_a, diag_mode, ad, flags, piped ? 1 + nbk * cg : 0, s);
+  int64_t nflags = piped ? 1 + (int64_t)nbk * cg : 0;
+  if (gemm_leaves) nflags = std::max(nflags, inv_flag_count(m));
+  int st = prep<T>(lower, m, a, rs_a, cs_a, diag_mode, ad, flags, (int)nflags, s);
   if (st) return st;
-  int epoch = 0;
-  if (piped && use_pipe(m, n))
-    return pipe<T>(lower, m, n, alpha, a, rs_a, cs_a, b, rs_b, cs_b, ad, flags, ++epoch, s);
-  return trsm_rec<T>(lower, ad, m, n, alpha, a, rs_a, cs_a, b, rs_b, cs_b, flags, epoch, s);
+  if (whole) return pipe<T>(lower, m, n, alpha, a, rs_a, cs_a, b, rs_b, cs_b, ad, flags, 1, s);
+  Leaves<T> lv{flags, 0, nullptr};
+  if (gemm_leaves) {
+    if ((st = inv_blocks<T>(lower, m, a, rs_a, cs_a, ad, flags, iv, s))) return st;
+    lv.epoch = 1;
+    lv.w = w;
+  }
+  return trsm_rec<T>(lower, ad, gemm_leaves ? iv : nullptr, m, n, alpha, a, rs_a, cs_a, b,
+                     rs_b, cs_b, lv, s);
 }
 
 // Same with workspace from the arena, released once the stream has drained.
@@ -579,9 +699,18 @@ int solve(bool lower, int diag_mode, int m, int n, T alpha, const T *a, int64_t 
           int64_t cs_a, T *b, int64_t rs_b, int64_t cs_b, hipStream_t s) {
   double *ad = (double *)mublis_rt_alloc(sizeof(double) * (size_t)ws_doubles(m));
   if (!ad) return MUBLIS_ENOMEM;
-  int st = solve_ws<T>(lower, diag_mode, m, n, alpha, a, rs_a, cs_a, b, rs_b, cs_b, ad, s);
+  T *iv = nullptr, *w = nullptr;
+  if (use_inv_leaves(m, n)) {
+    // without room for them, the pipelined / panel leaves instead
+    iv = (T *)mublis_rt_alloc(sizeof(T) * (size_t)inv_leaf_elems(m));
+    w = (T *)mublis_rt_alloc(sizeof(T) * (size_t)kInvLeaf * (size_t)n);
+  }
+  int st = solve_ws<T>(lower, diag_mode, m, n, alpha, a, rs_a, cs_a, b, rs_b, cs_b, ad, iv, w,
+                       s);
   if (hipStreamSynchronize(s) != hipSuccess && st == MUBLIS_OK) st = MUBLIS_EHIP;
   mublis_rt_free(ad);
+  mublis_rt_free(iv);
+  mublis_rt_free(w);
   return st;
 }
 
@@ -645,7 +774,7 @@ static int trsm_ws_dev(int lower, int unit, int m, int n, T alpha, const T *a, i
   if (alpha == T(0)) return gfx950_fill<T>(m, n, T(0), b, rs_b, cs_b, s);
   if (!ws || ws_bytes < mublis_gfx950_trsm_ws_bytes(m)) return MUBLIS_EINVAL;
   return solve_ws<T>(lower != 0, unit ? 1 : 0, m, n, alpha, a, rs_a, cs_a, b, rs_b, cs_b,
-                     (double *)ws, s);
+                     (double *)ws, (T *)nullptr, (T *)nullptr, s);
 }
 
 extern "C" int mublis_gfx950_dtrsm_ws_dev(int lower, int unit, int m, int n, double alpha,

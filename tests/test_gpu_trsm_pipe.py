@@ -10,9 +10,11 @@ Bars:
 * the pipelined and the recursive schedule agree to a relative forward
   difference of 50 s eps (both solve the same well-conditioned system);
 * ragged shapes: m not a multiple of the 64-row block, n not a multiple of
-  the column group (16, 32 or 64 columns), a single right-hand side; every
-  shape here is one the default dispatch sends to the pipeline (n <= 128 or
-  m n <= 2^20).
+  the column group (16, 32 or 64 columns), a single right-hand side; the
+  SHAPES are ones the default dispatch sends to the pipeline (n <= 128 or
+  m n <= 2^20), the wider ones go through the recursive schedule with
+  GEMM leaves (512-row diagonal blocks inverted by a batch of pipelined
+  solves against the identity).
 """
 import ctypes
 import itertools
@@ -138,13 +140,12 @@ def test_pipelined_right_side(lib, prod, dt):
 
 @pytest.mark.parametrize("dt", L.DTYPES)
 @pytest.mark.parametrize("m,n", [(2000, 1000), (1100, 2100)])
-def test_recursive_schedule_with_pipelined_leaves(lib, prod, dt, m, n):
-    """Wide B: the recursive schedule splits A and hands its diagonal
-    sub-problems of at most 512 rows to the pipeline (n < 2048: 2000 ->
-    1024 + 976 -> 512-row pieces) or to column-panel workgroups (n >= 2048:
-    1100 -> 576 + 524 -> pieces of 256 to 320 rows); compared with the
-    recursive schedule down to 64-row leaves (pipeline disabled) and checked
-    by the residual probe."""
+def test_recursive_schedule_with_gemm_leaves_wide(lib, prod, dt, m, n):
+    """Wide B (n >= 256): the 512-row diagonal blocks are inverted in one
+    launch of independent pipelined solves and the recursion, split at
+    multiples of 512 (2000 -> 1024 + 976 -> 512 + 464; 1100 -> 1024 + 76),
+    ends in GEMM leaves; compared with the recursive schedule down to 64-row
+    leaves (pipeline disabled) and checked by the residual probe."""
     rng = np.random.default_rng(21)
     failures = []
     for lower, trans in ((True, False), (False, True)):
@@ -176,3 +177,52 @@ def test_pipe_threshold_knob(lib):
     finally:
         lib.mublis_gfx950_trsm_pipe_max_n(old)
     assert lib.mublis_gfx950_trsm_pipe_max_n(-1) == old
+
+
+GEMM_LEAF_CASES = [
+    # (m, n, order, right): the solve's order is n on the right side
+    (1500, 300, "F", False),  # leaves 512, 512, 476 rows
+    (2600, 256, "F", False),  # 1536 + 1064 -> ... + a 40-row last leaf
+    (777, 513, "C", False),   # row-major B: 512 + 265
+    (300, 1500, "F", True),   # right side: B's transposed view, 1500-row solve
+    (257, 777, "C", True),
+]
+
+
+@pytest.mark.parametrize("dt", L.DTYPES)
+@pytest.mark.parametrize("m,n,order,right", GEMM_LEAF_CASES)
+def test_gemm_leaves_ragged(lib, prod, dt, m, n, order, right):
+    """GEMM leaves on ragged shapes: short last 512-row blocks (their
+    identity and pipeline rows past the end), row-major storage and the
+    right side; every uplo / trans / diag combination, against the
+    recursive schedule down to 64-row leaves and the residual probe."""
+    co = L.ROW if order == "C" else L.COL
+    s_ = n if right else m
+    rng = np.random.default_rng(m * 7 + n)
+    failures = []
+    for lower, trans, unit in itertools.product((True, False), repeat=3):
+        a = L.conditioned_triangular(rng, s_, dt, order, lower, unit)
+        b0 = L.rand(rng, (m, n), dt, order)
+        alpha = -0.5
+        lda = s_
+        ldb = n if order == "C" else m
+        outs = []
+        for max_n in (-1, 0):
+            old = lib.mublis_gfx950_trsm_pipe_max_n(max_n)
+            try:
+                x = np.array(b0, copy=True, order=order)
+                prod.call(dt, "trsm", co, L.RIGHT if right else L.LEFT,
+                          L.LO if lower else L.UP, L.TR if trans else L.NOTR,
+                          L.UNIT if unit else L.NONUNIT, m, n, alpha, a, lda, x, ldb)
+            finally:
+                lib.mublis_gfx950_trsm_pipe_max_n(old)
+            outs.append(x)
+        # logical matrices: the probe copies them to column-major itself
+        res, lim = residual(a, outs[0], b0, alpha, right, lower, trans, unit)
+        diff = np.linalg.norm(outs[0].astype(np.float64) - outs[1]) / np.linalg.norm(outs[1])
+        tag = f"l{int(lower)} t{int(trans)} u{int(unit)}"
+        if not np.isfinite(outs[0]).all() or not res <= lim:
+            failures.append(f"{tag}: residual {res:.3e} > {lim:.3e}")
+        if not diff <= 50 * s_ * EPS[np.dtype(dt)]:
+            failures.append(f"{tag}: vs 64-row leaves {diff:.3e}")
+    assert not failures, "\n".join(failures)
