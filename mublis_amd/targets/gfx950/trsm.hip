@@ -187,11 +187,13 @@ __device__ inline void load_ablk(double (&av)[16], const T *a, int64_t rs_a, int
 // Independent solves in one launch (blockIdx.y = solve z): solve z covers
 // rows z * m_blk .. of the m-row problem, its A, B, inverted blocks and
 // flags start `a`, `b`, `inv`, `flags` elements after solve z - 1's.  A
-// single solve is m_blk = m with zero strides.
+// single solve is m_blk = m with zero strides.  `eye`: every solve's
+// right-hand side is the identity.
 struct PipeBatch {
   int m_blk;
   int64_t a, b, inv;
   int flags;
+  int eye;
 };
 
 template <typename T, bool LOWER, int CW>
@@ -218,6 +220,15 @@ __global__ __launch_bounds__(256) void trsm_pipe(int m, int n, T alpha,
   // Block rows past a short last solve: nothing to do, and no workgroup
   // waits for them (waits go to lower block rows only).
   if (j >= nbk) return;
+  // Identity right-hand side with CW == PB: column group c of X(q) is zero
+  // for q < c (lower) or q > c (upper).  Such workgroups have nothing to
+  // store (B holds those zeros already) and no workgroup waits for them;
+  // the others start their updates at the first nonzero block.
+  int i0 = 0;
+  if (CW == PB && bt.eye) {
+    i0 = LOWER ? cgi : max(0, nbk - 1 - cgi);
+    if (j < i0) return;
+  }
   const int p = LOWER ? j : nbk - 1 - j;  // physical block row
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   const int lr = lane & 15, lk = lane >> 4;
@@ -239,7 +250,7 @@ __global__ __launch_bounds__(256) void trsm_pipe(int m, int n, T alpha,
 #pragma unroll
   for (int t = 0; t < TN; ++t) acc[t] = d4{0, 0, 0, 0};
   double av[16], xr[TV];
-  int i = 0;
+  int i = i0;
   while (i < j) {
     // A(p, q(i)) before the wait; then the run of published blocks i .. f-1
     load_ablk<T>(av, a, rs_a, cs_a, m, arow, LOWER ? i : nbk - 1 - i, lk);
@@ -522,7 +533,7 @@ int pipe(bool lower, int m, int n, T alpha, const T *a, int64_t rs_a, int64_t cs
          int64_t rs_b, int64_t cs_b, const double *inv, int *flags, int epoch,
          hipStream_t s, const PipeBatch *batch = nullptr) {
   const int cw = pipe_cw(n);
-  const PipeBatch bt = batch ? *batch : PipeBatch{m, 0, 0, 0, 0};
+  const PipeBatch bt = batch ? *batch : PipeBatch{m, 0, 0, 0, 0, 0};
 #define MUBLIS_PIPE(L, W)                                                                   \
   return pipe_launch<T, L, W>(m, n, alpha, a, rs_a, cs_a, b, rs_b, cs_b, inv, flags, epoch, \
                               bt, s)
@@ -544,27 +555,32 @@ int pipe(bool lower, int m, int n, T alpha, const T *a, int64_t rs_a, int64_t cs
 // skips the inverse's zero triangle.  No leaf then waits on a chain of block
 // rows.  Explicitly inverted diagonal blocks are the usual GPU TRSM form;
 // their forward error grows with the blocks' conditioning, not with m.
-constexpr int kInvLeaf = 512;
+// Block rows: 1024 for double data, 512 for float, whose GEMMs are fast
+// enough that the longer inversion chain of 1024-row blocks does not pay.
+template <typename T>
+constexpr int kInvLeaf = sizeof(T) == 8 ? 1024 : 512;
 constexpr int kInvMinN = 256;
 
+template <typename T>
 bool use_inv_leaves(int m, int n) {
-  return m > kInvLeaf && n >= kInvMinN && g_pipe_max_n >= kInvLeaf;
+  return m > kInvLeaf<T> && n >= kInvMinN && g_pipe_max_n >= kInvLeaf<T>;
 }
 
+template <typename T>
 int64_t inv_leaf_elems(int m) {
-  return (int64_t)((m + kInvLeaf - 1) / kInvLeaf) * kInvLeaf * kInvLeaf;
+  return (int64_t)((m + kInvLeaf<T> - 1) / kInvLeaf<T>) * kInvLeaf<T> * kInvLeaf<T>;
 }
 
 // nz = ceil(m / kInvLeaf) identity blocks, kInvLeaf x kInvLeaf column-major
 // each; rows past m stay zero.
 template <typename T>
 __global__ void eye_blocks(int m, T *x, int64_t total) {
-  constexpr int64_t KK = (int64_t)kInvLeaf * kInvLeaf;
+  constexpr int64_t KK = (int64_t)kInvLeaf<T> * kInvLeaf<T>;
   for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
        t += (int64_t)gridDim.x * blockDim.x) {
     const int64_t z = t / KK, e = t - z * KK;
-    const int i = (int)(e % kInvLeaf), j = (int)(e / kInvLeaf);
-    x[t] = (i == j && z * kInvLeaf + i < m) ? T(1) : T(0);
+    const int i = (int)(e % kInvLeaf<T>), j = (int)(e / kInvLeaf<T>);
+    x[t] = (i == j && z * kInvLeaf<T> + i < m) ? T(1) : T(0);
   }
 }
 
@@ -574,23 +590,26 @@ __global__ void eye_blocks(int m, T *x, int64_t total) {
 template <typename T>
 int inv_blocks(bool lower, int m, const T *a, int64_t rs_a, int64_t cs_a, const double *ad,
                int *flags, T *iv, hipStream_t s) {
-  const int64_t total = inv_leaf_elems(m);
+  const int64_t total = inv_leaf_elems<T>(m);
   const int grid = (int)std::min<int64_t>((total + 255) / 256, 1 << 16);
   hipLaunchKernelGGL(eye_blocks<T>, dim3(grid), dim3(256), 0, s, m, iv, total);
   if (hipGetLastError() != hipSuccess) return MUBLIS_EHIP;
-  const int cg = (kInvLeaf + pipe_cw(kInvLeaf) - 1) / pipe_cw(kInvLeaf);
-  const PipeBatch bt{kInvLeaf, (int64_t)kInvLeaf * (rs_a + cs_a),
-                     (int64_t)kInvLeaf * kInvLeaf, (int64_t)(kInvLeaf / NB) * NB * NB,
-                     1 + (kInvLeaf / NB) * cg};
-  return pipe<T>(lower, m, kInvLeaf, T(1), a, rs_a, cs_a, iv, 1, kInvLeaf, ad, flags, 1, s,
-                 &bt);
+  const int cg = (kInvLeaf<T> + pipe_cw(kInvLeaf<T>) - 1) / pipe_cw(kInvLeaf<T>);
+  static_assert(kInvLeaf<T> >= 128, "identity solves need 64-column groups (pipe_cw)");
+  const PipeBatch bt{kInvLeaf<T>, (int64_t)kInvLeaf<T> * (rs_a + cs_a),
+                     (int64_t)kInvLeaf<T> * kInvLeaf<T>,
+                     (int64_t)(kInvLeaf<T> / NB) * NB * NB,
+                     1 + (kInvLeaf<T> / NB) * cg, 1};
+  return pipe<T>(lower, m, kInvLeaf<T>, T(1), a, rs_a, cs_a, iv, 1, kInvLeaf<T>, ad, flags, 1,
+                 s, &bt);
 }
 
 // Flags for inv_blocks: one abort flag and (kInvLeaf / NB) x cg block
 // flags per diagonal block.
+template <typename T>
 int64_t inv_flag_count(int m) {
-  const int cg = (kInvLeaf + pipe_cw(kInvLeaf) - 1) / pipe_cw(kInvLeaf);
-  return (int64_t)((m + kInvLeaf - 1) / kInvLeaf) * (1 + (kInvLeaf / NB) * cg);
+  const int cg = (kInvLeaf<T> + pipe_cw(kInvLeaf<T>) - 1) / pipe_cw(kInvLeaf<T>);
+  return (int64_t)((m + kInvLeaf<T> - 1) / kInvLeaf<T>) * (1 + (kInvLeaf<T> / NB) * cg);
 }
 
 // How the recursion's diagonal sub-problems end: GEMM leaves when `w` (a
@@ -610,10 +629,10 @@ template <typename T>
 int trsm_rec(bool lower, const double *ad, const T *iv, int m, int n, T alpha, const T *a,
              int64_t rs_a, int64_t cs_a, T *b, int64_t rs_b, int64_t cs_b, Leaves<T> &lv,
              hipStream_t s) {
-  if (lv.w && m <= kInvLeaf) {
+  if (lv.w && m <= kInvLeaf<T>) {
     int st = gfx950_copy<T>(m, n, b, rs_b, cs_b, lv.w, 1, m, s);
     if (st) return st;
-    return gfx950_gemm_tri_a<T>(m, n, m, alpha, iv, 1, kInvLeaf, lv.w, 1, m, T(0), b, rs_b,
+    return gfx950_gemm_tri_a<T>(m, n, m, alpha, iv, 1, kInvLeaf<T>, lv.w, 1, m, T(0), b, rs_b,
                                 cs_b, lower ? 1 : 2, s);
   }
   if (m <= NB) return leaf<T>(m, n, alpha, ad, b, rs_b, cs_b, s);
@@ -631,7 +650,7 @@ int trsm_rec(bool lower, const double *ad, const T *iv, int m, int n, T alpha, c
                    s);
   // First part: a multiple of NB (of kInvLeaf with GEMM leaves) close to
   // half, so GEMM K and M stay tile friendly at every level.
-  const int unit = lv.w ? kInvLeaf : NB;
+  const int unit = lv.w ? kInvLeaf<T> : NB;
   int m1 = (m / 2 + unit - 1) / unit * unit;
   if (m1 >= m) m1 = m - NB;
   const int m2 = m - m1;
@@ -641,7 +660,8 @@ int trsm_rec(bool lower, const double *ad, const T *iv, int m, int n, T alpha, c
   const T *a22 = a + (int64_t)m1 * rs_a + (int64_t)m1 * cs_a;
   T *b1 = b, *b2 = b + (int64_t)m1 * rs_b;
   const double *ad1 = ad, *ad2 = ad + (int64_t)(m1 / NB) * NB * NB;
-  const T *iv1 = iv, *iv2 = iv ? iv + (int64_t)(m1 / kInvLeaf) * kInvLeaf * kInvLeaf : iv;
+  constexpr int64_t KL = kInvLeaf<T>;
+  const T *iv1 = iv, *iv2 = iv ? iv + (int64_t)(m1 / KL) * KL * KL : iv;
   int st;
   if (lower) {
     if ((st = trsm_rec<T>(lower, ad1, iv1, m1, n, alpha, a11, rs_a, cs_a, b1, rs_b, cs_b, lv,
@@ -665,8 +685,8 @@ int trsm_rec(bool lower, const double *ad, const T *iv, int m, int n, T alpha, c
 
 // Invert the diagonal blocks into `ad` (ws_doubles(m) doubles) and run the
 // pipelined or the recursive schedule; asynchronous on s.  With `iv`
-// (inv_leaf_elems(m) elements) and `w` (kInvLeaf * n elements) given and
-// use_inv_leaves(m, n), the recursion ends in GEMM leaves.
+// (inv_leaf_elems<T>(m) elements) and `w` (kInvLeaf<T> * n elements) given and
+// use_inv_leaves<T>(m, n), the recursion ends in GEMM leaves.
 template <typename T>
 int solve_ws(bool lower, int diag_mode, int m, int n, T alpha, const T *a, int64_t rs_a,
              int64_t cs_a, T *b, int64_t rs_b, int64_t cs_b, double *ad, T *iv, T *w,
@@ -674,12 +694,12 @@ int solve_ws(bool lower, int diag_mode, int m, int n, T alpha, const T *a, int64
   const int nbk = (m + NB - 1) / NB;
   const bool piped = m > NB && n <= g_pipe_max_n && flags_fit(m, n);
   const bool whole = piped && use_pipe(m, n);
-  const bool gemm_leaves = !whole && iv && w && use_inv_leaves(m, n);
+  const bool gemm_leaves = !whole && iv && w && use_inv_leaves<T>(m, n);
   int *flags = (piped || gemm_leaves) ? reinterpret_cast<int *>(ad + (int64_t)nbk * NB * NB)
                                       : nullptr;
   const int cg = (n + pipe_cw(n) - 1) / pipe_cw(n);
   int64_t nflags = piped ? 1 + (int64_t)nbk * cg : 0;
-  if (gemm_leaves) nflags = std::max(nflags, inv_flag_count(m));
+  if (gemm_leaves) nflags = std::max(nflags, inv_flag_count<T>(m));
   int st = prep<T>(lower, m, a, rs_a, cs_a, diag_mode, ad, flags, (int)nflags, s);
   if (st) return st;
   if (whole) return pipe<T>(lower, m, n, alpha, a, rs_a, cs_a, b, rs_b, cs_b, ad, flags, 1, s);
@@ -700,10 +720,10 @@ int solve(bool lower, int diag_mode, int m, int n, T alpha, const T *a, int64_t 
   double *ad = (double *)mublis_rt_alloc(sizeof(double) * (size_t)ws_doubles(m));
   if (!ad) return MUBLIS_ENOMEM;
   T *iv = nullptr, *w = nullptr;
-  if (use_inv_leaves(m, n)) {
+  if (use_inv_leaves<T>(m, n)) {
     // without room for them, the pipelined / panel leaves instead
-    iv = (T *)mublis_rt_alloc(sizeof(T) * (size_t)inv_leaf_elems(m));
-    w = (T *)mublis_rt_alloc(sizeof(T) * (size_t)kInvLeaf * (size_t)n);
+    iv = (T *)mublis_rt_alloc(sizeof(T) * (size_t)inv_leaf_elems<T>(m));
+    w = (T *)mublis_rt_alloc(sizeof(T) * (size_t)kInvLeaf<T> * (size_t)n);
   }
   int st = solve_ws<T>(lower, diag_mode, m, n, alpha, a, rs_a, cs_a, b, rs_b, cs_b, ad, iv, w,
                        s);

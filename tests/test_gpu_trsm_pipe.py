@@ -141,11 +141,12 @@ def test_pipelined_right_side(lib, prod, dt):
 @pytest.mark.parametrize("dt", L.DTYPES)
 @pytest.mark.parametrize("m,n", [(2000, 1000), (1100, 2100)])
 def test_recursive_schedule_with_gemm_leaves_wide(lib, prod, dt, m, n):
-    """Wide B (n >= 256): the 512-row diagonal blocks are inverted in one
-    launch of independent pipelined solves and the recursion, split at
-    multiples of 512 (2000 -> 1024 + 976 -> 512 + 464; 1100 -> 1024 + 76),
-    ends in GEMM leaves; compared with the recursive schedule down to 64-row
-    leaves (pipeline disabled) and checked by the residual probe."""
+    """Wide B (n >= 256): the diagonal blocks of 512 (float) or 1024
+    (double) rows are inverted in one launch of independent pipelined
+    solves and the recursion, split at multiples of that size (float:
+    2000 -> 1024 + 976 -> 512 + 464; double: 2000 -> 1024 + 976), ends in
+    GEMM leaves; compared with the recursive schedule down to 64-row leaves
+    (pipeline disabled) and checked by the residual probe."""
     rng = np.random.default_rng(21)
     failures = []
     for lower, trans in ((True, False), (False, True)):
@@ -180,12 +181,14 @@ def test_pipe_threshold_knob(lib):
 
 
 GEMM_LEAF_CASES = [
-    # (m, n, order, right): the solve's order is n on the right side
-    (1500, 300, "F", False),  # leaves 512, 512, 476 rows
-    (2600, 256, "F", False),  # 1536 + 1064 -> ... + a 40-row last leaf
-    (777, 513, "C", False),   # row-major B: 512 + 265
+    # (m, n, order, right): the solve's order is n on the right side;
+    # leaves for float (512-row blocks) / double (1024-row blocks)
+    (1500, 300, "F", False),  # 512, 512, 476 / 1024, 476
+    (2600, 256, "F", False),  # ..., a 40-row last leaf / 2048 + 552
+    (777, 513, "C", False),   # row-major B: 512 + 265 / pipelined leaves
     (300, 1500, "F", True),   # right side: B's transposed view, 1500-row solve
     (257, 777, "C", True),
+    (128, 2100, "F", True),   # double: 1024 + 1024 + 52
 ]
 
 
