@@ -220,13 +220,14 @@ __global__ __launch_bounds__(256) void trsm_pipe(int m, int n, T alpha,
   // Block rows past a short last solve: nothing to do, and no workgroup
   // waits for them (waits go to lower block rows only).
   if (j >= nbk) return;
-  // Identity right-hand side with CW == PB: column group c of X(q) is zero
-  // for q < c (lower) or q > c (upper).  Such workgroups have nothing to
-  // store (B holds those zeros already) and no workgroup waits for them;
+  // Identity right-hand side: block q of X is zero in the columns c0 ..
+  // c0 + CW - 1 when all its rows lie above them (lower: 64 q + 63 < c0) or
+  // below them (upper: 64 q > c0 + CW - 1).  Such workgroups have nothing
+  // to store (B holds those zeros already) and no workgroup waits for them;
   // the others start their updates at the first nonzero block.
   int i0 = 0;
-  if (CW == PB && bt.eye) {
-    i0 = LOWER ? cgi : max(0, nbk - 1 - cgi);
+  if (bt.eye) {
+    i0 = LOWER ? (cgi * CW) / PB : max(0, nbk - 1 - (cgi * CW + CW - 1) / PB);
     if (j < i0) return;
   }
   const int p = LOWER ? j : nbk - 1 - j;  // physical block row
@@ -531,8 +532,8 @@ int pipe_launch(int m, int n, T alpha, const T *a, int64_t rs_a, int64_t cs_a, T
 template <typename T>
 int pipe(bool lower, int m, int n, T alpha, const T *a, int64_t rs_a, int64_t cs_a, T *b,
          int64_t rs_b, int64_t cs_b, const double *inv, int *flags, int epoch,
-         hipStream_t s, const PipeBatch *batch = nullptr) {
-  const int cw = pipe_cw(n);
+         hipStream_t s, const PipeBatch *batch = nullptr, int cw = 0) {
+  if (!cw) cw = pipe_cw(n);
   const PipeBatch bt = batch ? *batch : PipeBatch{m, 0, 0, 0, 0, 0};
 #define MUBLIS_PIPE(L, W)                                                                   \
   return pipe_launch<T, L, W>(m, n, alpha, a, rs_a, cs_a, b, rs_b, cs_b, inv, flags, epoch, \
@@ -560,6 +561,9 @@ int pipe(bool lower, int m, int n, T alpha, const T *a, int64_t rs_a, int64_t cs
 template <typename T>
 constexpr int kInvLeaf = sizeof(T) == 8 ? 1024 : 512;
 constexpr int kInvMinN = 256;
+// Column group of the identity solves: narrow groups keep the pipeline's
+// LDS small, so all inversions run at once instead of in waves.
+constexpr int kEyeCw = 32;
 
 template <typename T>
 bool use_inv_leaves(int m, int n) {
@@ -594,21 +598,20 @@ int inv_blocks(bool lower, int m, const T *a, int64_t rs_a, int64_t cs_a, const 
   const int grid = (int)std::min<int64_t>((total + 255) / 256, 1 << 16);
   hipLaunchKernelGGL(eye_blocks<T>, dim3(grid), dim3(256), 0, s, m, iv, total);
   if (hipGetLastError() != hipSuccess) return MUBLIS_EHIP;
-  const int cg = (kInvLeaf<T> + pipe_cw(kInvLeaf<T>) - 1) / pipe_cw(kInvLeaf<T>);
-  static_assert(kInvLeaf<T> >= 128, "identity solves need 64-column groups (pipe_cw)");
+  const int cg = kInvLeaf<T> / kEyeCw;
   const PipeBatch bt{kInvLeaf<T>, (int64_t)kInvLeaf<T> * (rs_a + cs_a),
                      (int64_t)kInvLeaf<T> * kInvLeaf<T>,
                      (int64_t)(kInvLeaf<T> / NB) * NB * NB,
                      1 + (kInvLeaf<T> / NB) * cg, 1};
   return pipe<T>(lower, m, kInvLeaf<T>, T(1), a, rs_a, cs_a, iv, 1, kInvLeaf<T>, ad, flags, 1,
-                 s, &bt);
+                 s, &bt, kEyeCw);
 }
 
 // Flags for inv_blocks: one abort flag and (kInvLeaf / NB) x cg block
 // flags per diagonal block.
 template <typename T>
 int64_t inv_flag_count(int m) {
-  const int cg = (kInvLeaf<T> + pipe_cw(kInvLeaf<T>) - 1) / pipe_cw(kInvLeaf<T>);
+  const int cg = kInvLeaf<T> / kEyeCw;
   return (int64_t)((m + kInvLeaf<T> - 1) / kInvLeaf<T>) * (1 + (kInvLeaf<T> / NB) * cg);
 }
 
