@@ -40,6 +40,7 @@ __global__ __launch_bounds__(NT, Tile<TILE>::MIN_WG_PER_CU) void gemm_kernel(Gem
     if (!map_tile_lpt(blockIdx.x, p.tiles_m, p.tiles_n, p.lpt, p.lpt_g, tm, tn)) return;
   } else {
     map_tile(blockIdx.x, p.ntiles, p.tiles_m, p.tiles_n, p.tri, p.ji, tm, tn);
+    if (tm >= p.tiles_m || tn >= p.tiles_n) return;  // mode 3 past an odd last tile
   }
   gemm_tile<T, TILE, A_KC, B_NC>(p, tm, tn, lds);
 }
@@ -70,7 +71,12 @@ template <typename T, int TILE>
 hipError_t launch(GemmArgs<T> p, bool akc, bool bnc, hipStream_t s) {
   p.tiles_m = (p.m + TILE - 1) / TILE;
   p.tiles_n = (p.n + TILE - 1) / TILE;
-  p.ntiles = p.tri ? p.tiles_m * (p.tiles_m + 1) / 2 : p.tiles_m * p.tiles_n;
+  const int t = p.tiles_m;
+  p.ntiles = p.tri == 1 ? t * (t + 1) / 2
+           : p.tri == 2 ? t * (t - 1) / 2
+           : p.tri == 3 ? 3 * ((t + 1) / 2)
+                        : p.tiles_m * p.tiles_n;
+  if (p.ntiles == 0) return hipSuccess;
   int wgs = p.ntiles;
   if (p.lpt) {
     const int lines = p.lpt <= 2 ? p.tiles_m : p.tiles_n;
@@ -169,8 +175,19 @@ int gemm_dev(int m, int n, int k, T alpha, const T *a, int64_t rs_a, int64_t cs_
     const int64_t t128 = (int64_t)((m + 127) / 128) * ((n + 127) / 128);
     const int64_t n128 = p.tri ? ((m + 127) / 128) * ((m + 127) / 128 + 1) / 2 : t128;
     const int64_t need = p.lpt ? 2048 : 512;
-    hipError_t e = n128 >= need ? launch<T, 128>(p, akc, bnc, s)
-                                : launch<T, 64>(p, akc, bnc, s);
+    hipError_t e;
+    if (p.tri && n128 >= need) {
+      // The triangle's diagonal 128-tiles would compute twice what they
+      // store and push the tile count past a round of 512: 128-tiles
+      // strictly inside the triangle, then the diagonal blocks as three
+      // 64-tiles each.
+      p.tri = 2;
+      e = launch<T, 128>(p, akc, bnc, s);
+      p.tri = 3;
+      if (e == hipSuccess) e = launch<T, 64>(p, akc, bnc, s);
+    } else {
+      e = n128 >= need ? launch<T, 128>(p, akc, bnc, s) : launch<T, 64>(p, akc, bnc, s);
+    }
     if (e != hipSuccess) st = MUBLIS_EHIP;
     if (st == MUBLIS_OK && a2 && !same2) {
       // Layouts differ between the two products: add the second one in a

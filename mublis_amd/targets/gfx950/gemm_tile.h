@@ -64,7 +64,11 @@ struct GemmArgs {
   T *c; int64_t rs_c, cs_c;
   int ji;          // 0 all, 1 lower (j <= i), 2 upper (j >= i)
   int tiles_m, tiles_n;
-  int tri;         // grid enumerates only tiles touching the ji triangle
+  // Grid over the ji triangle (SYRK/SYR2K): 0 every tile; 1 the tiles
+  // touching the triangle; 2 only the tiles strictly inside it; 3 the tiles
+  // of the diagonal blocks of twice the tile size that touch the triangle
+  // (three per block), which complete mode 2.
+  int tri;
   int ntiles;
   int vec_a, vec_b, vec_a2, vec_b2;  // 16-byte loads are aligned
   // Triangular operand structure, used only to skip K slices that are zero:
@@ -117,13 +121,21 @@ __device__ inline void map_tile(int w, int ntiles, int tiles_m, int tiles_n,
   const int q = ntiles / 8, r = ntiles % 8;
   const int xcd = w % 8, local = w / 8;
   const int id = (xcd < r ? xcd * (q + 1) : r * (q + 1) + (xcd - r) * q) + local;
+  if (tri == 3) {
+    // diagonal block d of twice the tile: (2d, 2d), the off-diagonal tile
+    // on the ji side, (2d + 1, 2d + 1)
+    const int d = id / 3, r = id - 3 * d;
+    const int a = 2 * d + (r >= 1), b = 2 * d + (r == 2);
+    if (ji == 2) { tm = b; tn = a; } else { tm = a; tn = b; }
+    return;
+  }
   if (tri) {
-    // Row-major walk over the lower triangle of tiles (tn <= tm); for an
-    // upper ji the same walk is transposed.
+    // Row-major walk over the lower triangle of tiles (tn <= tm, or tn < tm
+    // for mode 2); for an upper ji the same walk is transposed.
     int t = (int)((sqrtf(8.0f * (float)id + 1.0f) - 1.0f) * 0.5f);
     while ((t + 1) * (t + 2) / 2 <= id) ++t;
     while (t * (t + 1) / 2 > id) --t;
-    int a = t, b = id - t * (t + 1) / 2;
+    int a = t + (tri == 2), b = id - t * (t + 1) / 2;
     if (ji == 2) { tm = b; tn = a; } else { tm = a; tn = b; }
     return;
   }
