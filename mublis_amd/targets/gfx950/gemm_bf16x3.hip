@@ -14,10 +14,11 @@
 // ([row][ldp], rows padded to the tile, K to 64, zero filled; ldp = K + 64
 // so that rows never share L2 sets at one k offset), so the GEMM kernel
 // streams aligned 16-byte vectors with no bounds checks.  The GEMM kernel:
-// 256 x 128 output tile per workgroup, one workgroup per CU, 4 waves of
-// 128 x 64, each a 4 x 2 grid of v_mfma_f32_32x32x16_bf16 tiles (6 MFMAs
-// per tile per k = 16), operands streamed by direct-to-LDS loads into a
-// 4-stage LDS ring, the XCD-aware tile order of the FP64/FP32 kernel.
+// 256 x 128 output tile per workgroup, one workgroup per CU, 8 waves of
+// 64 x 64 (two per SIMD), each a 2 x 2 grid of v_mfma_f32_32x32x16_bf16
+// tiles (6 MFMAs per tile per k = 16), operands streamed by direct-to-LDS
+// loads into a 4-stage LDS ring, the XCD-aware tile order of the FP64/FP32
+// kernel.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -36,7 +37,7 @@ typedef u16 u16x8 __attribute__((ext_vector_type(8)));
 constexpr int XTM = 256;   // output tile rows
 constexpr int XTN = 128;   // output tile columns
 constexpr int XBK = 16;    // k per stage (one MFMA k step)
-constexpr int XNT = 256;
+constexpr int kWaves = 8;  // waves per workgroup (4 or 8)
 
 __device__ inline u16 bf16_rne(float x) {
   const uint32_t u = __float_as_uint(x);
@@ -149,7 +150,15 @@ __device__ inline void dma16(const u16 *g, uint32_t lds) {
 }
 #pragma clang diagnostic pop
 
-__global__ __launch_bounds__(XNT, 1) void gemm_bf16x3_kernel(XArgs p) {
+// NW = 4: waves of 128 x 64 (4 x 2 MFMA tiles, two accumulators: 256
+// AGPRs, one wave per SIMD).  NW = 8: waves of 64 x 64 (2 x 2 tiles, 128
+// AGPRs), two waves per SIMD, so one wave's MFMAs cover the other's LDS
+// reads and barrier; waves 0-3 stream the operands either way.
+template <int NW>
+__global__ __launch_bounds__(64 * NW, 1) void gemm_bf16x3_kernel(XArgs p) {
+  constexpr int MW = NW / 2;        // waves along M
+  constexpr int WM = XTM / MW;      // rows per wave
+  constexpr int TI = WM / 32;       // 32-row MFMA tiles per wave
   __shared__ __attribute__((aligned(1024))) u16 R0[XSTAGE_U16];
   __shared__ __attribute__((aligned(1024))) u16 R1[XSTAGE_U16];
   __shared__ __attribute__((aligned(1024))) u16 R2[XSTAGE_U16];
@@ -159,7 +168,8 @@ __global__ __launch_bounds__(XNT, 1) void gemm_bf16x3_kernel(XArgs p) {
   const int m0 = tm * XTM, n0 = tn * XTN;
   const int tid = threadIdx.x, lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-  const int wm = (wave & 1) * 128, wn = (wave >> 1) * 64;
+  const int wm = (wave % MW) * WM, wn = (wave / MW) * 64;
+  const bool streams = wave < 4;
 
   // This lane's source element offset (u16) for each of the wave's 9 blocks
   // of a stage, at k = 0; block b = wave + 4 j.
@@ -167,23 +177,24 @@ __global__ __launch_bounds__(XNT, 1) void gemm_bf16x3_kernel(XArgs p) {
   const u16 *src[9];
 #pragma unroll
   for (int j = 0; j < 9; ++j) {
-    const int b = wave + 4 * j;
+    const int sw = wave & 3, b = sw + 4 * j;
     if (j < 6) {  // A plane b / 8, rows 32 (b % 8) ..
       src[j] = p.a[b >> 3] + (int64_t)(m0 + 32 * (b & 7) + lrow) * p.ldp + 8 * lhalf;
     } else {      // B plane j - 6, rows 32 wave ..
-      src[j] = p.b[j - 6] + (int64_t)(n0 + 32 * wave + lrow) * p.ldp + 8 * lhalf;
+      src[j] = p.b[j - 6] + (int64_t)(n0 + 32 * sw + lrow) * p.ldp + 8 * lhalf;
     }
   }
   auto issue = [&](u16 *slot, int stage) {
+    if (!streams) return;
     const int64_t k0 = (int64_t)stage * XBK;
     const uint32_t base = (uint32_t)(uintptr_t)(__attribute__((address_space(3))) u16 *)slot;
 #pragma unroll
     for (int j = 0; j < 9; ++j) dma16(src[j] + k0, base + (wave + 4 * j) * 1024);
   };
 
-  f16v big[4][2], small[4][2];
+  f16v big[TI][2], small[TI][2];
 #pragma unroll
-  for (int i = 0; i < 4; ++i)
+  for (int i = 0; i < TI; ++i)
 #pragma unroll
     for (int j = 0; j < 2; ++j) {
       big[i][j] = f16v{};
@@ -210,15 +221,15 @@ __global__ __launch_bounds__(XNT, 1) void gemm_bf16x3_kernel(XArgs p) {
                                                (wn + j * 32 + fr) * 16 + fsw));
 #pragma unroll
     for (int pa = 0; pa < 3; ++pa) {
-      bf8 af[4];
+      bf8 af[TI];
 #pragma unroll
-      for (int i = 0; i < 4; ++i)
+      for (int i = 0; i < TI; ++i)
         af[i] = __builtin_bit_cast(bf8, *reinterpret_cast<const u16x8 *>(
                                             slot + pa * 4096 + (wm + i * 32 + fr) * 16 + fsw));
 #pragma unroll
       for (int pb = 0; pb < 3 - pa; ++pb)
 #pragma unroll
-        for (int i = 0; i < 4; ++i)
+        for (int i = 0; i < TI; ++i)
 #pragma unroll
           for (int j = 0; j < 2; ++j) {
             if (pa == 0 && pb == 0)
@@ -254,7 +265,7 @@ __global__ __launch_bounds__(XNT, 1) void gemm_bf16x3_kernel(XArgs p) {
 
   const float alpha = p.alpha, beta = p.beta;
 #pragma unroll
-  for (int i = 0; i < 4; ++i)
+  for (int i = 0; i < TI; ++i)
 #pragma unroll
     for (int j = 0; j < 2; ++j) {
       const int col = n0 + wn + j * 32 + (lane & 31);
@@ -317,7 +328,7 @@ extern "C" int mublis_gfx950_sgemm_bf16x3_dev(int m, int n, int k, float alpha,
     }
     p.c = c; p.rs_c = rs_c; p.cs_c = cs_c;
     p.tiles_m = mp / XTM; p.tiles_n = np / XTN; p.ntiles = p.tiles_m * p.tiles_n;
-    hipLaunchKernelGGL(gemm_bf16x3_kernel, dim3(p.ntiles), dim3(XNT), 0, s, p);
+    hipLaunchKernelGGL(gemm_bf16x3_kernel<kWaves>, dim3(p.ntiles), dim3(64 * kWaves), 0, s, p);
     if (hipGetLastError() != hipSuccess) st = MUBLIS_EHIP;
   }
   if (hipStreamSynchronize(s) != hipSuccess && st == MUBLIS_OK) st = MUBLIS_EHIP;
