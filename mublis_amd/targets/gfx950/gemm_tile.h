@@ -92,8 +92,8 @@ struct GemmArgs {
 // which lets the short tiles fill the slots the long ones leave (longest
 // processing time first).  The grid is 8 * ceil(units / 8) * g workgroups;
 // returns false for the padding workgroups.
-__device__ inline bool map_tile_lpt(int w, int tiles_m, int tiles_n, int lpt, int g, int &tm,
-                                    int &tn) {
+__host__ __device__ inline bool map_tile_lpt(int w, int tiles_m, int tiles_n, int lpt, int g,
+                                             int &tm, int &tn) {
   const bool rows = lpt <= 2;
   const int lines = rows ? tiles_m : tiles_n, per = rows ? tiles_n : tiles_m;
   const int upl = (per + g - 1) / g;  // units per line
@@ -116,8 +116,8 @@ __host__ __device__ inline int lpt_unit(int lines, int per) { return lines >= 32
 // that each XCD owns a contiguous range, and that range is walked in groups
 // of 8 tile rows, so the workgroups resident on one XCD share A and B panels
 // in its L2.
-__device__ inline void map_tile(int w, int ntiles, int tiles_m, int tiles_n,
-                                int tri, int ji, int &tm, int &tn) {
+__host__ __device__ inline void map_tile(int w, int ntiles, int tiles_m, int tiles_n,
+                                         int tri, int ji, int &tm, int &tn) {
   const int q = ntiles / 8, r = ntiles % 8;
   const int xcd = w % 8, local = w / 8;
   const int id = (xcd < r ? xcd * (q + 1) : r * (q + 1) + (xcd - r) * q) + local;
@@ -143,7 +143,7 @@ __device__ inline void map_tile(int w, int ntiles, int tiles_m, int tiles_n,
   const int per_group = GROUP * tiles_n;
   const int g = id / per_group;
   const int first_m = g * GROUP;
-  const int gsize = min(tiles_m - first_m, GROUP);
+  const int gsize = tiles_m - first_m < GROUP ? tiles_m - first_m : GROUP;
   const int in = id - g * per_group;
   tm = first_m + in % gsize;
   tn = in / gsize;

@@ -1,0 +1,148 @@
+// Host check of the GEMM kernels' workgroup -> tile maps (gemm_tile.h),
+// compiled from the same header the kernels use:
+//  * map_tile, full grid and triangle modes 1..3: every workgroup lands on a
+//    distinct tile inside the grid, and modes 2 + 3 together cover exactly
+//    the 64-tiles touching the ji triangle, each once;
+//  * map_tile_lpt, lines along rows or columns, long end high or low: a
+//    bijection onto the tiles (padding workgroups excepted), every XCD
+//    (workgroup id mod 8) receives the same total K up to one dealt unit,
+//    and each XCD meets its tiles in non-increasing K.
+#include <cstdio>
+#include <cstdlib>
+#include <set>
+#include <utility>
+#include <vector>
+
+#include "gemm_tile.h"
+
+using gfx950::lpt_unit;
+using gfx950::map_tile;
+using gfx950::map_tile_lpt;
+
+static int failures = 0;
+#define CHECK(c, ...)                       \
+  do {                                      \
+    if (!(c)) {                             \
+      if (failures++ < 20) {                \
+        std::printf("FAIL %s: ", #c);       \
+        std::printf(__VA_ARGS__);           \
+        std::printf("\n");                  \
+      }                                     \
+    }                                       \
+  } while (0)
+
+static void full_grid(int tm_, int tn_) {
+  std::set<std::pair<int, int>> seen;
+  const int nt = tm_ * tn_;
+  for (int w = 0; w < nt; ++w) {
+    int a, b;
+    map_tile(w, nt, tm_, tn_, 0, 0, a, b);
+    CHECK(a >= 0 && a < tm_ && b >= 0 && b < tn_, "full %dx%d w %d -> %d,%d", tm_, tn_, w, a, b);
+    CHECK(seen.insert({a, b}).second, "full %dx%d duplicate %d,%d", tm_, tn_, a, b);
+  }
+  CHECK((int)seen.size() == nt, "full %dx%d covers %zu", tm_, tn_, seen.size());
+}
+
+static void triangle(int t, int ji) {
+  std::set<std::pair<int, int>> seen;
+  const int nt = t * (t + 1) / 2;
+  for (int w = 0; w < nt; ++w) {
+    int a, b;
+    map_tile(w, nt, t, t, 1, ji, a, b);
+    const bool in = ji == 1 ? b <= a : a <= b;
+    CHECK(in && a < t && b < t, "tri %d ji %d w %d -> %d,%d", t, ji, w, a, b);
+    CHECK(seen.insert({a, b}).second, "tri %d duplicate", t);
+  }
+  CHECK((int)seen.size() == nt, "tri %d covers %zu", t, seen.size());
+}
+
+// Modes 2 (128-tiles strictly inside) + 3 (diagonal 128-blocks as 64-tiles)
+// against the 64-tiles of an n x n triangle.
+static void split_triangle(int n, int ji) {
+  const int t128 = (n + 127) / 128, t64 = (n + 63) / 64;
+  std::vector<int> hits(t64 * t64, 0);
+  const int n2 = t128 * (t128 - 1) / 2;
+  for (int w = 0; w < n2; ++w) {
+    int a, b;
+    map_tile(w, n2, t128, t128, 2, ji, a, b);
+    CHECK(a < t128 && b < t128 && (ji == 1 ? b < a : a < b), "strict n %d -> %d,%d", n, a, b);
+    for (int i = 2 * a; i < 2 * a + 2 && i < t64; ++i)
+      for (int j = 2 * b; j < 2 * b + 2 && j < t64; ++j) hits[i * t64 + j]++;
+  }
+  const int n3 = 3 * ((t64 + 1) / 2);
+  for (int w = 0; w < n3; ++w) {
+    int a, b;
+    map_tile(w, n3, t64, t64, 3, ji, a, b);
+    if (a >= t64 || b >= t64) continue;  // the kernel's bounds exit
+    CHECK(a / 2 == b / 2, "diag n %d -> %d,%d off the diagonal block", n, a, b);
+    hits[a * t64 + b]++;
+  }
+  for (int i = 0; i < t64; ++i)
+    for (int j = 0; j < t64; ++j) {
+      const bool want = ji == 1 ? j <= i : i <= j;
+      CHECK(hits[i * t64 + j] == (want ? 1 : 0), "split n %d ji %d tile %d,%d hit %d", n, ji,
+            i, j, hits[i * t64 + j]);
+    }
+}
+
+static void lpt(int tm_, int tn_, int mode) {
+  const bool rows = mode <= 2;
+  const int lines = rows ? tm_ : tn_, per = rows ? tn_ : tm_;
+  const int g = lpt_unit(lines, per);
+  const long units = (long)lines * ((per + g - 1) / g);
+  const int wgs = (int)(8 * ((units + 7) / 8) * g);
+  // K of a tile: its line's rank from the long end, plus one
+  auto kof = [&](int a, int b) {
+    const int line = rows ? a : b;
+    return (mode == 1 || mode == 3) ? line + 1 : lines - line;
+  };
+  std::set<std::pair<int, int>> seen;
+  long sum[8] = {0};
+  int last[8];
+  for (int x = 0; x < 8; ++x) last[x] = 1 << 30;
+  for (int w = 0; w < wgs; ++w) {
+    int a, b;
+    if (!map_tile_lpt(w, tm_, tn_, mode, g, a, b)) continue;
+    CHECK(a >= 0 && a < tm_ && b >= 0 && b < tn_, "lpt %dx%d m%d w %d -> %d,%d", tm_, tn_,
+          mode, w, a, b);
+    CHECK(seen.insert({a, b}).second, "lpt %dx%d m%d duplicate", tm_, tn_, mode);
+    const int x = w % 8, k = kof(a, b);
+    CHECK(k <= last[x], "lpt %dx%d m%d XCD %d K rises %d -> %d", tm_, tn_, mode, x, last[x], k);
+    last[x] = k;
+    sum[x] += k;
+  }
+  CHECK((int)seen.size() == tm_ * tn_, "lpt %dx%d m%d covers %zu", tm_, tn_, mode, seen.size());
+  long lo = sum[0], hi = sum[0];
+  for (int x = 1; x < 8; ++x) {
+    lo = sum[x] < lo ? sum[x] : lo;
+    hi = sum[x] > hi ? sum[x] : hi;
+  }
+  // Snake dealing: two consecutive rounds give every XCD a pair of units
+  // whose K ranks sum to the same value, so with K linear in the rank the
+  // first-order differences cancel; round-robin dealing would leave XCD 0
+  // ahead by ~7/8 of the longest unit's K.  Few units: at most one unit.
+  const long unit_k = (long)lines * g;
+  const long bound = units >= 64 ? unit_k / 4 + 8L * g : unit_k;
+  CHECK(hi - lo <= bound, "lpt %dx%d m%d XCD K spread %ld > %ld", tm_, tn_, mode, hi - lo,
+        bound);
+}
+
+int main() {
+  for (int a = 1; a <= 40; ++a)
+    for (int b : {1, 2, 3, 7, 8, 9, 16, 33}) full_grid(a, b);
+  for (int t = 1; t <= 70; ++t) {
+    triangle(t, 1);
+    triangle(t, 2);
+  }
+  for (int n : {1, 63, 64, 65, 127, 128, 129, 200, 255, 256, 1000, 4100, 8192})
+    for (int ji : {1, 2}) split_triangle(n, ji);
+  for (int a : {1, 2, 5, 8, 9, 16, 31, 32, 33, 64, 128, 129})
+    for (int b : {1, 3, 8, 16, 17, 64})
+      for (int m = 1; m <= 4; ++m) lpt(a, b, m);
+  if (failures) {
+    std::printf("%d failures\n", failures);
+    return 1;
+  }
+  std::printf("ok\n");
+  return 0;
+}
