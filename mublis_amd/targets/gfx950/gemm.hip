@@ -22,6 +22,7 @@
 #include <stdint.h>
 
 #include <algorithm>
+#include <cstring>
 
 #include "mublis_gfx950.h"
 #include "gfx950_common.h"
@@ -35,6 +36,12 @@ using namespace gfx950;
 template <typename T, int TILE, bool A_KC, bool B_NC>
 __global__ __launch_bounds__(NT, Tile<TILE>::MIN_WG_PER_CU) void gemm_kernel(GemmArgs<T> p) {
   __shared__ T lds[LdsSize<T, TILE>::value];
+  if (gridDim.y > 1) {
+    const int64_t zi = blockIdx.y % p.bat, zo = blockIdx.y / p.bat;
+    p.a += zi * p.bsa[0] + zo * p.bsa[1];
+    p.b += zi * p.bsb[0] + zo * p.bsb[1];
+    p.c += zi * p.bsc[0] + zo * p.bsc[1];
+  }
   int tm, tn;
   if (p.lpt) {
     if (!map_tile_lpt(blockIdx.x, p.tiles_m, p.tiles_n, p.lpt, p.lpt_g, tm, tn)) return;
@@ -68,7 +75,7 @@ bool aligned16(const T *x, int64_t ld) {
 }
 
 template <typename T, int TILE>
-hipError_t launch(GemmArgs<T> p, bool akc, bool bnc, hipStream_t s) {
+hipError_t launch(GemmArgs<T> p, bool akc, bool bnc, hipStream_t s, int nbatch = 1) {
   p.tiles_m = (p.m + TILE - 1) / TILE;
   p.tiles_n = (p.n + TILE - 1) / TILE;
   const int t = p.tiles_m;
@@ -85,7 +92,7 @@ hipError_t launch(GemmArgs<T> p, bool akc, bool bnc, hipStream_t s) {
     const int64_t units = (int64_t)lines * ((per + p.lpt_g - 1) / p.lpt_g);
     wgs = (int)(8 * ((units + 7) / 8) * p.lpt_g);
   }
-  const dim3 grid(wgs), block(NT);
+  const dim3 grid(wgs, nbatch), block(NT);
   if (akc && bnc) hipLaunchKernelGGL((gemm_kernel<T, TILE, true, true>), grid, block, 0, s, p);
   else if (akc) hipLaunchKernelGGL((gemm_kernel<T, TILE, true, false>), grid, block, 0, s, p);
   else if (bnc) hipLaunchKernelGGL((gemm_kernel<T, TILE, false, true>), grid, block, 0, s, p);
@@ -251,6 +258,56 @@ template int gfx950_gemm<float>(int, int, int, float, const float *, int64_t, in
 template int gfx950_gemm<double>(int, int, int, double, const double *, int64_t,
                                  int64_t, const double *, int64_t, int64_t, double,
                                  double *, int64_t, int64_t, hipStream_t);
+
+// Batched form for the TRSM inversion: `count` problems of one shape whose
+// operands are offset per problem as GemmArgs::bat / bs* describe (strides
+// {inner, outer} per operand, `bat` problems per outer step).  A and B need
+// one unit stride each; no workspace, no densify.
+template <typename T>
+int gfx950_gemm_batched(int m, int n, int k, T alpha, const T *a, int64_t rs_a, int64_t cs_a,
+                        const int64_t bsa[2], const T *b, int64_t rs_b, int64_t cs_b,
+                        const int64_t bsb[2], T beta, T *c, int64_t rs_c, int64_t cs_c,
+                        const int64_t bsc[2], int count, int bat, int tri_a, int tri_b,
+                        hipStream_t s) {
+  if (m <= 0 || n <= 0 || count <= 0) return MUBLIS_OK;
+  if (k <= 0 || bat <= 0 || count > 65535) return MUBLIS_EINVAL;
+  if ((rs_a != 1 && cs_a != 1) || (rs_b != 1 && cs_b != 1)) return MUBLIS_EINVAL;
+  const bool akc = (cs_a == 1), bnc = (rs_b != 1);
+  GemmArgs<T> p;
+  memset(&p, 0, sizeof p);
+  p.m = m; p.n = n; p.k = k; p.alpha = alpha; p.beta = beta;
+  p.a = a; p.rs_a = rs_a; p.cs_a = cs_a;
+  p.b = b; p.rs_b = rs_b; p.cs_b = cs_b;
+  p.c = c; p.rs_c = rs_c; p.cs_c = cs_c;
+  p.tri_a = tri_a; p.tri_b = tri_b;
+  p.lpt = tri_a ? (tri_a == 1 ? 1 : 2) : tri_b ? (tri_b == 2 ? 3 : 4) : 0;
+  p.lpt_g = 1;
+  // every problem's operands must keep the 16-byte alignment of the first
+  auto al = [](const int64_t *bs) {
+    return (bs[0] * (int64_t)sizeof(T)) % 16 == 0 && (bs[1] * (int64_t)sizeof(T)) % 16 == 0;
+  };
+  p.vec_a = aligned16(a, akc ? rs_a : cs_a) && al(bsa);
+  p.vec_b = aligned16(b, bnc ? rs_b : cs_b) && al(bsb);
+  p.bat = bat;
+  for (int q = 0; q < 2; ++q) {
+    p.bsa[q] = bsa[q];
+    p.bsb[q] = bsb[q];
+    p.bsc[q] = bsc[q];
+  }
+  const int64_t t128 = (int64_t)((m + 127) / 128) * ((n + 127) / 128) * count;
+  const hipError_t e = t128 >= 2048 ? launch<T, 128>(p, akc, bnc, s, count)
+                                    : launch<T, 64>(p, akc, bnc, s, count);
+  return e == hipSuccess ? MUBLIS_OK : MUBLIS_EHIP;
+}
+template int gfx950_gemm_batched<float>(int, int, int, float, const float *, int64_t, int64_t,
+                                        const int64_t *, const float *, int64_t, int64_t,
+                                        const int64_t *, float, float *, int64_t, int64_t,
+                                        const int64_t *, int, int, int, int, hipStream_t);
+template int gfx950_gemm_batched<double>(int, int, int, double, const double *, int64_t,
+                                         int64_t, const int64_t *, const double *, int64_t,
+                                         int64_t, const int64_t *, double, double *, int64_t,
+                                         int64_t, const int64_t *, int, int, int, int,
+                                         hipStream_t);
 
 template <typename T>
 int gfx950_gemm_tri_a(int m, int n, int k, T alpha, const T *a, int64_t rs_a, int64_t cs_a,

@@ -80,6 +80,11 @@ struct GemmArgs {
   // tile rows are the lines, longest at the high / low end; 3 / 4 the same
   // for tile columns.  lpt_g: tiles per dealt unit (a whole line, or 1).
   int lpt, lpt_g;
+  // Batched launch (gridDim.y problems, the one-problem kernel only):
+  // problem z starts at a + (z % bat) * bsa[0] + (z / bat) * bsa[1], and
+  // likewise for b and c.
+  int bat;
+  int64_t bsa[2], bsb[2], bsc[2];
 };
 
 // Long-first walk: with a triangular operand, a tile's K extent grows

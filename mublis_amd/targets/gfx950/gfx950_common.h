@@ -28,6 +28,15 @@ template <typename T>
 int gfx950_gemm(int m, int n, int k, T alpha, const T *a, int64_t rs_a, int64_t cs_a,
                 const T *b, int64_t rs_b, int64_t cs_b, T beta, T *c, int64_t rs_c,
                 int64_t cs_c, hipStream_t s);
+// `count` problems of one shape in one launch; operand X of problem z starts
+// at x + (z % bat) * bsx[0] + (z / bat) * bsx[1].  A and B need a unit
+// stride.  tri_a / tri_b as for gfx950_gemm_tri_a.
+template <typename T>
+int gfx950_gemm_batched(int m, int n, int k, T alpha, const T *a, int64_t rs_a, int64_t cs_a,
+                        const int64_t bsa[2], const T *b, int64_t rs_b, int64_t cs_b,
+                        const int64_t bsb[2], T beta, T *c, int64_t rs_c, int64_t cs_c,
+                        const int64_t bsc[2], int count, int bat, int tri_a, int tri_b,
+                        hipStream_t s);
 // Same with a triangular A whose zero triangle's K slices are skipped
 // (tri_a 1: A(i,p) = 0 for p > i, 2: for p < i).
 template <typename T>

@@ -550,16 +550,17 @@ int pipe(bool lower, int m, int n, T alpha, const T *a, int64_t rs_a, int64_t cs
 }
 
 // Wide right-hand sides: every kInvLeaf-row diagonal block of A is
-// inverted whole (one launch of independent pipelined solves against the
-// identity), and the recursion, split at multiples of kInvLeaf, ends in
-// leaves X = alpha * inv(A_ii) * B_i -- a copy of B_i and one GEMM that
-// skips the inverse's zero triangle.  No leaf then waits on a chain of block
-// rows.  Explicitly inverted diagonal blocks are the usual GPU TRSM form;
-// their forward error grows with the blocks' conditioning, not with m.
-// Block rows: 1024 for double data, 512 for float, whose GEMMs are fast
-// enough that the longer inversion chain of 1024-row blocks does not pay.
+// inverted whole (by doubling the NB-block inverses with batched GEMMs, or
+// by one launch of independent pipelined solves against the identity when A
+// has no unit stride), and the recursion, split at multiples of kInvLeaf,
+// ends in leaves X = alpha * inv(A_ii) * B_i -- a copy of B_i and one GEMM
+// that skips the inverse's zero triangle.  No leaf then waits on a chain of
+// block rows.  Explicitly inverted diagonal blocks are the usual GPU TRSM
+// form; their forward error grows with the blocks' conditioning, not with m.
+// Block rows (both types): the leaves then hold the fewest small GEMMs,
+// and inverting 1024-row blocks by doubling costs a few batched GEMMs.
 template <typename T>
-constexpr int kInvLeaf = sizeof(T) == 8 ? 1024 : 512;
+constexpr int kInvLeaf = 1024;
 constexpr int kInvMinN = 256;
 // Column group of the identity solves: narrow groups keep the pipeline's
 // LDS small, so all inversions run at once instead of in waves.
@@ -605,6 +606,87 @@ int inv_blocks(bool lower, int m, const T *a, int64_t rs_a, int64_t cs_a, const 
                      1 + (kInvLeaf<T> / NB) * cg, 1};
   return pipe<T>(lower, m, kInvLeaf<T>, T(1), a, rs_a, cs_a, iv, 1, kInvLeaf<T>, ad, flags, 1,
                  s, &bt, kEyeCw);
+}
+
+// iv := the block diagonal of the NB-block inverses in `ad` (row-major NB x NB
+// each) in inv_blocks' layout, zero elsewhere.
+template <typename T>
+__global__ void inv_scatter(int m, const double *__restrict__ ad, T *__restrict__ iv,
+                            int64_t total) {
+  constexpr int64_t L = kInvLeaf<T>;
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t z = t / (L * L), e = t - z * L * L;
+    const int64_t r = z * L + e % L, c = z * L + e / L;  // global row, column
+    T v = T(0);
+    if (r < m && c < m && r / NB == c / NB)
+      v = (T)ad[(r / NB) * NB * NB + (r % NB) * NB + c % NB];
+    iv[t] = v;
+  }
+}
+
+// The same inverses by doubling the NB-block ones: for h = NB, 2 NB, ..
+// kInvLeaf / 2, every aligned 2h-block [A11 0; A21 A22] (lower) gets
+//   inv21 = -inv22 * (A21 * inv11)
+// and every [A11 A12; 0 A22] (upper) inv12 = -inv11 * (A12 * inv22) -- two
+// batched GEMMs per level over all such blocks at once, each skipping its
+// triangular operand's zero triangle, instead of a chain of dependent block
+// rows.  The last pair of a level may be short (m not a multiple of 2h):
+// two exact-size GEMMs.  `tmp` holds m * kInvLeaf / 4 elements.  A needs a
+// unit stride.
+template <typename T>
+int inv_double(bool lower, int m, const T *a, int64_t rs_a, int64_t cs_a, const double *ad,
+               T *iv, T *tmp, hipStream_t s) {
+  constexpr int L = kInvLeaf<T>;
+  const int64_t total = inv_leaf_elems<T>(m);
+  const int grid = (int)std::min<int64_t>((total + 255) / 256, 1 << 16);
+  hipLaunchKernelGGL(inv_scatter<T>, dim3(grid), dim3(256), 0, s, m, ad, iv, total);
+  if (hipGetLastError() != hipSuccess) return MUBLIS_EHIP;
+  const int64_t da = rs_a + cs_a;  // one diagonal step in A
+  int st = MUBLIS_OK;
+  for (int h = NB; h < L && st == MUBLIS_OK; h *= 2) {
+    const int per = L / (2 * h);  // pairs per L-block
+    const int full = m / (2 * h);
+    const int64_t sA[2] = {2 * h * da, (int64_t)L * da};
+    const int64_t sI[2] = {(int64_t)2 * h * (1 + L), (int64_t)L * L};
+    const int64_t sT[2] = {(int64_t)h * h, (int64_t)per * h * h};
+    const int64_t d1 = (int64_t)h * (1 + L);  // second diagonal half of a pair
+    if (full > 0) {
+      if (lower) {
+        st = gfx950_gemm_batched<T>(h, h, h, T(1), a + (int64_t)h * rs_a, rs_a, cs_a, sA, iv,
+                                    1, L, sI, T(0), tmp, 1, h, sT, full, per, 0, 1, s);
+        if (st == MUBLIS_OK)
+          st = gfx950_gemm_batched<T>(h, h, h, T(-1), iv + d1, 1, L, sI, tmp, 1, h, sT, T(0),
+                                      iv + h, 1, L, sI, full, per, 1, 0, s);
+      } else {
+        st = gfx950_gemm_batched<T>(h, h, h, T(1), a + (int64_t)h * cs_a, rs_a, cs_a, sA,
+                                    iv + d1, 1, L, sI, T(0), tmp, 1, h, sT, full, per, 0, 2, s);
+        if (st == MUBLIS_OK)
+          st = gfx950_gemm_batched<T>(h, h, h, T(-1), iv, 1, L, sI, tmp, 1, h, sT, T(0),
+                                      iv + (int64_t)h * L, 1, L, sI, full, per, 2, 0, s);
+      }
+    }
+    const int o = full * 2 * h, h2 = m - o - h;  // a short last pair
+    if (st == MUBLIS_OK && h2 > 0) {
+      T *ivz = iv + (int64_t)(o / L) * L * L;
+      const int64_t r0 = o % L;
+      const T *a0 = a + (int64_t)o * da;
+      if (lower) {
+        st = gfx950_gemm<T>(h2, h, h, T(1), a0 + (int64_t)h * rs_a, rs_a, cs_a,
+                            ivz + r0 * (1 + L), 1, L, T(0), tmp, 1, h2, s);
+        if (st == MUBLIS_OK)
+          st = gfx950_gemm<T>(h2, h, h2, T(-1), ivz + r0 * (1 + L) + d1, 1, L, tmp, 1, h2,
+                              T(0), ivz + r0 * (1 + L) + h, 1, L, s);
+      } else {
+        st = gfx950_gemm<T>(h, h2, h2, T(1), a0 + (int64_t)h * cs_a, rs_a, cs_a,
+                            ivz + r0 * (1 + L) + d1, 1, L, T(0), tmp, 1, h, s);
+        if (st == MUBLIS_OK)
+          st = gfx950_gemm<T>(h, h2, h, T(-1), ivz + r0 * (1 + L), 1, L, tmp, 1, h, T(0),
+                              ivz + r0 * (1 + L) + (int64_t)h * L, 1, L, s);
+      }
+    }
+  }
+  return st;
 }
 
 // Flags for inv_blocks: one abort flag and (kInvLeaf / NB) x cg block
@@ -692,7 +774,7 @@ int trsm_rec(bool lower, const double *ad, const T *iv, int m, int n, T alpha, c
 // use_inv_leaves<T>(m, n), the recursion ends in GEMM leaves.
 template <typename T>
 int solve_ws(bool lower, int diag_mode, int m, int n, T alpha, const T *a, int64_t rs_a,
-             int64_t cs_a, T *b, int64_t rs_b, int64_t cs_b, double *ad, T *iv, T *w,
+             int64_t cs_a, T *b, int64_t rs_b, int64_t cs_b, double *ad, T *iv, T *w, T *tmp,
              hipStream_t s) {
   const int nbk = (m + NB - 1) / NB;
   const bool piped = m > NB && n <= g_pipe_max_n && flags_fit(m, n);
@@ -708,7 +790,11 @@ int solve_ws(bool lower, int diag_mode, int m, int n, T alpha, const T *a, int64
   if (whole) return pipe<T>(lower, m, n, alpha, a, rs_a, cs_a, b, rs_b, cs_b, ad, flags, 1, s);
   Leaves<T> lv{flags, 0, nullptr};
   if (gemm_leaves) {
-    if ((st = inv_blocks<T>(lower, m, a, rs_a, cs_a, ad, flags, iv, s))) return st;
+    if (tmp && (rs_a == 1 || cs_a == 1))
+      st = inv_double<T>(lower, m, a, rs_a, cs_a, ad, iv, tmp, s);
+    else
+      st = inv_blocks<T>(lower, m, a, rs_a, cs_a, ad, flags, iv, s);
+    if (st) return st;
     lv.epoch = 1;
     lv.w = w;
   }
@@ -722,18 +808,21 @@ int solve(bool lower, int diag_mode, int m, int n, T alpha, const T *a, int64_t 
           int64_t cs_a, T *b, int64_t rs_b, int64_t cs_b, hipStream_t s) {
   double *ad = (double *)mublis_rt_alloc(sizeof(double) * (size_t)ws_doubles(m));
   if (!ad) return MUBLIS_ENOMEM;
-  T *iv = nullptr, *w = nullptr;
+  T *iv = nullptr, *w = nullptr, *tmp = nullptr;
   if (use_inv_leaves<T>(m, n)) {
     // without room for them, the pipelined / panel leaves instead
     iv = (T *)mublis_rt_alloc(sizeof(T) * (size_t)inv_leaf_elems<T>(m));
     w = (T *)mublis_rt_alloc(sizeof(T) * (size_t)kInvLeaf<T> * (size_t)n);
+    // doubling workspace; without it the inverses come from identity solves
+    tmp = (T *)mublis_rt_alloc(sizeof(T) * (size_t)m * kInvLeaf<T> / 4);
   }
   int st = solve_ws<T>(lower, diag_mode, m, n, alpha, a, rs_a, cs_a, b, rs_b, cs_b, ad, iv, w,
-                       s);
+                       tmp, s);
   if (hipStreamSynchronize(s) != hipSuccess && st == MUBLIS_OK) st = MUBLIS_EHIP;
   mublis_rt_free(ad);
   mublis_rt_free(iv);
   mublis_rt_free(w);
+  mublis_rt_free(tmp);
   return st;
 }
 
@@ -797,7 +886,7 @@ static int trsm_ws_dev(int lower, int unit, int m, int n, T alpha, const T *a, i
   if (alpha == T(0)) return gfx950_fill<T>(m, n, T(0), b, rs_b, cs_b, s);
   if (!ws || ws_bytes < mublis_gfx950_trsm_ws_bytes(m)) return MUBLIS_EINVAL;
   return solve_ws<T>(lower != 0, unit ? 1 : 0, m, n, alpha, a, rs_a, cs_a, b, rs_b, cs_b,
-                     (double *)ws, (T *)nullptr, (T *)nullptr, s);
+                     (double *)ws, (T *)nullptr, (T *)nullptr, (T *)nullptr, s);
 }
 
 extern "C" int mublis_gfx950_dtrsm_ws_dev(int lower, int unit, int m, int n, double alpha,

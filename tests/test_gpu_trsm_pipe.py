@@ -13,8 +13,8 @@ Bars:
   the column group (16, 32 or 64 columns), a single right-hand side; the
   SHAPES are ones the default dispatch sends to the pipeline (n <= 128 or
   m n <= 2^20), the wider ones go through the recursive schedule with
-  GEMM leaves (512-row diagonal blocks inverted by a batch of pipelined
-  solves against the identity).
+  GEMM leaves (1024-row diagonal blocks inverted by doubling the 64-row
+  inverses with batched GEMMs).
 """
 import ctypes
 import itertools
@@ -141,10 +141,9 @@ def test_pipelined_right_side(lib, prod, dt):
 @pytest.mark.parametrize("dt", L.DTYPES)
 @pytest.mark.parametrize("m,n", [(2000, 1000), (1100, 2100)])
 def test_recursive_schedule_with_gemm_leaves_wide(lib, prod, dt, m, n):
-    """Wide B (n >= 256): the diagonal blocks of 512 (float) or 1024
-    (double) rows are inverted in one launch of independent pipelined
-    solves and the recursion, split at multiples of that size (float:
-    2000 -> 1024 + 976 -> 512 + 464; double: 2000 -> 1024 + 976), ends in
+    """Wide B (n >= 256): the 1024-row diagonal blocks are inverted by
+    doubling the 64-row inverses with batched GEMMs and the recursion, split
+    at multiples of 1024 (2000 -> 1024 + 976; 1100 -> 1024 + 76), ends in
     GEMM leaves; compared with the recursive schedule down to 64-row leaves
     (pipeline disabled) and checked by the residual probe."""
     rng = np.random.default_rng(21)
@@ -181,14 +180,14 @@ def test_pipe_threshold_knob(lib):
 
 
 GEMM_LEAF_CASES = [
-    # (m, n, order, right): the solve's order is n on the right side;
-    # leaves for float (512-row blocks) / double (1024-row blocks)
-    (1500, 300, "F", False),  # 512, 512, 476 / 1024, 476
-    (2600, 256, "F", False),  # ..., a 40-row last leaf / 2048 + 552
-    (777, 513, "C", False),   # row-major B: 512 + 265 / pipelined leaves
+    # (m, n, order, right): the solve's order is n on the right side; the
+    # inverted blocks have 1024 rows, so the doubling's last pairs are short
+    (1500, 300, "F", False),  # leaves 1024 + 476
+    (2600, 256, "F", False),  # 2048 + 552
+    (777, 513, "C", False),   # row-major B, pipelined leaves (m < 1024)
     (300, 1500, "F", True),   # right side: B's transposed view, 1500-row solve
-    (257, 777, "C", True),
-    (128, 2100, "F", True),   # double: 1024 + 1024 + 52
+    (257, 1111, "C", True),   # row-major, 1024 + 87
+    (128, 2100, "F", True),   # 1024 + 1024 + 52: a 52-row last block
 ]
 
 
@@ -229,3 +228,34 @@ def test_gemm_leaves_ragged(lib, prod, dt, m, n, order, right):
         if not diff <= 50 * s_ * EPS[np.dtype(dt)]:
             failures.append(f"{tag}: vs 64-row leaves {diff:.3e}")
     assert not failures, "\n".join(failures)
+
+
+@pytest.mark.parametrize("dt", L.DTYPES)
+@pytest.mark.parametrize("lower", [True, False])
+def test_gemm_leaves_without_unit_stride(lib, dt, lower):
+    """mublis_?trsm on an A with no unit stride (every other row of a
+    2m x m column-major array: rs = 2, cs = 2m): the GEMM leaves' inverses
+    then come from the batched pipelined identity solves instead of the
+    doubling GEMMs; against the 64-row-leaf schedule and the residual
+    probe."""
+    m, n = 1100, 300
+    rng = np.random.default_rng(1100 + lower)
+    a = L.conditioned_triangular(rng, m, dt, "F", lower, False)
+    big = np.zeros((2 * m, m), dtype=dt, order="F")
+    big[0::2, :] = a
+    b0 = L.rand(rng, (m, n), dt, "F")
+    f = lib.mublis_dtrsm if np.dtype(dt) == np.float64 else lib.mublis_strsm
+    outs = []
+    for max_n in (-1, 0):
+        old = lib.mublis_gfx950_trsm_pipe_max_n(max_n)
+        try:
+            x = np.array(b0, copy=True, order="F")
+            st = f(0, 1 if lower else 2, 0, 0, m, n, 1.5, M.ptr(big), 2, 2 * m, M.ptr(x), 1, m)
+        finally:
+            lib.mublis_gfx950_trsm_pipe_max_n(old)
+        assert st == 0
+        outs.append(x)
+    res, lim = residual(a, outs[0], b0, 1.5, False, lower, False, False)
+    diff = np.linalg.norm(outs[0].astype(np.float64) - outs[1]) / np.linalg.norm(outs[1])
+    assert np.isfinite(outs[0]).all() and res <= lim, (res, lim)
+    assert diff <= 50 * m * EPS[np.dtype(dt)], diff
