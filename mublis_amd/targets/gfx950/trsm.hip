@@ -466,7 +466,8 @@ __global__ __launch_bounds__(256) void trsm_panel(int m, int n, T alpha,
 // up to m * n = 2^20; beyond, the recursive schedule's large GEMMs fill the
 // GPU better than one chain of block rows (measured crossover on MI355X,
 // FP64: n ~ 1024 at m = 1024, 256 at 4096, 128 at 8192).  The recursive
-// schedule in turn ends in GEMM leaves (B from 256 columns, see
+// schedule in turn ends in GEMM leaves (B from 256 columns for double, from
+// 64 for float, which then skips the whole-solve pipeline too; see
 // use_inv_leaves), else hands its diagonal sub-problems of at most kPipeLeaf
 // rows to the pipeline (narrower B) or to column-panel workgroups (wider B,
 // when the solve brings no room for the GEMM leaves' inverses).
@@ -561,14 +562,19 @@ int pipe(bool lower, int m, int n, T alpha, const T *a, int64_t rs_a, int64_t cs
 // and inverting 1024-row blocks by doubling costs a few batched GEMMs.
 template <typename T>
 constexpr int kInvLeaf = 1024;
-constexpr int kInvMinN = 256;
+// Narrowest B for GEMM leaves.  Float solves take them whenever they
+// apply (their pipeline runs the block updates in FP64: 4096 x 256 solves in
+// 1.00 ms pipelined, 0.48 ms with GEMM leaves); double solves keep the
+// pipeline where it is at least as fast (n <= 128 or m n <= 2^20).
+template <typename T>
+constexpr int kInvMinN = sizeof(T) == 8 ? 256 : 64;
 // Column group of the identity solves: narrow groups keep the pipeline's
 // LDS small, so all inversions run at once instead of in waves.
 constexpr int kEyeCw = 32;
 
 template <typename T>
 bool use_inv_leaves(int m, int n) {
-  return m > kInvLeaf<T> && n >= kInvMinN && g_pipe_max_n >= kInvLeaf<T>;
+  return m > kInvLeaf<T> && n >= kInvMinN<T> && g_pipe_max_n >= kInvLeaf<T>;
 }
 
 template <typename T>
@@ -778,8 +784,9 @@ int solve_ws(bool lower, int diag_mode, int m, int n, T alpha, const T *a, int64
              hipStream_t s) {
   const int nbk = (m + NB - 1) / NB;
   const bool piped = m > NB && n <= g_pipe_max_n && flags_fit(m, n);
-  const bool whole = piped && use_pipe(m, n);
-  const bool gemm_leaves = !whole && iv && w && use_inv_leaves<T>(m, n);
+  const bool gemm_ok = iv && w && use_inv_leaves<T>(m, n);
+  const bool whole = piped && use_pipe(m, n) && !(sizeof(T) == 4 && gemm_ok);
+  const bool gemm_leaves = !whole && gemm_ok;
   int *flags = (piped || gemm_leaves) ? reinterpret_cast<int *>(ad + (int64_t)nbk * NB * NB)
                                       : nullptr;
   const int cg = (n + pipe_cw(n) - 1) / pipe_cw(n);

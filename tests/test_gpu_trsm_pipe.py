@@ -188,6 +188,8 @@ GEMM_LEAF_CASES = [
     (300, 1500, "F", True),   # right side: B's transposed view, 1500-row solve
     (257, 1111, "C", True),   # row-major, 1024 + 87
     (128, 2100, "F", True),   # 1024 + 1024 + 52: a 52-row last block
+    (1100, 64, "F", False),   # float: GEMM leaves from n = 64 (double: pipeline)
+    (2050, 100, "C", False),  # float: 2048 + 2 rows
 ]
 
 
