@@ -130,8 +130,10 @@ size_t mublis_gfx950_trsm_ws_bytes(int m);
 /* Widest B (columns after the side transformation) that the left-side
  * solves hand to the pipelined GEMMTRSM kernel (one launch, block rows
  * chained by device-side flags) instead of the recursive schedule; set it
- * with max_n >= 0 (at most 4096), query with max_n < 0.  Returns the
- * previous value.  Tuning knob; the frame's calls share it. */
+ * with max_n >= 0 (at most 4096), query with max_n < 0.  Values below 1024
+ * also turn off the recursive schedule's GEMM leaves (inverted 1024-row
+ * diagonal blocks), so 0 leaves the plain recursion with 64-row leaves.
+ * Returns the previous value.  Tuning knob; the frame's calls share it. */
 int mublis_gfx950_trsm_pipe_max_n(int max_n);
 int mublis_gfx950_dtrsm_ws_dev(int lower, int unit, int m, int n, double alpha,
                                const double *a, int64_t rs_a, int64_t cs_a, double *b,
