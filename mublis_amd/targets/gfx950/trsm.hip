@@ -206,6 +206,12 @@ __global__ __launch_bounds__(256) void trsm_pipe(int m, int n, T alpha,
   constexpr int TN = CW / 16;          // 16-column MFMA tiles per wave
   constexpr int TV = PB * CW / 256;    // tile elements per thread
   __shared__ double xs[2][PB][CW + 1];
+  // Folding the last dependency (below) pays with 64-column groups, where
+  // the chain it takes off the critical path is as long as forming M; with
+  // narrower groups it measured 10 % slower.  A(p, q) and M use the second
+  // X buffer.
+  constexpr bool FOLD = CW == PB;
+  double(*ab)[PB + 1] = reinterpret_cast<double(*)[PB + 1]>(&xs[1][0][0]);
   __shared__ int s_front;
   {
     const int z = blockIdx.y;
@@ -251,14 +257,19 @@ __global__ __launch_bounds__(256) void trsm_pipe(int m, int n, T alpha,
 #pragma unroll
   for (int t = 0; t < TN; ++t) acc[t] = d4{0, 0, 0, 0};
   double av[16], xr[TV];
+  // Every dependency is accumulated as it is published -- with FOLD all but
+  // the last, X(j - 1), which is folded in after the wait as one product
+  // with M = inv(A(p, p)) A(p, q(j - 1)), formed beforehand.
+  const bool fold = FOLD && j > i0;
+  const int jend = fold ? j - 1 : j;
   int i = i0;
-  while (i < j) {
+  while (i < jend) {
     // A(p, q(i)) before the wait; then the run of published blocks i .. f-1
     load_ablk<T>(av, a, rs_a, cs_a, m, arow, LOWER ? i : nbk - 1 - i, lk);
     if (tid == 0) {
       wait_flag(&flags[i * cg + cgi], epoch, abort_flag);
       int f = i + 1;
-      while (f < j &&
+      while (f < jend &&
              __hip_atomic_load(&flags[f * cg + cgi], __ATOMIC_RELAXED,
                                __HIP_MEMORY_SCOPE_AGENT) == epoch)
         ++f;
@@ -290,8 +301,8 @@ __global__ __launch_bounds__(256) void trsm_pipe(int m, int n, T alpha,
     }
     __syncthreads();  // the next run reuses both buffers
   }
-  // R = alpha * B(p) - update, in xs[0] (FP64 MFMA output row of register r
-  // in lane group g is g + 4 r)
+  // Y = inv(A(p, p)) (alpha * B(p) - update) (FP64 MFMA output row of
+  // register r in lane group g is g + 4 r); R goes through xs[0]
   tile_to_lds<CW>(breg, xs[0], cm);
   __syncthreads();
   const double al = (double)alpha;
@@ -312,6 +323,49 @@ __global__ __launch_bounds__(256) void trsm_pipe(int m, int n, T alpha,
     for (int t = 0; t < TN; ++t)
       out[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(ireg[kk], xs[0][4 * kk + lk][16 * t + lr],
                                                     out[t], 0, 0, 0);
+  if (FOLD && fold) {
+    // M = inv(A(p, p)) A(p, ql) while X(j - 1) is still being solved: A(p,
+    // ql) through LDS as the B operand, M back through LDS into the A-operand
+    // layout, negated.  After the wait only X(p) = Y - M X(ql) remains.
+    const int ql = LOWER ? j - 1 : nbk - j;
+    {
+      double at[PB * PB / 256];
+      load_tile<T, PB>(at, a, rs_a, cs_a, m, m, p, ql * PB);
+      tile_to_lds<PB>(at, ab, rs_a == 1);
+    }
+    __syncthreads();  // A(p, ql) staged; every wave is past its Y MFMAs
+    d4 mm[PB / 16];
+#pragma unroll
+    for (int t = 0; t < PB / 16; ++t) mm[t] = d4{0, 0, 0, 0};
+#pragma unroll
+    for (int kk = 0; kk < 16; ++kk)
+#pragma unroll
+      for (int t = 0; t < PB / 16; ++t)
+        mm[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(ireg[kk], ab[4 * kk + lk][16 * t + lr],
+                                                     mm[t], 0, 0, 0);
+    __syncthreads();  // every wave has read A(p, ql)
+#pragma unroll
+    for (int t = 0; t < PB / 16; ++t)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) ab[16 * wave + lk + 4 * r][16 * t + lr] = mm[t][r];
+    __syncthreads();
+    double mreg[16];
+#pragma unroll
+    for (int kk = 0; kk < 16; ++kk) mreg[kk] = -ab[16 * wave + lr][4 * kk + lk];
+    if (tid == 0) wait_flag(&flags[(j - 1) * cg + cgi], epoch, abort_flag);
+    __syncthreads();  // also: every wave has read M (ab may be xs[1])
+    // acquire: X(ql) was published with a release store
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    load_tile<T, CW>(xr, b, rs_b, cs_b, m, n, ql, c0);
+    tile_to_lds<CW>(xr, xs[0], cm);
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < 16; ++kk)
+#pragma unroll
+      for (int t = 0; t < TN; ++t)
+        out[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(mreg[kk], xs[0][4 * kk + lk][16 * t + lr],
+                                                      out[t], 0, 0, 0);
+  }
 #pragma unroll
   for (int t = 0; t < TN; ++t)
 #pragma unroll
