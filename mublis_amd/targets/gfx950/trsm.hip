@@ -516,10 +516,14 @@ __global__ __launch_bounds__(256) void trsm_panel(int m, int n, T alpha,
 }
 
 // Widest B solved by the pipelined kernel, at most 4096; 0 disables it.
-// Below that cap a whole solve takes the pipeline for B up to 128 columns or
-// up to m * n = 2^20; beyond, the recursive schedule's large GEMMs fill the
-// GPU better than one chain of block rows (measured crossover on MI355X,
-// FP64: n ~ 1024 at m = 1024, 256 at 4096, 128 at 8192).  The recursive
+// Below that cap a whole solve takes the pipeline for B up to 128 columns,
+// up to m * n = 2^21, or up to 768 columns and m * n = 2^23; beyond, the
+// recursive schedule's large GEMMs fill the GPU better than one chain of
+// block rows (measured on MI355X, FP64 with the folded pipeline: pipelined
+// 8192 x 512 1.54 ms vs 1.65, 6000 x 700 1.11 vs 1.48, 8192 x 200 1.27 vs
+// 2.05; recursive 8192 x 1024 2.03 ms vs 2.42, 4096 x 2048 1.04 vs 1.30,
+// 4096 x 1024 0.77 vs 0.79).  Float solves prefer the recursion's GEMM
+// leaves whenever those apply (below).  The recursive
 // schedule in turn ends in GEMM leaves (B from 256 columns for double, from
 // 64 for float, which then skips the whole-solve pipeline too; see
 // use_inv_leaves), else hands its diagonal sub-problems of at most kPipeLeaf
@@ -532,7 +536,9 @@ constexpr int kPipeLeaf = 512;
 constexpr int kPanelMinN = 2048;
 
 bool use_pipe(int m, int n) {
-  return m > NB && n <= g_pipe_max_n && (n <= 128 || (int64_t)m * n <= (int64_t(1) << 20));
+  const int64_t mn = (int64_t)m * n;
+  return m > NB && n <= g_pipe_max_n &&
+         (n <= 128 || mn <= (int64_t(1) << 21) || (n <= 768 && mn <= (int64_t(1) << 23)));
 }
 
 int pipe_cw(int n) { return n <= 32 ? 16 : (n <= 128 ? 32 : 64); }
@@ -619,7 +625,7 @@ constexpr int kInvLeaf = 1024;
 // Narrowest B for GEMM leaves.  Float solves take them whenever they
 // apply (their pipeline runs the block updates in FP64: 4096 x 256 solves in
 // 1.00 ms pipelined, 0.48 ms with GEMM leaves); double solves keep the
-// pipeline where it is at least as fast (n <= 128 or m n <= 2^20).
+// pipeline where it is at least as fast (use_pipe).
 template <typename T>
 constexpr int kInvMinN = sizeof(T) == 8 ? 256 : 64;
 // Column group of the identity solves: narrow groups keep the pipeline's

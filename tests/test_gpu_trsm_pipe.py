@@ -11,8 +11,10 @@ Bars:
   difference of 50 s eps (both solve the same well-conditioned system);
 * ragged shapes: m not a multiple of the 64-row block, n not a multiple of
   the column group (16, 32 or 64 columns), a single right-hand side; the
-  SHAPES are ones the default dispatch sends to the pipeline (n <= 128 or
-  m n <= 2^20), the wider ones go through the recursive schedule with
+  SHAPES are ones the default dispatch sends to the pipeline (n <= 128,
+  m n <= 2^21, or n <= 768 and m n <= 2^23 for double; float takes GEMM
+  leaves wherever they apply), the wider ones go through the recursive
+  schedule with
   GEMM leaves (1024-row diagonal blocks inverted by doubling the 64-row
   inverses with batched GEMMs).
 """
@@ -139,11 +141,12 @@ def test_pipelined_right_side(lib, prod, dt):
 
 
 @pytest.mark.parametrize("dt", L.DTYPES)
-@pytest.mark.parametrize("m,n", [(2000, 1000), (1100, 2100)])
+@pytest.mark.parametrize("m,n", [(2000, 1100), (1100, 2100)])
 def test_recursive_schedule_with_gemm_leaves_wide(lib, prod, dt, m, n):
     """Wide B (n >= 256): the 1024-row diagonal blocks are inverted by
     doubling the 64-row inverses with batched GEMMs and the recursion, split
-    at multiples of 1024 (2000 -> 1024 + 976; 1100 -> 1024 + 76), ends in
+    at multiples of 1024 (2000 -> 1024 + 976; 1100 -> 1024 + 76; B wide
+    enough that double solves take this schedule too), ends in
     GEMM leaves; compared with the recursive schedule down to 64-row leaves
     (pipeline disabled) and checked by the residual probe."""
     rng = np.random.default_rng(21)
@@ -181,7 +184,9 @@ def test_pipe_threshold_knob(lib):
 
 GEMM_LEAF_CASES = [
     # (m, n, order, right): the solve's order is n on the right side; the
-    # inverted blocks have 1024 rows, so the doubling's last pairs are short
+    # inverted blocks have 1024 rows, so the doubling's last pairs are short.
+    # Float takes GEMM leaves in all of these; double in the last two (B
+    # wider than 768 and m n > 2^21) and the pipeline in the others.
     (1500, 300, "F", False),  # leaves 1024 + 476
     (2600, 256, "F", False),  # 2048 + 552
     (777, 513, "C", False),   # row-major B, pipelined leaves (m < 1024)
@@ -190,6 +195,8 @@ GEMM_LEAF_CASES = [
     (128, 2100, "F", True),   # 1024 + 1024 + 52: a 52-row last block
     (1100, 64, "F", False),   # float: GEMM leaves from n = 64 (double: pipeline)
     (2050, 100, "C", False),  # float: 2048 + 2 rows
+    (2600, 900, "F", False),  # 2048 + 552
+    (1100, 2000, "C", True),  # right side, row-major: 1024 + 976
 ]
 
 
