@@ -71,21 +71,31 @@ __global__ __launch_bounds__(64) void trsm_inv_prep(int m, const T *__restrict__
   const int nbr = min(PB, m - base);
   // lanes run along A's unit stride
   const bool cm = rs_a == 1;
-  // 16 loads in flight per batch, then the LDS stores
+  // 16 loads in flight per batch, then the LDS stores.  A whole block is
+  // loaded without bounds checks (a float load under a per-element check
+  // gets its widening, and a wait, folded into the branch) and the triangle
+  // is selected at the store.
 #pragma unroll
   for (int u0 = 0; u0 < PB; u0 += 16) {
-    double v[16];
+    T v[16];
+    if (nbr == PB) {
 #pragma unroll
-    for (int d = 0; d < 16; ++d) {
-      const int u = u0 + d, r = cm ? lane : u, c = cm ? u : lane;
-      v[d] = (r < nbr && c < nbr && (LOWER ? r > c : r < c))
-                 ? (double)a[(int64_t)(base + r) * rs_a + (int64_t)(base + c) * cs_a]
-                 : 0.0;
+      for (int d = 0; d < 16; ++d) {
+        const int u = u0 + d, r = cm ? lane : u, c = cm ? u : lane;
+        v[d] = a[(int64_t)(base + r) * rs_a + (int64_t)(base + c) * cs_a];
+      }
+    } else {
+#pragma unroll
+      for (int d = 0; d < 16; ++d) {
+        const int u = u0 + d, r = cm ? lane : u, c = cm ? u : lane;
+        v[d] = (r < nbr && c < nbr) ? a[(int64_t)(base + r) * rs_a + (int64_t)(base + c) * cs_a]
+                                    : T(0);
+      }
     }
 #pragma unroll
     for (int d = 0; d < 16; ++d) {
       const int u = u0 + d, r = cm ? lane : u, c = cm ? u : lane;
-      LT[c][r] = v[d];
+      LT[c][r] = (LOWER ? r > c : r < c) ? (double)v[d] : 0.0;
     }
   }
   {
@@ -140,8 +150,13 @@ __device__ inline void wait_flag(int *f, int epoch, int *abort_flag) {
 }
 
 // Registers <-> block row `blk` x columns c0 .. c0+CW of B, along B's unit
-// stride (FP64; zero outside the matrix).  Element e = it * 256 + tid of the
-// tile is (r, cc) = (e % PB, e / PB) when rs_b == 1, else (e / CW, e % CW).
+// stride (zero outside the matrix).  Element e = it * 256 + tid of the tile
+// is (r, cc) = (e % PB, e / PB) when rs_b == 1, else (e / CW, e % CW).
+// Loads keep the operand's own type; values are widened to FP64 where they
+// are used (the LDS store, the MFMA operand).  Tiles inside the matrix load
+// without bounds checks: a float load under a per-element check has its
+// widening folded into the check's branch, together with a wait for the
+// load, which serialises the tile's loads.
 template <int CW>
 __device__ inline void tile_rc(int e, bool colmajor, int &r, int &cc) {
   if (colmajor) { r = e % PB; cc = e / PB; }
@@ -149,38 +164,53 @@ __device__ inline void tile_rc(int e, bool colmajor, int &r, int &cc) {
 }
 
 template <typename T, int CW>
-__device__ inline void load_tile(double (&v)[PB * CW / 256], const T *b, int64_t rs_b,
+__device__ inline void load_tile(T (&v)[PB * CW / 256], const T *b, int64_t rs_b,
                                  int64_t cs_b, int m, int n, int blk, int c0) {
   const bool cm = rs_b == 1;
+  if ((blk + 1) * PB <= m && c0 + CW <= n) {
+#pragma unroll
+    for (int it = 0; it < PB * CW / 256; ++it) {
+      int r, cc;
+      tile_rc<CW>(it * 256 + (int)threadIdx.x, cm, r, cc);
+      v[it] = b[(int64_t)(blk * PB + r) * rs_b + (int64_t)(c0 + cc) * cs_b];
+    }
+    return;
+  }
 #pragma unroll
   for (int it = 0; it < PB * CW / 256; ++it) {
     int r, cc;
     tile_rc<CW>(it * 256 + (int)threadIdx.x, cm, r, cc);
     const int gr = blk * PB + r, gc = c0 + cc;
-    v[it] = (gr < m && gc < n) ? (double)b[(int64_t)gr * rs_b + (int64_t)gc * cs_b] : 0.0;
+    v[it] = (gr < m && gc < n) ? b[(int64_t)gr * rs_b + (int64_t)gc * cs_b] : T(0);
   }
 }
 
-template <int CW>
-__device__ inline void tile_to_lds(const double (&v)[PB * CW / 256], double (*xs)[CW + 1],
-                                   bool cm) {
+template <int CW, typename T>
+__device__ inline void tile_to_lds(const T (&v)[PB * CW / 256], double (*xs)[CW + 1], bool cm) {
 #pragma unroll
   for (int it = 0; it < PB * CW / 256; ++it) {
     int r, cc;
     tile_rc<CW>(it * 256 + (int)threadIdx.x, cm, r, cc);
-    xs[r][cc] = v[it];
+    xs[r][cc] = (double)v[it];
   }
 }
 
-// This lane's 16 values of A(p rows, block q columns) as the MFMA A operand.
+// This lane's 16 values of A(p rows, block q columns) as the MFMA A operand
+// (widened at the MFMA); `inside`: block row p and block column q lie inside
+// the matrix (no bounds checks).
 template <typename T>
-__device__ inline void load_ablk(double (&av)[16], const T *a, int64_t rs_a, int64_t cs_a,
-                                 int m, int arow, int q, int lk) {
+__device__ inline void load_ablk(T (&av)[16], const T *a, int64_t rs_a, int64_t cs_a,
+                                 int m, int arow, int q, int lk, bool inside) {
+  if (inside) {
+#pragma unroll
+    for (int kk = 0; kk < 16; ++kk)
+      av[kk] = a[(int64_t)arow * rs_a + (int64_t)(q * PB + 4 * kk + lk) * cs_a];
+    return;
+  }
 #pragma unroll
   for (int kk = 0; kk < 16; ++kk) {
     const int col = q * PB + 4 * kk + lk;
-    av[kk] = (arow < m && col < m) ? (double)a[(int64_t)arow * rs_a + (int64_t)col * cs_a]
-                                   : 0.0;
+    av[kk] = (arow < m && col < m) ? a[(int64_t)arow * rs_a + (int64_t)col * cs_a] : T(0);
   }
 }
 
@@ -206,12 +236,13 @@ __global__ __launch_bounds__(256) void trsm_pipe(int m, int n, T alpha,
   constexpr int TN = CW / 16;          // 16-column MFMA tiles per wave
   constexpr int TV = PB * CW / 256;    // tile elements per thread
   __shared__ double xs[2][PB][CW + 1];
-  // Folding the last dependency (below) pays with 64-column groups, where
-  // the chain it takes off the critical path is as long as forming M; with
-  // narrower groups it measured 10 % slower.  A(p, q) and M use the second
-  // X buffer.
-  constexpr bool FOLD = CW == PB;
-  double(*ab)[PB + 1] = reinterpret_cast<double(*)[PB + 1]>(&xs[1][0][0]);
+  // Folding the last dependency (below) needs M = inv(A(p, p)) A(p, q),
+  // which depends on no X: it is formed when the workgroup starts, staged
+  // through the X buffers viewed as one 64 x 65 array (room for it from
+  // 32-column groups up).
+  constexpr bool FOLD = CW >= 32;
+  static_assert(!FOLD || 2 * (CW + 1) >= PB + 1, "M staging needs the X buffers");
+  double(*ab)[PB + 1] = reinterpret_cast<double(*)[PB + 1]>(&xs[0][0][0]);
   __shared__ int s_front;
   {
     const int z = blockIdx.y;
@@ -242,30 +273,65 @@ __global__ __launch_bounds__(256) void trsm_pipe(int m, int n, T alpha,
   const int c0 = cgi * CW;
   const bool cm = rs_b == 1;
   const int arow = p * PB + 16 * wave + lr;  // row of A / inv this lane feeds
+  const bool prow = (p + 1) * PB <= m;       // block row p is whole
   int *abort_flag = flags;  // flags[0]; block (j, c) publishes flags[1 + j * cg + c]
   flags += 1;
   // independent of the other block rows: this block's right-hand side and
   // inverse rows, fetched before any wait
-  double breg[TV], ireg[16];
+  T breg[TV];
+  double ireg[16];
   load_tile<T, CW>(breg, b, rs_b, cs_b, m, n, p, c0);
   {
     const double *ip = inv + (int64_t)p * PB * PB + (16 * wave + lr) * PB;
 #pragma unroll
     for (int kk = 0; kk < 16; ++kk) ireg[kk] = ip[4 * kk + lk];
   }
+  // Every dependency is accumulated as it is published -- with FOLD all but
+  // the last, X(j - 1), which is folded in after the wait as one product
+  // with M = inv(A(p, p)) A(p, q(j - 1)), formed here, before any wait: A(p,
+  // q) through LDS as the B operand, M back through LDS into the A-operand
+  // layout, negated.
+  const bool fold = FOLD && j > i0;
+  const int jend = fold ? j - 1 : j;
+  const int ql = LOWER ? j - 1 : nbk - j;  // physical block of X(j - 1)
+  double mreg[FOLD ? 16 : 1];
+  if (FOLD && fold) {
+    {
+      T at[PB * PB / 256];
+      load_tile<T, PB>(at, a, rs_a, cs_a, m, m, p, ql * PB);
+      tile_to_lds<PB>(at, ab, rs_a == 1);
+    }
+    __syncthreads();
+    d4 mm[PB / 16];
+#pragma unroll
+    for (int t = 0; t < PB / 16; ++t) mm[t] = d4{0, 0, 0, 0};
+#pragma unroll
+    for (int kk = 0; kk < 16; ++kk)
+#pragma unroll
+      for (int t = 0; t < PB / 16; ++t)
+        mm[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(ireg[kk], ab[4 * kk + lk][16 * t + lr],
+                                                     mm[t], 0, 0, 0);
+    __syncthreads();  // every wave has read A(p, ql)
+#pragma unroll
+    for (int t = 0; t < PB / 16; ++t)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) ab[16 * wave + lk + 4 * r][16 * t + lr] = mm[t][r];
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < (FOLD ? 16 : 1); ++kk) mreg[kk] = -ab[16 * wave + lr][4 * kk + lk];
+    __syncthreads();  // the X buffers are free again
+  }
   d4 acc[TN];
 #pragma unroll
   for (int t = 0; t < TN; ++t) acc[t] = d4{0, 0, 0, 0};
-  double av[16], xr[TV];
-  // Every dependency is accumulated as it is published -- with FOLD all but
-  // the last, X(j - 1), which is folded in after the wait as one product
-  // with M = inv(A(p, p)) A(p, q(j - 1)), formed beforehand.
-  const bool fold = FOLD && j > i0;
-  const int jend = fold ? j - 1 : j;
+  T av[16], xr[TV];
   int i = i0;
   while (i < jend) {
     // A(p, q(i)) before the wait; then the run of published blocks i .. f-1
-    load_ablk<T>(av, a, rs_a, cs_a, m, arow, LOWER ? i : nbk - 1 - i, lk);
+    {
+      const int q = LOWER ? i : nbk - 1 - i;
+      load_ablk<T>(av, a, rs_a, cs_a, m, arow, q, lk, prow && (q + 1) * PB <= m);
+    }
     if (tid == 0) {
       wait_flag(&flags[i * cg + cgi], epoch, abort_flag);
       int f = i + 1;
@@ -285,12 +351,12 @@ __global__ __launch_bounds__(256) void trsm_pipe(int m, int n, T alpha,
       tile_to_lds<CW>(xr, xs[buf], cm);
       double ac[16];
 #pragma unroll
-      for (int kk = 0; kk < 16; ++kk) ac[kk] = av[kk];
+      for (int kk = 0; kk < 16; ++kk) ac[kk] = (double)av[kk];
       __syncthreads();
       if (i + 1 < f) {  // next block of the run, in flight during the MFMAs
         const int q1 = LOWER ? i + 1 : nbk - 2 - i;
         load_tile<T, CW>(xr, b, rs_b, cs_b, m, n, q1, c0);
-        load_ablk<T>(av, a, rs_a, cs_a, m, arow, q1, lk);
+        load_ablk<T>(av, a, rs_a, cs_a, m, arow, q1, lk, prow && (q1 + 1) * PB <= m);
       }
 #pragma unroll
       for (int kk = 0; kk < 16; ++kk)
@@ -324,43 +390,16 @@ __global__ __launch_bounds__(256) void trsm_pipe(int m, int n, T alpha,
       out[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(ireg[kk], xs[0][4 * kk + lk][16 * t + lr],
                                                     out[t], 0, 0, 0);
   if (FOLD && fold) {
-    // M = inv(A(p, p)) A(p, ql) while X(j - 1) is still being solved: A(p,
-    // ql) through LDS as the B operand, M back through LDS into the A-operand
-    // layout, negated.  After the wait only X(p) = Y - M X(ql) remains.
-    const int ql = LOWER ? j - 1 : nbk - j;
-    {
-      double at[PB * PB / 256];
-      load_tile<T, PB>(at, a, rs_a, cs_a, m, m, p, ql * PB);
-      tile_to_lds<PB>(at, ab, rs_a == 1);
-    }
-    __syncthreads();  // A(p, ql) staged; every wave is past its Y MFMAs
-    d4 mm[PB / 16];
-#pragma unroll
-    for (int t = 0; t < PB / 16; ++t) mm[t] = d4{0, 0, 0, 0};
-#pragma unroll
-    for (int kk = 0; kk < 16; ++kk)
-#pragma unroll
-      for (int t = 0; t < PB / 16; ++t)
-        mm[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(ireg[kk], ab[4 * kk + lk][16 * t + lr],
-                                                     mm[t], 0, 0, 0);
-    __syncthreads();  // every wave has read A(p, ql)
-#pragma unroll
-    for (int t = 0; t < PB / 16; ++t)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) ab[16 * wave + lk + 4 * r][16 * t + lr] = mm[t][r];
-    __syncthreads();
-    double mreg[16];
-#pragma unroll
-    for (int kk = 0; kk < 16; ++kk) mreg[kk] = -ab[16 * wave + lr][4 * kk + lk];
+    // X(p) = Y - M X(ql): wait for X(ql), the only step after the wait
     if (tid == 0) wait_flag(&flags[(j - 1) * cg + cgi], epoch, abort_flag);
-    __syncthreads();  // also: every wave has read M (ab may be xs[1])
+    __syncthreads();  // also: every wave is past its Y MFMAs on xs[0]
     // acquire: X(ql) was published with a release store
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     load_tile<T, CW>(xr, b, rs_b, cs_b, m, n, ql, c0);
     tile_to_lds<CW>(xr, xs[0], cm);
     __syncthreads();
 #pragma unroll
-    for (int kk = 0; kk < 16; ++kk)
+    for (int kk = 0; kk < (FOLD ? 16 : 0); ++kk)
 #pragma unroll
       for (int t = 0; t < TN; ++t)
         out[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(mreg[kk], xs[0][4 * kk + lk][16 * t + lr],
@@ -405,10 +444,13 @@ __global__ __launch_bounds__(256) void trsm_inv_apply(int nb, int n, T alpha,
   const double *ip = inv + (16 * wave + lr) * PB;
 #pragma unroll
   for (int kk = 0; kk < 16; ++kk) ireg[kk] = ip[4 * kk + lk];
-  load_tile<T, CW>(v, b, rs_b, cs_b, nb, n, 0, c0);
-  const double al = (double)alpha;
+  {
+    T raw[TV];
+    load_tile<T, CW>(raw, b, rs_b, cs_b, nb, n, 0, c0);
+    const double al = (double)alpha;
 #pragma unroll
-  for (int it = 0; it < TV; ++it) v[it] *= al;
+    for (int it = 0; it < TV; ++it) v[it] = al * (double)raw[it];
+  }
   tile_to_lds<CW>(v, xs, cm);
   __syncthreads();
   d4 out[TN];
@@ -463,7 +505,8 @@ __global__ __launch_bounds__(256) void trsm_panel(int m, int n, T alpha,
   for (int j = 0; j < nbk; ++j) {
     const int p = LOWER ? j : nbk - 1 - j;
     const int arow = p * PB + 16 * wave + lr;
-    double breg[TV], ireg[16];
+    T breg[TV];
+    double ireg[16];
     load_tile<T, PW>(breg, b, rs_b, cs_b, m, n, p, c0);
     {
       const double *ip = inv + (int64_t)p * PB * PB + (16 * wave + lr) * PB;
@@ -471,15 +514,22 @@ __global__ __launch_bounds__(256) void trsm_panel(int m, int n, T alpha,
       for (int kk = 0; kk < 16; ++kk) ireg[kk] = ip[4 * kk + lk];
     }
     d4 acc = d4{0, 0, 0, 0};
-    double av[16], an[16];
-    if (j > 0) load_ablk<T>(av, a, rs_a, cs_a, m, arow, LOWER ? 0 : nbk - 1, lk);
+    T av[16], an[16];
+    const bool prow = (p + 1) * PB <= m;
+    if (j > 0) {
+      const int q = LOWER ? 0 : nbk - 1;
+      load_ablk<T>(av, a, rs_a, cs_a, m, arow, q, lk, prow && (q + 1) * PB <= m);
+    }
     for (int i = 0; i < j; ++i) {
       const int q = LOWER ? i : nbk - 1 - i;
-      if (i + 1 < j) load_ablk<T>(an, a, rs_a, cs_a, m, arow, LOWER ? i + 1 : nbk - 2 - i, lk);
+      if (i + 1 < j) {
+        const int q1 = LOWER ? i + 1 : nbk - 2 - i;
+        load_ablk<T>(an, a, rs_a, cs_a, m, arow, q1, lk, prow && (q1 + 1) * PB <= m);
+      }
 #pragma unroll
       for (int kk = 0; kk < 16; ++kk)
-        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av[kk], xs[q * PB + 4 * kk + lk][lr], acc,
-                                                   0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f64_16x16x4f64((double)av[kk], xs[q * PB + 4 * kk + lk][lr],
+                                                   acc, 0, 0, 0);
 #pragma unroll
       for (int kk = 0; kk < 16; ++kk) av[kk] = an[kk];
     }
@@ -488,7 +538,7 @@ __global__ __launch_bounds__(256) void trsm_panel(int m, int n, T alpha,
     for (int it = 0; it < TV; ++it) {
       int r, cc;
       tile_rc<PW>(it * 256 + tid, cm, r, cc);
-      xs[p * PB + r][cc] = al * breg[it];
+      xs[p * PB + r][cc] = al * (double)breg[it];
     }
     __syncthreads();
 #pragma unroll
@@ -522,11 +572,11 @@ __global__ __launch_bounds__(256) void trsm_panel(int m, int n, T alpha,
 // block rows (measured on MI355X, FP64 with the folded pipeline: pipelined
 // 8192 x 512 1.54 ms vs 1.65, 6000 x 700 1.11 vs 1.48, 8192 x 200 1.27 vs
 // 2.05; recursive 8192 x 1024 2.03 ms vs 2.42, 4096 x 2048 1.04 vs 1.30,
-// 4096 x 1024 0.77 vs 0.79).  Float solves prefer the recursion's GEMM
-// leaves whenever those apply (below).  The recursive
+// 4096 x 1024 0.77 vs 0.79).  Float solves wider than 128 columns prefer
+// the recursion's GEMM leaves whenever those apply (float 8192 x 256: 0.93
+// ms vs 1.09 pipelined; 8192 x 128: pipelined 0.82 vs 0.93).  The recursive
 // schedule in turn ends in GEMM leaves (B from 256 columns for double, from
-// 64 for float, which then skips the whole-solve pipeline too; see
-// use_inv_leaves), else hands its diagonal sub-problems of at most kPipeLeaf
+// 64 for float; see use_inv_leaves), else hands its diagonal sub-problems of at most kPipeLeaf
 // rows to the pipeline (narrower B) or to column-panel workgroups (wider B,
 // when the solve brings no room for the GEMM leaves' inverses).
 int g_pipe_max_n = 4096;
@@ -576,13 +626,21 @@ int leaf(int nb, int n, T alpha, const double *inv, T *b, int64_t rs_b, int64_t 
   return hipGetLastError() == hipSuccess ? MUBLIS_OK : MUBLIS_EHIP;
 }
 
+constexpr size_t kLdsOneWg = 96 * 1024;
+
 template <typename T, bool LOWER, int CW>
 int pipe_launch(int m, int n, T alpha, const T *a, int64_t rs_a, int64_t cs_a, T *b,
                 int64_t rs_b, int64_t cs_b, const double *inv, int *flags, int epoch,
                 const PipeBatch &bt, hipStream_t s) {
   const int nbk = (bt.m_blk + PB - 1) / PB, cg = (n + CW - 1) / CW;
   const int nz = (m + bt.m_blk - 1) / bt.m_blk;
-  hipLaunchKernelGGL((trsm_pipe<T, LOWER, CW>), dim3(nbk * cg, nz), dim3(256), 0, s, m, n,
+  // A single solve holds one workgroup per CU: a second workgroup of the
+  // same chain on a CU slowed it (float 8192 x 128, 32-column groups: 1.17
+  // ms at two per CU, 0.82 ms at one).  Dynamic LDS past half of the CU's
+  // 160 KB keeps the second one off.  Batches of independent solves keep
+  // the kernel's own occupancy.
+  const size_t lds = nz == 1 ? kLdsOneWg - sizeof(double) * 2 * PB * (CW + 1) : 0;
+  hipLaunchKernelGGL((trsm_pipe<T, LOWER, CW>), dim3(nbk * cg, nz), dim3(256), lds, s, m, n,
                      alpha, a, rs_a, cs_a, b, rs_b, cs_b, inv, flags, cg, epoch, bt);
   return hipGetLastError() == hipSuccess ? MUBLIS_OK : MUBLIS_EHIP;
 }
@@ -845,7 +903,7 @@ int solve_ws(bool lower, int diag_mode, int m, int n, T alpha, const T *a, int64
   const int nbk = (m + NB - 1) / NB;
   const bool piped = m > NB && n <= g_pipe_max_n && flags_fit(m, n);
   const bool gemm_ok = iv && w && use_inv_leaves<T>(m, n);
-  const bool whole = piped && use_pipe(m, n) && !(sizeof(T) == 4 && gemm_ok);
+  const bool whole = piped && use_pipe(m, n) && !(sizeof(T) == 4 && gemm_ok && n > 128);
   const bool gemm_leaves = !whole && gemm_ok;
   int *flags = (piped || gemm_leaves) ? reinterpret_cast<int *>(ad + (int64_t)nbk * NB * NB)
                                       : nullptr;

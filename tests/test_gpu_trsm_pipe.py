@@ -12,11 +12,12 @@ Bars:
 * ragged shapes: m not a multiple of the 64-row block, n not a multiple of
   the column group (16, 32 or 64 columns), a single right-hand side; the
   SHAPES are ones the default dispatch sends to the pipeline (n <= 128,
-  m n <= 2^21, or n <= 768 and m n <= 2^23 for double; float takes GEMM
-  leaves wherever they apply), the wider ones go through the recursive
-  schedule with
-  GEMM leaves (1024-row diagonal blocks inverted by doubling the 64-row
-  inverses with batched GEMMs).
+  m n <= 2^21, or n <= 768 and m n <= 2^23 for double; float past 128
+  columns takes GEMM leaves wherever they apply, and 2100 x 100 / 1100 x 72
+  are float solves that the pipeline takes from the GEMM leaves), the wider
+  ones go through the recursive schedule with GEMM leaves (1024-row
+  diagonal blocks inverted by doubling the 64-row inverses with batched
+  GEMMs).
 """
 import ctypes
 import itertools
@@ -84,7 +85,8 @@ def residual(a, x, b0, alpha, right, lower, trans, unit, seed=0):
     return res, lim
 
 
-SHAPES = [(65, 1), (200, 16), (257, 17), (1000, 33), (640, 128), (1536, 200), (3000, 300)]
+SHAPES = [(65, 1), (200, 16), (257, 17), (1000, 33), (640, 128), (1100, 72), (2100, 100),
+          (1536, 200), (3000, 300)]
 
 
 @pytest.mark.parametrize("dt", L.DTYPES)
