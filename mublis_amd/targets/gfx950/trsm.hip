@@ -226,6 +226,12 @@ struct PipeBatch {
   int eye;
 };
 
+// LDS of a pipeline workgroup: two 64 x (CW + 1) X buffers, at least one
+// 64 x 65 array (the folded product's staging).
+constexpr int pipe_lds_doubles(int cw) {
+  return 2 * PB * (cw + 1) > PB * (PB + 1) ? 2 * PB * (cw + 1) : PB * (PB + 1);
+}
+
 template <typename T, bool LOWER, int CW>
 __global__ __launch_bounds__(256) void trsm_pipe(int m, int n, T alpha,
                                                  const T *__restrict__ a, int64_t rs_a,
@@ -235,14 +241,13 @@ __global__ __launch_bounds__(256) void trsm_pipe(int m, int n, T alpha,
   typedef double d4 __attribute__((ext_vector_type(4)));
   constexpr int TN = CW / 16;          // 16-column MFMA tiles per wave
   constexpr int TV = PB * CW / 256;    // tile elements per thread
-  __shared__ double xs[2][PB][CW + 1];
   // Folding the last dependency (below) needs M = inv(A(p, p)) A(p, q),
   // which depends on no X: it is formed when the workgroup starts, staged
-  // through the X buffers viewed as one 64 x 65 array (room for it from
-  // 32-column groups up).
-  constexpr bool FOLD = CW >= 32;
-  static_assert(!FOLD || 2 * (CW + 1) >= PB + 1, "M staging needs the X buffers");
-  double(*ab)[PB + 1] = reinterpret_cast<double(*)[PB + 1]>(&xs[0][0][0]);
+  // through the LDS of the two X buffers viewed as one 64 x 65 array (sized
+  // for it with 16-column groups).
+  __shared__ double lds[pipe_lds_doubles(CW)];
+  double(*xs)[PB][CW + 1] = reinterpret_cast<double(*)[PB][CW + 1]>(lds);
+  double(*ab)[PB + 1] = reinterpret_cast<double(*)[PB + 1]>(lds);
   __shared__ int s_front;
   {
     const int z = blockIdx.y;
@@ -286,16 +291,16 @@ __global__ __launch_bounds__(256) void trsm_pipe(int m, int n, T alpha,
 #pragma unroll
     for (int kk = 0; kk < 16; ++kk) ireg[kk] = ip[4 * kk + lk];
   }
-  // Every dependency is accumulated as it is published -- with FOLD all but
-  // the last, X(j - 1), which is folded in after the wait as one product
+  // Every dependency is accumulated as it is published -- all but the last,
+  // X(j - 1), which is folded in after the wait as one product
   // with M = inv(A(p, p)) A(p, q(j - 1)), formed here, before any wait: A(p,
   // q) through LDS as the B operand, M back through LDS into the A-operand
   // layout, negated.
-  const bool fold = FOLD && j > i0;
+  const bool fold = j > i0;
   const int jend = fold ? j - 1 : j;
   const int ql = LOWER ? j - 1 : nbk - j;  // physical block of X(j - 1)
-  double mreg[FOLD ? 16 : 1];
-  if (FOLD && fold) {
+  double mreg[16];
+  if (fold) {
     {
       T at[PB * PB / 256];
       load_tile<T, PB>(at, a, rs_a, cs_a, m, m, p, ql * PB);
@@ -318,7 +323,7 @@ __global__ __launch_bounds__(256) void trsm_pipe(int m, int n, T alpha,
       for (int r = 0; r < 4; ++r) ab[16 * wave + lk + 4 * r][16 * t + lr] = mm[t][r];
     __syncthreads();
 #pragma unroll
-    for (int kk = 0; kk < (FOLD ? 16 : 1); ++kk) mreg[kk] = -ab[16 * wave + lr][4 * kk + lk];
+    for (int kk = 0; kk < 16; ++kk) mreg[kk] = -ab[16 * wave + lr][4 * kk + lk];
     __syncthreads();  // the X buffers are free again
   }
   d4 acc[TN];
@@ -389,7 +394,7 @@ __global__ __launch_bounds__(256) void trsm_pipe(int m, int n, T alpha,
     for (int t = 0; t < TN; ++t)
       out[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(ireg[kk], xs[0][4 * kk + lk][16 * t + lr],
                                                     out[t], 0, 0, 0);
-  if (FOLD && fold) {
+  if (fold) {
     // X(p) = Y - M X(ql): wait for X(ql), the only step after the wait
     if (tid == 0) wait_flag(&flags[(j - 1) * cg + cgi], epoch, abort_flag);
     __syncthreads();  // also: every wave is past its Y MFMAs on xs[0]
@@ -399,7 +404,7 @@ __global__ __launch_bounds__(256) void trsm_pipe(int m, int n, T alpha,
     tile_to_lds<CW>(xr, xs[0], cm);
     __syncthreads();
 #pragma unroll
-    for (int kk = 0; kk < (FOLD ? 16 : 0); ++kk)
+    for (int kk = 0; kk < 16; ++kk)
 #pragma unroll
       for (int t = 0; t < TN; ++t)
         out[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(mreg[kk], xs[0][4 * kk + lk][16 * t + lr],
@@ -639,7 +644,7 @@ int pipe_launch(int m, int n, T alpha, const T *a, int64_t rs_a, int64_t cs_a, T
   // ms at two per CU, 0.82 ms at one).  Dynamic LDS past half of the CU's
   // 160 KB keeps the second one off.  Batches of independent solves keep
   // the kernel's own occupancy.
-  const size_t lds = nz == 1 ? kLdsOneWg - sizeof(double) * 2 * PB * (CW + 1) : 0;
+  const size_t lds = nz == 1 ? kLdsOneWg - sizeof(double) * pipe_lds_doubles(CW) : 0;
   hipLaunchKernelGGL((trsm_pipe<T, LOWER, CW>), dim3(nbk * cg, nz), dim3(256), lds, s, m, n,
                      alpha, a, rs_a, cs_a, b, rs_b, cs_b, inv, flags, cg, epoch, bt);
   return hipGetLastError() == hipSuccess ? MUBLIS_OK : MUBLIS_EHIP;
