@@ -30,7 +30,7 @@ sys.path.insert(0, ROOT)
 PEAK = {"f64": 78.6, "f32": 157.3}  # dense MFMA TFLOP/s, MI355X
 METRIC = ("TFLOP/s & %MFMA peak, D/SGEMM+DTRSM n=8192; p50/p99 mixed-L3 request "
           "latency")
-COL, ROW, NOTR, TR, UP, LO, NONUNIT, LEFT = 102, 101, 111, 112, 121, 122, 131, 141
+COL, ROW, NOTR, TR, UP, LO, NONUNIT, LEFT, RIGHT = 102, 101, 111, 112, 121, 122, 131, 141, 142
 SERVE_RATE = 16000.0  # requests/s of the serving workload's Poisson arrivals
 
 
@@ -56,7 +56,7 @@ def parse():
     p.add_argument("--cpu-slice", type=int, default=2048,
                    help="M = N of the CPU-baseline DGEMM slice (K stays 8192)")
     p.add_argument("--only", choices=["dgemm", "dtrsm", "dsyrk", "sgemm", "sgemm_bf16x3",
-                                      "batched", "serving"],
+                                      "batched", "batched_trsm", "serving"],
                    help="profiling aid: run just this operation (3 calls) and exit")
     p.add_argument("--traffic-probe", action="store_true",
                    help="profiling aid: one 512 MiB device copy (counter "
@@ -156,9 +156,9 @@ def profile_mode(args, torch, M, lib, dev, g):
             Bt = B0.clone()
             lib.cblas_dtrsm(COL, LEFT, LO, NOTR, NONUNIT, m, nrhs, 1.0, M.ptr(At), m,
                             M.ptr(Bt), m)
-    elif what == "batched":
-        from mublis_amd.workloads import grouped_problems
-        ops, keep, specs = grouped_problems(torch)
+    elif what in ("batched", "batched_trsm"):
+        from mublis_amd.workloads import grouped_problems, grouped_trsm_problems
+        ops, keep, specs = (grouped_problems if what == "batched" else grouped_trsm_problems)(torch)
         plan = M.GroupedPlan(ops)
         for _ in range(calls):
             plan.execute(lib.mublis_rt_stream())
@@ -309,6 +309,39 @@ def run_batched(args, torch, M, lib, extra):
         "bound": "mfma" if info["flops"] / (PEAK["f32"] * 1e12) >= info["bytes"] / 8e12
                  else "hbm",
         "work_items": info["items"], "problems_per_tile_shape_32_64_128": info["tiles"],
+        "serial_cblas_calls_ms": round(w_s, 3),
+        "speedup_vs_serial_calls": round(w_s / ev_g, 2),
+    }
+    run_batched_trsm(torch, M, lib, extra)
+
+
+def run_batched_trsm(torch, M, lib, extra):
+    """Batched STRSM: 1024 problems with m, n log-uniform [16, 512], side,
+    uplo and transpose random, diagonally dominant A, in one grouped plan
+    (FP64 inverses of every diagonal block in one launch, then 16-column
+    panels) replayed from its hipGraph; against the same problems as serial
+    cblas_strsm calls.  Each replay solves in place again."""
+    from mublis_amd.workloads import grouped_trsm_problems
+    ops, keep, specs = grouped_trsm_problems(torch)
+    plan = M.GroupedPlan(ops)
+    info = plan.info()
+    stream = lib.mublis_rt_stream()
+
+    def replay():
+        plan.execute(stream)
+    _, ev_g = timed_calls(torch, lib, replay, 10, 2)
+    plan.close()
+
+    def serial():
+        for s in specs:
+            lib.cblas_strsm(COL, RIGHT if s["side"] else LEFT, UP if s["upper"] else LO,
+                            TR if s["trans"] else NOTR, NONUNIT, s["m"], s["n"], 1.0,
+                            M.ptr(s["A"]), s["order"], M.ptr(s["B"]), s["m"])
+    w_s, _ = timed_calls(torch, lib, serial, 1, 0)
+    extra["batched_1024strsm"] = {
+        "tflops": round(info["flops"] / (ev_g * 1e-3) / 1e12, 3),
+        "ms_per_launch": round(ev_g, 4),
+        "flops": info["flops"], "work_items": info["items"],
         "serial_cblas_calls_ms": round(w_s, 3),
         "speedup_vs_serial_calls": round(w_s / ev_g, 2),
     }

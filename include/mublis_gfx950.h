@@ -216,20 +216,31 @@ int mublis_get_sgemm_mode(void);
 
 /* ---------------------------------------------- batched extensions ---- */
 /*
- * Grouped batched FP32 GEMM / SYRK: many uneven problems in ONE persistent
- * launch, captured once in a hipGraph and replayed.  No counterpart in the
- * reference (which is one call per problem, frame/l3/l3_driver.c:49-249);
- * each problem's result is bit-identical to the corresponding cblas_sgemm
- * or cblas_ssyrk call on the same device operands.
+ * Grouped batched FP32 GEMM / SYRK / SYR2K / TRSM: many uneven problems in
+ * one set of persistent launches, captured once in a hipGraph and replayed.
+ * No counterpart in the reference (which is one call per problem,
+ * frame/l3/l3_driver.c:49-249); each GEMM/SYRK/SYR2K problem's result is
+ * bit-identical to the corresponding cblas_sgemm / cblas_ssyrk /
+ * cblas_ssyr2k call on the same device operands.
  *
  * Operands are column-major DEVICE pointers.  GEMM: C := alpha*op(A)*op(B)
  * + beta*C with op from trans_a / trans_b (0 or 1), C m x n.  SYRK:
  * C := alpha*op(A)*op(A)' + beta*C on the `uplo` triangle (MUBLIS_LOWER /
  * MUBLIS_UPPER) of the n x n C, op(A) n x k (trans_a), b unused.  SYR2K:
  * C := alpha*(op(A)*op(B)' + op(B)*op(A)') + beta*C on the `uplo` triangle,
- * op(A) and op(B) n x k (trans_a applies to both).
+ * op(A) and op(B) n x k (trans_a applies to both).  TRSM: the m x n B at
+ * c (ldc) is overwritten by alpha*inv(op(A))*B (side MUBLIS_LEFT, A m x m)
+ * or alpha*B*inv(op(A)) (MUBLIS_RIGHT, A n x n), A triangular (`uplo`,
+ * trans_a, `diag` MUBLIS_UNIT_DIAG or MUBLIS_NON_UNIT_DIAG); b, k, beta
+ * unused.  The 64-row diagonal blocks are inverted in FP64 and the block
+ * updates run on FP64 MFMA, as in cblas_strsm; problems whose triangle has
+ * order up to 1024 run as 16-column panels inside the grouped launches,
+ * larger ones (and alpha = 0) as their own solve in the same plan or
+ * launch.  Problems of one batch must not write what another reads or
+ * writes.  side and diag are ignored by the other ops (zero them).
  */
-enum { MUBLIS_BATCH_GEMM = 0, MUBLIS_BATCH_SYRK = 1, MUBLIS_BATCH_SYR2K = 2 };
+enum { MUBLIS_BATCH_GEMM = 0, MUBLIS_BATCH_SYRK = 1, MUBLIS_BATCH_SYR2K = 2,
+       MUBLIS_BATCH_TRSM = 3 };
 typedef struct {
   int op;
   int trans_a, trans_b;
@@ -242,6 +253,7 @@ typedef struct {
   int ldb;
   float *c;
   int ldc;
+  int side, diag; /* TRSM only */
 } mublis_sbatch_op_t;
 typedef struct mublis_grouped_plan_s *mublis_grouped_plan_t;
 /* flags for plan creation: force one tile shape (0: 32, 1: 64, 2: 128) */

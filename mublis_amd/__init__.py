@@ -26,14 +26,16 @@ OK, ECONTEXT, ENOMEM, EINVAL, EHIP = 0, 1, 2, 3, 4
 _lib = None
 
 # include/mublis_gfx950.h batched extensions
-BATCH_GEMM, BATCH_SYRK, BATCH_SYR2K = 0, 1, 2
+BATCH_GEMM, BATCH_SYRK, BATCH_SYR2K, BATCH_TRSM = 0, 1, 2, 3
 
 
 class SbatchOp(ctypes.Structure):
-    """mublis_sbatch_op_t: one FP32 GEMM, SYRK or SYR2K problem on device memory."""
+    """mublis_sbatch_op_t: one FP32 GEMM, SYRK, SYR2K or TRSM problem on device
+    memory (TRSM: B at c is solved in place; side and diag are its own)."""
     _fields_ = [("op", _I), ("trans_a", _I), ("trans_b", _I), ("uplo", _I),
                 ("m", _I), ("n", _I), ("k", _I), ("alpha", _F), ("beta", _F),
-                ("a", _VP), ("lda", _I), ("b", _VP), ("ldb", _I), ("c", _VP), ("ldc", _I)]
+                ("a", _VP), ("lda", _I), ("b", _VP), ("ldb", _I), ("c", _VP), ("ldc", _I),
+                ("side", _I), ("diag", _I)]
 
 
 def build(jobs=8):

@@ -1,4 +1,5 @@
-// Grouped persistent batched L3 kernel for gfx950 (FP32 GEMM and SYRK).
+// Grouped persistent batched L3 kernel for gfx950 (FP32 GEMM, SYRK, SYR2K
+// and TRSM).
 //
 // Thousands of uneven problems run as ONE launch: the host expands every
 // problem into output tiles (SYRK: only the tiles that touch its ji
@@ -9,7 +10,9 @@
 // (gemm_tile.h), so a problem's result is bit-identical
 // to what cblas_sgemm / cblas_ssyrk compute for it.  The 128-tile items and
 // the smaller ones are two launches (different LDS and occupancy), captured
-// once in a hipGraph and replayed.
+// once in a hipGraph and replayed.  TRSM problems run as independent
+// 16-column panels after one launch that inverts all their diagonal blocks
+// (trsm.hip, gfx950_trsm_grouped), in the same graph.
 //
 // The reference has no batched interface; this is the batched addition on
 // top of its per-call driver (frame/l3/l3_driver.c:49-249).
@@ -142,6 +145,9 @@ struct mublis_grouped_plan_s {
   GemmArgs<float> *d_probs = nullptr;
   GroupItem *d_items = nullptr;
   int nitems = 0, ncls[3] = {0, 0, 0};
+  void *d_trsm = nullptr;  // TRSM descriptors, work lists, inverses, solo workspace
+  hipStream_t side = nullptr;  // second branch of the captured TRSM launches
+  hipEvent_t fork = nullptr, join = nullptr;
   hipStream_t cap = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -163,22 +169,97 @@ int mublis_grouped_load_table(const char *path) {
 
 namespace {
 
+// A TRSM problem solved by itself (triangle of order above kGroupTrsmMax,
+// or alpha = 0), in the left form, with its workspace offset.
+struct SoloTrsm {
+  int lower, unit, m, n;
+  float alpha;
+  const float *a;
+  int64_t rs_a, cs_a;
+  float *b;
+  int64_t rs_b, cs_b;
+  size_t ws_off, ws_bytes;
+};
+
 struct Work {
   std::vector<GemmArgs<float>> probs;
   std::vector<GroupItem> items;  // by tile shape (128, 64, 32), each by cost
   int ncls[3] = {0, 0, 0};       // items per tile shape (index 0: 32, 1: 64, 2: 128)
   double flops = 0, bytes = 0;
   int hist[3] = {0, 0, 0};
+  // TRSM: descriptors, the six work lists of gfx950_trsm_grouped back to
+  // back, FP64 inverse workspace, and the problems solved by themselves
+  std::vector<TrsmGroupProb<float>> tprobs;
+  std::vector<TrsmGroupItem> titems;
+  int tcounts[6] = {0, 0, 0, 0, 0, 0};
+  int64_t inv_doubles = 0;
+  std::vector<SoloTrsm> solo;
+  size_t solo_bytes = 0;
 };
+
+size_t align256(size_t x) { return (x + 255) / 256 * 256; }
+
+// Left form of a TRSM op as the frame maps it (frame/l3/trsm.c:35-54):
+// Right side solves the transposed system (B's strides and dimensions
+// swapped, op(A) transposed); a transposed A is a stride swap that turns
+// the triangle over.
+int add_trsm(const mublis_sbatch_op_t &o, Work &w,
+             std::vector<std::vector<TrsmGroupItem>> &lists) {
+  const bool left = o.side == MUBLIS_LEFT;
+  if (o.m < 0 || o.n < 0 || (o.side != MUBLIS_LEFT && o.side != MUBLIS_RIGHT) ||
+      (o.uplo != MUBLIS_LOWER && o.uplo != MUBLIS_UPPER) || (o.trans_a & ~1) ||
+      (o.diag & ~1))
+    return MUBLIS_EINVAL;
+  const int order = left ? o.m : o.n;
+  if (o.ldc < std::max(1, o.m) || o.lda < std::max(1, order)) return MUBLIS_EINVAL;
+  int64_t rsa = 1, csa = o.lda, rsb = 1, csb = o.ldc;
+  int mm = o.m, nn = o.n;
+  bool trans = o.trans_a != 0, lower = o.uplo == MUBLIS_LOWER;
+  if (!left) {
+    std::swap(mm, nn);
+    std::swap(rsb, csb);
+    trans = !trans;
+  }
+  if (trans) {
+    std::swap(rsa, csa);
+    lower = !lower;
+  }
+  w.flops += (double)mm * mm * nn;
+  w.bytes += 4.0 * ((double)mm * (mm + 1) / 2 + 2.0 * mm * nn);
+  if (mm == 0 || nn == 0) return MUBLIS_OK;
+  if (mm > kGroupTrsmMax || o.alpha == 0.0f) {
+    const size_t ws = o.alpha == 0.0f ? 0 : align256(mublis_gfx950_trsm_ws_bytes(mm));
+    w.solo.push_back({lower, o.diag, mm, nn, o.alpha, o.a, rsa, csa, o.c, rsb, csb,
+                      w.solo_bytes, ws});
+    w.solo_bytes += ws;
+    return MUBLIS_OK;
+  }
+  const int pid = (int)w.tprobs.size();
+  w.tprobs.push_back({lower, o.diag, mm, nn, o.alpha, o.a, rsa, csa, o.c, rsb, csb,
+                      w.inv_doubles});
+  const int nbk = (mm + 63) / 64;
+  w.inv_doubles += (int64_t)nbk * 64 * 64;
+  for (int p = 0; p < nbk; ++p) lists[lower ? 0 : 1].push_back({pid, p});
+  const int cls = (mm <= kGroupTrsmSmall ? 2 : 4) + (lower ? 0 : 1);
+  for (int c = 0; c < (nn + kGroupTrsmCols - 1) / kGroupTrsmCols; ++c)
+    lists[cls].push_back({pid, c});
+  return MUBLIS_OK;
+}
 
 // Expand ops into problems and a cost-sorted tile list.
 int build_work(const mublis_sbatch_op_t *ops, int count, int flags, Work &w) {
   std::vector<GroupItem> items;
   std::vector<double> cost;
+  std::vector<std::vector<TrsmGroupItem>> tlists(6);
   w.probs.reserve(count);
   const int forced = (flags >> 4) & 3;  // MUBLIS_GROUPED_FORCE_TILE(t) stores t + 1
   for (int i = 0; i < count; ++i) {
     const mublis_sbatch_op_t &o = ops[i];
+    if (o.op == MUBLIS_BATCH_TRSM) {
+      const int st = add_trsm(o, w, tlists);
+      if (st) return st;
+      continue;
+    }
     const bool syrk = o.op == MUBLIS_BATCH_SYRK, syr2k = o.op == MUBLIS_BATCH_SYR2K;
     const bool sym = syrk || syr2k;
     const int m = sym ? o.n : o.m, n = o.n, k = o.k;
@@ -288,7 +369,58 @@ int build_work(const mublis_sbatch_op_t *ops, int count, int flags, Work &w) {
       --left;
     }
   }
+  // TRSM panels run longest first (a panel's work grows with the square of
+  // its problem's order)
+  for (int c = 2; c < 6; ++c)
+    std::stable_sort(tlists[c].begin(), tlists[c].end(),
+                     [&](const TrsmGroupItem &x, const TrsmGroupItem &y) {
+                       return w.tprobs[x.prob].m > w.tprobs[y.prob].m;
+                     });
+  for (int c = 0; c < 6; ++c) {
+    w.tcounts[c] = (int)tlists[c].size();
+    w.titems.insert(w.titems.end(), tlists[c].begin(), tlists[c].end());
+  }
   return MUBLIS_OK;
+}
+
+// Device bytes of a work list's TRSM part: descriptors, items, inverses,
+// solo workspace (each 256-byte aligned, in that order).
+size_t trsm_bytes(const Work &w) {
+  return align256(sizeof(TrsmGroupProb<float>) * w.tprobs.size()) +
+         align256(sizeof(TrsmGroupItem) * w.titems.size()) +
+         align256(sizeof(double) * (size_t)w.inv_doubles) + w.solo_bytes;
+}
+// Bytes of the TRSM part that come from the host (descriptors and items).
+size_t trsm_host_bytes(const Work &w) {
+  return align256(sizeof(TrsmGroupProb<float>) * w.tprobs.size()) +
+         align256(sizeof(TrsmGroupItem) * w.titems.size());
+}
+void trsm_stage(const Work &w, char *h) {
+  memcpy(h, w.tprobs.data(), sizeof(TrsmGroupProb<float>) * w.tprobs.size());
+  memcpy(h + align256(sizeof(TrsmGroupProb<float>) * w.tprobs.size()), w.titems.data(),
+         sizeof(TrsmGroupItem) * w.titems.size());
+}
+// The TRSM launches of a work list whose TRSM part sits at dev (laid out as
+// trsm_bytes describes), on s (upper-triangle launches on s2 between the
+// fork and join events, when given).
+int launch_trsm(const Work &w, char *dev, hipStream_t s, hipStream_t s2 = nullptr,
+                hipEvent_t fork = nullptr, hipEvent_t join = nullptr) {
+  const size_t pb = align256(sizeof(TrsmGroupProb<float>) * w.tprobs.size());
+  const size_t ib = align256(sizeof(TrsmGroupItem) * w.titems.size());
+  const size_t vb = align256(sizeof(double) * (size_t)w.inv_doubles);
+  int st = MUBLIS_OK;
+  if (!w.titems.empty())
+    st = gfx950_trsm_grouped<float>((const TrsmGroupProb<float> *)dev,
+                                    (const TrsmGroupItem *)(dev + pb), w.tcounts,
+                                    (double *)(dev + pb + ib), s, s2, fork, join);
+  char *solo_ws = dev + pb + ib + vb;
+  for (size_t i = 0; i < w.solo.size() && !st; ++i) {
+    const SoloTrsm &t = w.solo[i];
+    st = mublis_gfx950_strsm_ws_dev(t.lower, t.unit, t.m, t.n, t.alpha, t.a, t.rs_a, t.cs_a,
+                                    t.b, t.rs_b, t.cs_b, t.ws_bytes ? solo_ws + t.ws_off : nullptr,
+                                    t.ws_bytes, s);
+  }
+  return st;
 }
 
 // Persistent grid: per_cu workgroups per CU, at most one per item, rounded
@@ -322,8 +454,6 @@ int launch_items(const GemmArgs<float> *probs, const GroupItem *items, const int
   return hipGetLastError() == hipSuccess ? MUBLIS_OK : MUBLIS_EHIP;
 }
 
-size_t align256(size_t x) { return (x + 255) / 256 * 256; }
-
 }  // namespace
 
 extern "C" {
@@ -341,7 +471,8 @@ int mublis_sgrouped_plan_create(const mublis_sbatch_op_t *ops, int count, int fl
   for (int i = 0; i < 3; ++i) plan->tile_hist[i] = w.hist[i];
   const std::vector<GemmArgs<float>> &probs = w.probs;
   const std::vector<GroupItem> &sorted = w.items;
-  plan->nitems = (int)sorted.size();
+  plan->nitems = (int)sorted.size() + w.tcounts[2] + w.tcounts[3] + w.tcounts[4] +
+                 w.tcounts[5] + (int)w.solo.size();
   for (int c = 0; c < 3; ++c) plan->ncls[c] = w.ncls[c];
   st = MUBLIS_OK;
   if (!probs.empty() && hipMalloc(&plan->d_probs, sizeof(GemmArgs<float>) * probs.size()))
@@ -356,13 +487,28 @@ int mublis_sgrouped_plan_create(const mublis_sbatch_op_t *ops, int count, int fl
       hipMemcpy(plan->d_items, sorted.data(), sizeof(GroupItem) * sorted.size(),
                 hipMemcpyHostToDevice))
     st = MUBLIS_EHIP;
+  const size_t tb = trsm_bytes(w);
+  if (!st && tb > 0 && hipMalloc(&plan->d_trsm, tb)) st = MUBLIS_ENOMEM;
+  if (!st && trsm_host_bytes(w) > 0) {
+    std::vector<char> h(trsm_host_bytes(w));
+    trsm_stage(w, h.data());
+    if (hipMemcpy(plan->d_trsm, h.data(), h.size(), hipMemcpyHostToDevice)) st = MUBLIS_EHIP;
+  }
   // Capture the launches once; executions replay the graph.
   if (!st && hipStreamCreateWithFlags(&plan->cap, hipStreamNonBlocking)) st = MUBLIS_EHIP;
+  if (!st && (w.tcounts[1] + w.tcounts[3] + w.tcounts[5]) > 0 &&
+      (w.tcounts[0] + w.tcounts[2] + w.tcounts[4]) > 0 &&
+      (hipStreamCreateWithFlags(&plan->side, hipStreamNonBlocking) ||
+       hipEventCreateWithFlags(&plan->fork, hipEventDisableTiming) ||
+       hipEventCreateWithFlags(&plan->join, hipEventDisableTiming)))
+    st = MUBLIS_EHIP;
   if (!st && hipStreamBeginCapture(plan->cap, hipStreamCaptureModeThreadLocal)) st = MUBLIS_EHIP;
   if (!st) {
-    const int l = launch_items(plan->d_probs, plan->d_items, plan->ncls, plan->cap);
+    int l = launch_items(plan->d_probs, plan->d_items, plan->ncls, plan->cap);
+    if (!l) l = launch_trsm(w, (char *)plan->d_trsm, plan->cap, plan->side, plan->fork,
+                            plan->join);
     hipError_t e2 = hipStreamEndCapture(plan->cap, &plan->graph);
-    if (l || e2) st = MUBLIS_EHIP;
+    if (l || e2) st = l ? l : MUBLIS_EHIP;
   }
   if (!st && hipGraphInstantiate(&plan->exec, plan->graph, nullptr, nullptr, 0))
     st = MUBLIS_EHIP;
@@ -378,7 +524,7 @@ size_t mublis_sgrouped_ws_bytes(const mublis_sbatch_op_t *ops, int count) {
   Work w;
   if (!ops || count < 0 || build_work(ops, count, 0, w)) return 0;
   return align256(sizeof(GemmArgs<float>) * w.probs.size()) +
-         align256(sizeof(GroupItem) * w.items.size()) + 256;
+         align256(sizeof(GroupItem) * w.items.size()) + trsm_bytes(w) + 256;
 }
 
 int mublis_sgrouped_launch_ws(const mublis_sbatch_op_t *ops, int count, void *host_stage,
@@ -389,15 +535,21 @@ int mublis_sgrouped_launch_ws(const mublis_sbatch_op_t *ops, int count, void *ho
   if (st) return st;
   const size_t pb = align256(sizeof(GemmArgs<float>) * w.probs.size());
   const size_t ib = align256(sizeof(GroupItem) * w.items.size());
-  if (pb + ib > ws_bytes) return MUBLIS_EINVAL;
-  if (w.items.empty()) return MUBLIS_OK;
+  const size_t hb = trsm_host_bytes(w);
+  if (pb + ib + trsm_bytes(w) > ws_bytes) return MUBLIS_EINVAL;
+  if (w.items.empty() && w.titems.empty() && w.solo.empty()) return MUBLIS_OK;
+  // host_stage mirrors dev_ws: GEMM descriptors and items, then the TRSM
+  // descriptors and items (the inverses and solo workspace stay on device)
   char *h = (char *)host_stage;
   memcpy(h, w.probs.data(), sizeof(GemmArgs<float>) * w.probs.size());
   memcpy(h + pb, w.items.data(), sizeof(GroupItem) * w.items.size());
+  trsm_stage(w, h + pb + ib);
   hipStream_t s = (hipStream_t)(stream ? stream : mublis_rt_stream());
-  if (hipMemcpyAsync(dev_ws, h, pb + ib, hipMemcpyHostToDevice, s)) return MUBLIS_EHIP;
-  return launch_items((const GemmArgs<float> *)dev_ws, (const GroupItem *)((char *)dev_ws + pb),
-                      w.ncls, s);
+  if (hipMemcpyAsync(dev_ws, h, pb + ib + hb, hipMemcpyHostToDevice, s)) return MUBLIS_EHIP;
+  st = launch_items((const GemmArgs<float> *)dev_ws, (const GroupItem *)((char *)dev_ws + pb),
+                    w.ncls, s);
+  if (!st) st = launch_trsm(w, (char *)dev_ws + pb + ib, s);
+  return st;
 }
 
 int mublis_grouped_plan_execute(mublis_grouped_plan_t plan, void *stream) {
@@ -423,8 +575,12 @@ int mublis_grouped_plan_destroy(mublis_grouped_plan_t plan) {
   if (plan->exec) (void)hipGraphExecDestroy(plan->exec);
   if (plan->graph) (void)hipGraphDestroy(plan->graph);
   if (plan->cap) (void)hipStreamDestroy(plan->cap);
+  if (plan->side) (void)hipStreamDestroy(plan->side);
+  if (plan->fork) (void)hipEventDestroy(plan->fork);
+  if (plan->join) (void)hipEventDestroy(plan->join);
   if (plan->d_probs) (void)hipFree(plan->d_probs);
   if (plan->d_items) (void)hipFree(plan->d_items);
+  if (plan->d_trsm) (void)hipFree(plan->d_trsm);
   delete plan;
   return MUBLIS_OK;
 }

@@ -55,4 +55,43 @@ int gfx950_pack_f2d(int m, int n, const float *src, int64_t rs, int64_t cs, int 
 int gfx950_refine_update(int m, int n, const float *d, double *x64, float *out,
                          int64_t rs_o, int64_t cs_o, hipStream_t s);
 
+// Grouped triangular solves (the batched interface's TRSM problems), each in
+// the left form B := alpha * inv(A) * B: A m x m lower (lower = 1) or upper
+// through (rs_a, cs_a), unit diagonal when unit = 1, B m x n through (rs_b,
+// cs_b).  The problem's FP64 inverted 64-row diagonal blocks live `inv`
+// doubles into the launch's inverse workspace (ceil(m / 64) * 64 * 64
+// doubles per problem).
+template <typename T>
+struct TrsmGroupProb {
+  int lower, unit, m, n;
+  T alpha;
+  const T *a;
+  int64_t rs_a, cs_a;
+  T *b;
+  int64_t rs_b, cs_b;
+  int64_t inv;
+};
+// Work item: problem, and its diagonal block (inverse launch) or column
+// panel of kGroupTrsmCols columns (solve launch).
+struct TrsmGroupItem {
+  int prob, idx;
+};
+constexpr int kGroupTrsmCols = 16;
+// Orders up to kGroupTrsmSmall run two workgroups per CU, up to
+// kGroupTrsmMax one; larger problems are solved by themselves.
+constexpr int kGroupTrsmSmall = 512;
+constexpr int kGroupTrsmMax = 1024;
+// Launches over six work lists stored back to back in `items` (device
+// memory), counts[c] items each: the inverse items of lower, then of upper
+// problems; the panels of lower, then upper problems of order at most
+// kGroupTrsmSmall; the panels of the larger lower, then upper problems.
+// Asynchronous on s, nothing allocated.  Given a second stream s2 and two
+// events, the upper problems' launches run on s2, forked from s and joined
+// back into s (concurrent graph branches when s is being captured).
+template <typename T>
+int gfx950_trsm_grouped(const TrsmGroupProb<T> *probs, const TrsmGroupItem *items,
+                        const int *counts, double *inv, hipStream_t s,
+                        hipStream_t s2 = nullptr, hipEvent_t fork = nullptr,
+                        hipEvent_t join = nullptr);
+
 #endif

@@ -57,16 +57,14 @@ constexpr int BLK = NB * NB + NB;
 // computed in FP64 by one prep launch that also clears the flags.
 constexpr int PB = NB;  // block rows of the pipeline
 
+// Inverse of diagonal block p of A into inv + p * PB * PB (row-major), by
+// one wavefront; LT and rd are its LDS (LT[k][r] = A(base + r, base + k) in
+// the triangle, rd the reciprocal diagonal).
 template <typename T, bool LOWER>
-__global__ __launch_bounds__(64) void trsm_inv_prep(int m, const T *__restrict__ a,
-                                                    int64_t rs_a, int64_t cs_a,
-                                                    int diag_mode,
-                                                    double *__restrict__ inv,
-                                                    int *__restrict__ flags, int nflags) {
-  __shared__ double LT[PB][PB + 1];  // LT[k][r] = A(base + r, base + k) in the triangle
-  __shared__ double rd[PB];
-  const int p = blockIdx.x, lane = threadIdx.x;
-  for (int i = p * PB + lane; i < nflags; i += gridDim.x * PB) flags[i] = 0;
+__device__ __forceinline__ void inv_prep_block(int p, int m, const T *__restrict__ a, int64_t rs_a,
+                                      int64_t cs_a, int diag_mode, double *__restrict__ inv,
+                                      double (*LT)[PB + 1], double *rd) {
+  const int lane = threadIdx.x;
   const int base = p * PB;
   const int nbr = min(PB, m - base);
   // lanes run along A's unit stride
@@ -131,6 +129,19 @@ __global__ __launch_bounds__(64) void trsm_inv_prep(int m, const T *__restrict__
   double *o = inv + (int64_t)p * PB * PB;  // row-major
 #pragma unroll
   for (int r = 0; r < PB; ++r) o[r * PB + c] = s[r];
+}
+
+template <typename T, bool LOWER>
+__global__ __launch_bounds__(64) void trsm_inv_prep(int m, const T *__restrict__ a,
+                                                    int64_t rs_a, int64_t cs_a,
+                                                    int diag_mode,
+                                                    double *__restrict__ inv,
+                                                    int *__restrict__ flags, int nflags) {
+  __shared__ double LT[PB][PB + 1];
+  __shared__ double rd[PB];
+  const int p = blockIdx.x;
+  for (int i = p * PB + (int)threadIdx.x; i < nflags; i += gridDim.x * PB) flags[i] = 0;
+  inv_prep_block<T, LOWER>(p, m, a, rs_a, cs_a, diag_mode, inv, LT, rd);
 }
 
 // Spin until *f == epoch (or the abort flag is up).  The loads are relaxed:
@@ -492,19 +503,18 @@ __global__ __launch_bounds__(256) void trsm_inv_apply(int nb, int n, T alpha,
 constexpr int kPanelRows = 512;
 constexpr int PW = 16;
 
+// Columns c0 .. c0 + PW - 1 of the solve, by one workgroup; xs holds at
+// least m rows.
 template <typename T, bool LOWER>
-__global__ __launch_bounds__(256) void trsm_panel(int m, int n, T alpha,
-                                                  const T *__restrict__ a, int64_t rs_a,
-                                                  int64_t cs_a, T *b, int64_t rs_b,
-                                                  int64_t cs_b,
-                                                  const double *__restrict__ inv) {
+__device__ __forceinline__ void panel_solve(int m, int n, T alpha, const T *__restrict__ a,
+                                   int64_t rs_a, int64_t cs_a, T *b, int64_t rs_b,
+                                   int64_t cs_b, const double *__restrict__ inv, int c0,
+                                   double (*xs)[PW + 1]) {
   typedef double d4 __attribute__((ext_vector_type(4)));
   constexpr int TV = PB * PW / 256;
-  __shared__ double xs[kPanelRows][PW + 1];
   const int nbk = (m + PB - 1) / PB;
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   const int lr = lane & 15, lk = lane >> 4;
-  const int c0 = blockIdx.x * PW;
   const bool cm = rs_b == 1;
   const double al = (double)alpha;
   for (int j = 0; j < nbk; ++j) {
@@ -570,6 +580,48 @@ __global__ __launch_bounds__(256) void trsm_panel(int m, int n, T alpha,
   }
 }
 
+template <typename T, bool LOWER>
+__global__ __launch_bounds__(256) void trsm_panel(int m, int n, T alpha,
+                                                  const T *__restrict__ a, int64_t rs_a,
+                                                  int64_t cs_a, T *b, int64_t rs_b,
+                                                  int64_t cs_b,
+                                                  const double *__restrict__ inv) {
+  __shared__ double xs[kPanelRows][PW + 1];
+  panel_solve<T, LOWER>(m, n, alpha, a, rs_a, cs_a, b, rs_b, cs_b, inv, blockIdx.x * PW, xs);
+}
+
+// Grouped solves: one workgroup per item of a work list of independent
+// items (no item waits for another).  The inverse launch inverts one
+// diagonal block per item; the solve launches run one column panel per
+// item, the same step as trsm_panel, with LDS for ROWS rows of X.  (A
+// persistent loop over the items made these register-heavy bodies spill.)
+static_assert(kGroupTrsmCols == PW, "grouped panels are the column panels");
+static_assert(kGroupTrsmSmall == kPanelRows, "small grouped panels fit trsm_panel's LDS");
+
+// One launch per triangle orientation (both in one kernel spill).
+template <typename T, bool LOWER>
+__global__ __launch_bounds__(64) void grouped_inv_prep(const TrsmGroupProb<T> *__restrict__ probs,
+                                                       const TrsmGroupItem *__restrict__ items,
+                                                       double *inv) {
+  __shared__ double LT[PB][PB + 1];
+  __shared__ double rd[PB];
+  const TrsmGroupItem w = items[blockIdx.x];
+  const TrsmGroupProb<T> &d = probs[w.prob];
+  inv_prep_block<T, LOWER>(w.idx, d.m, d.a, d.rs_a, d.cs_a, d.unit, inv + d.inv, LT, rd);
+}
+
+template <typename T, bool LOWER, int ROWS>
+__global__ __launch_bounds__(256) void grouped_panels(const TrsmGroupProb<T> *__restrict__ probs,
+                                                      const TrsmGroupItem *__restrict__ items,
+                                                      const double *inv) {
+  __shared__ double xs[ROWS][PW + 1];
+  const TrsmGroupItem w = items[blockIdx.x];
+  const TrsmGroupProb<T> &d = probs[w.prob];
+  panel_solve<T, LOWER>(d.m, d.n, d.alpha, d.a, d.rs_a, d.cs_a, d.b, d.rs_b, d.cs_b,
+                        inv + d.inv, w.idx * PW, xs);
+}
+
+
 // Widest B solved by the pipelined kernel, at most 4096; 0 disables it.
 // Below that cap a whole solve takes the pipeline for B up to 128 columns,
 // up to m * n = 2^21, or up to 768 columns and m * n = 2^23; beyond, the
@@ -581,9 +633,10 @@ __global__ __launch_bounds__(256) void trsm_panel(int m, int n, T alpha,
 // the recursion's GEMM leaves whenever those apply (float 8192 x 256: 0.93
 // ms vs 1.09 pipelined; 8192 x 128: pipelined 0.82 vs 0.93).  The recursive
 // schedule in turn ends in GEMM leaves (B from 256 columns for double, from
-// 64 for float; see use_inv_leaves), else hands its diagonal sub-problems of at most kPipeLeaf
-// rows to the pipeline (narrower B) or to column-panel workgroups (wider B,
-// when the solve brings no room for the GEMM leaves' inverses).
+// 64 for float; see use_inv_leaves), else hands its diagonal sub-problems of
+// at most kPipeLeaf rows to the pipeline (narrower B) or to column-panel
+// workgroups (wider B, when the solve brings no room for the GEMM leaves'
+// inverses).
 int g_pipe_max_n = 4096;
 constexpr int kPipeLeaf = 512;
 // From this many columns up, the pipelined leaves of at most kPanelRows
@@ -967,6 +1020,52 @@ int trsm_dev(int lower, int unit, int m, int n, T alpha, const T *a, int64_t rs_
 }
 
 }  // namespace
+
+template <typename T>
+int gfx950_trsm_grouped(const TrsmGroupProb<T> *probs, const TrsmGroupItem *items,
+                        const int *counts, double *inv, hipStream_t s, hipStream_t s2,
+                        hipEvent_t fork, hipEvent_t join) {
+  // The inverse kernel keeps a 64-double column per lane in registers (one
+  // wavefront per SIMD); the panels hold 512 (1024) rows of X in LDS, two
+  // (one) workgroups per CU.  Lists run longest items first.  Lower and
+  // upper problems are independent: with a second stream they run as two
+  // concurrent branches, each launch alone leaving CUs idle.
+  const bool two = s2 && fork && join;
+  if (two && (hipEventRecord(fork, s) != hipSuccess || hipStreamWaitEvent(s2, fork, 0) != hipSuccess))
+    return MUBLIS_EHIP;
+  hipStream_t su = two ? s2 : s;
+  const TrsmGroupItem *it[6];
+  it[0] = items;
+  for (int c = 1; c < 6; ++c) it[c] = it[c - 1] + counts[c - 1];
+  if (counts[0] > 0)
+    hipLaunchKernelGGL((grouped_inv_prep<T, true>), dim3(counts[0]), dim3(64), 0, s, probs,
+                       it[0], inv);
+  if (counts[1] > 0)
+    hipLaunchKernelGGL((grouped_inv_prep<T, false>), dim3(counts[1]), dim3(64), 0, su, probs,
+                       it[1], inv);
+  if (counts[2] > 0)
+    hipLaunchKernelGGL((grouped_panels<T, true, kGroupTrsmSmall>), dim3(counts[2]), dim3(256),
+                       0, s, probs, it[2], inv);
+  if (counts[3] > 0)
+    hipLaunchKernelGGL((grouped_panels<T, false, kGroupTrsmSmall>), dim3(counts[3]), dim3(256),
+                       0, su, probs, it[3], inv);
+  if (counts[4] > 0)
+    hipLaunchKernelGGL((grouped_panels<T, true, kGroupTrsmMax>), dim3(counts[4]), dim3(256), 0,
+                       s, probs, it[4], inv);
+  if (counts[5] > 0)
+    hipLaunchKernelGGL((grouped_panels<T, false, kGroupTrsmMax>), dim3(counts[5]), dim3(256), 0,
+                       su, probs, it[5], inv);
+  if (hipGetLastError() != hipSuccess) return MUBLIS_EHIP;
+  if (two && (hipEventRecord(join, s2) != hipSuccess || hipStreamWaitEvent(s, join, 0) != hipSuccess))
+    return MUBLIS_EHIP;
+  return MUBLIS_OK;
+}
+template int gfx950_trsm_grouped<float>(const TrsmGroupProb<float> *, const TrsmGroupItem *,
+                                        const int *, double *, hipStream_t, hipStream_t,
+                                        hipEvent_t, hipEvent_t);
+template int gfx950_trsm_grouped<double>(const TrsmGroupProb<double> *, const TrsmGroupItem *,
+                                         const int *, double *, hipStream_t, hipStream_t,
+                                         hipEvent_t, hipEvent_t);
 
 extern "C" int mublis_gfx950_dtrsm_inverted_dev(int lower, int m, int n,
                                                 const double *a, int64_t rs_a,

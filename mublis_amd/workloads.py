@@ -3,7 +3,8 @@
 `grouped_problems` draws the batched configuration: GEMM problems with M,
 N, K log-uniform in a range and SYRK problems with n and k log-uniform,
 entries uniform[-1, 1], column-major device tensors, plus the
-`SbatchOp` descriptors that point at them.  Tensors are stored transposed
+`SbatchOp` descriptors that point at them.  `grouped_trsm_problems` draws
+batched STRSM problems the same way.  Tensors are stored transposed
 (shape (cols, ld)) so that a row-major torch tensor is a column-major
 matrix.
 """
@@ -11,7 +12,7 @@ import math
 
 import numpy as np
 
-from . import BATCH_GEMM, BATCH_SYRK, SbatchOp
+from . import BATCH_GEMM, BATCH_SYRK, BATCH_TRSM, SbatchOp
 
 
 def _logu(rng, lo, hi, size):
@@ -59,6 +60,33 @@ def grouped_problems(torch, seed=314159, n_gemm=4096, n_syrk=1024, gemm_range=(1
                             A.data_ptr(), A.shape[1], None, 1, C.data_ptr(), n))
         keep += [A, C]
         specs.append(dict(op="syrk", n=n, k=k, trans=tr, upper=up, A=A, C=C, lda=A.shape[1]))
+    return ops, keep, specs
+
+
+def grouped_trsm_problems(torch, seed=271828, count=1024, size=(16, 512), alpha=1.0,
+                          device="cuda"):
+    """`count` STRSM problems for a grouped launch: B m x n with m, n
+    log-uniform in `size`, side, uplo and trans_a drawn uniformly, non-unit
+    diagonal; A's stored triangle has off-diagonal entries uniform[-1, 1] /
+    order and diagonal uniform[1, 2] (diagonally dominant), B uniform[-1, 1].
+    Returns (ops, keep, specs) like grouped_problems."""
+    rng = np.random.default_rng(seed)
+    g = torch.Generator(device=device).manual_seed(seed)
+    ops, keep, specs = [], [], []
+    ms = _logu(rng, *size, count)
+    ns = _logu(rng, *size, count)
+    for i in range(count):
+        m, n = int(ms[i]), int(ns[i])
+        side, up, tr = (int(x) for x in rng.integers(0, 2, size=3))
+        order = n if side else m
+        A = (torch.rand((order, order), device=device, generator=g) * 2 - 1) / order
+        A.diagonal().copy_(1 + torch.rand(order, device=device, generator=g))
+        B = torch.rand((n, m), device=device, generator=g) * 2 - 1
+        ops.append(SbatchOp(BATCH_TRSM, tr, 0, 2 if up else 1, m, n, 0, alpha, 0.0,
+                            A.data_ptr(), order, None, 1, B.data_ptr(), m, side, 0))
+        keep += [A, B]
+        specs.append(dict(op="trsm", m=m, n=n, side=side, upper=up, trans=tr, A=A, B=B,
+                          order=order))
     return ops, keep, specs
 
 

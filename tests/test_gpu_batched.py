@@ -139,3 +139,154 @@ def test_autotuner_writes_and_uses_table(lib, tmp_path):
         expect[table[(lg(s["m"]), lg(s["n"]), lg(s["k"]))]] += 1
     assert hist == expect
     lib.mublis_grouped_load_table(None)
+
+
+# Grouped TRSM problems: (side, uplo, trans_a, diag, m, n, alpha).  Orders
+# (m for Left, n for Right) cover one partial 64-row block, the two panel
+# classes (<= 512 and <= 1024 rows of X in LDS) at their limits, problems
+# solved by themselves (order > 1024) and alpha = 0 (B := 0, A unread).
+TRSM_CASES = [
+    ("L", "lo", 0, 0, 1, 1, 1.0),
+    ("L", "up", 1, 1, 17, 33, -1.5),
+    ("R", "lo", 0, 0, 40, 64, 0.75),
+    ("R", "up", 1, 0, 100, 130, 1.0),
+    ("L", "lo", 1, 0, 511, 16, 2.0),
+    ("L", "up", 0, 1, 512, 70, 1.0),
+    ("L", "lo", 0, 0, 513, 20, -1.0),
+    ("R", "up", 0, 0, 90, 1024, 0.5),
+    ("L", "up", 1, 0, 1100, 40, 1.0),
+    ("R", "lo", 1, 1, 30, 1500, 1.0),
+    ("L", "up", 0, 0, 200, 50, 0.0),
+]
+
+
+def trsm_batch(torch, seed):
+    """SbatchOp list for TRSM_CASES on device operands (NaN outside each
+    triangle and on unit diagonals), plus specs for checking."""
+    rng = np.random.default_rng(seed)
+    ops, specs = [], []
+    for side, uplo, trans, diag, m, n, alpha in TRSM_CASES:
+        left, lower = side == "L", uplo == "lo"
+        order = m if left else n
+        a = L.conditioned_triangular(rng, order, np.float32, "F", lower, diag == 1)
+        b0 = L.rand(rng, (m, n), np.float32, "F")
+        A = torch.from_numpy(np.ascontiguousarray(a.T)).cuda()
+        B = torch.from_numpy(np.ascontiguousarray(b0.T)).cuda()
+        ops.append(M.SbatchOp(M.BATCH_TRSM, trans, 0, 1 if lower else 2, m, n, 0, alpha, 0.0,
+                              M.ptr(A), order, None, 1, M.ptr(B), m, 0 if left else 1, diag))
+        specs.append(dict(left=left, lower=lower, trans=trans, unit=diag == 1, m=m, n=n,
+                          alpha=alpha, a=a, b0=b0, A=A, B=B, order=order))
+    return ops, specs
+
+
+def trsm_single(lib, torch, s):
+    """The same solve as one cblas_strsm call, on a copy of B0."""
+    X = torch.from_numpy(np.ascontiguousarray(s["b0"].T)).cuda()
+    lib.cblas_strsm(102, 141 if s["left"] else 142, 122 if s["lower"] else 121,
+                    112 if s["trans"] else 111, 132 if s["unit"] else 131, s["m"], s["n"],
+                    s["alpha"], M.ptr(s["A"]), s["order"], M.ptr(X), s["m"])
+    torch.cuda.synchronize()
+    return X.cpu().numpy().T
+
+
+def check_trsm(lib, torch, specs):
+    """Each grouped result: residual probe in long double within
+    2 s eps ||A|| ||X|| ||v|| (s the triangle's order), and within a relative
+    forward difference of 50 s eps of the single cblas_strsm call."""
+    from test_gpu_trsm_pipe import residual
+    eps = np.finfo(np.float32).eps
+    bad = []
+    for i, s in enumerate(specs):
+        got = s["B"].cpu().numpy().T
+        want = trsm_single(lib, torch, s)
+        tag = f"case {i} {TRSM_CASES[i]}"
+        if s["alpha"] == 0.0:
+            if not (np.all(got == 0) and np.all(want == 0)):
+                bad.append(f"{tag}: alpha = 0 must give B = 0")
+            continue
+        res, lim = residual(s["a"], got, s["b0"], s["alpha"], not s["left"], s["lower"],
+                            s["trans"], s["unit"])
+        if not np.isfinite(got).all() or not res <= lim:
+            bad.append(f"{tag}: residual {res:.3e} > {lim:.3e}")
+        diff = np.linalg.norm(got.astype(np.float64) - want) / np.linalg.norm(want)
+        if not diff <= 50 * s["order"] * eps:
+            bad.append(f"{tag}: grouped vs cblas_strsm {diff:.3e}")
+    assert not bad, "\n".join(bad)
+
+
+def test_grouped_trsm_mixed_with_gemm_and_syrk(lib):
+    """TRSM problems in one plan with GEMM and SYRK problems: the TRSM
+    results meet the residual and single-call bars, the GEMM/SYRK results
+    stay bit-identical to single calls, and a graph replay solves again
+    (B restored first) to the same bits."""
+    torch = pytest.importorskip("torch")
+    gops, keep, gspecs = grouped_problems(torch, seed=11, n_gemm=60, n_syrk=16,
+                                          syrk_n=(32, 300), vary_flags=True)
+    initial = [s["C"].clone() for s in gspecs]
+    want = single_calls(lib, torch, gspecs, initial, 1.0, 1.0)
+    for s, c0 in zip(gspecs, initial):
+        s["C"].copy_(c0)
+    tops, tspecs = trsm_batch(torch, 5)
+    ops = gops[:30] + tops + gops[30:]
+    plan = M.GroupedPlan(ops)
+    try:
+        plan.execute()
+        torch.cuda.synchronize()
+        info = plan.info()
+        first = [s["B"].clone() for s in tspecs]
+        check_trsm(lib, torch, tspecs)
+        bad = [i for i, (s, w) in enumerate(zip(gspecs, want)) if not torch.equal(s["C"], w)]
+        assert not bad, f"{len(bad)} GEMM/SYRK problems differ"
+        for s in tspecs:
+            s["B"].copy_(torch.from_numpy(np.ascontiguousarray(s["b0"].T)).cuda())
+        for s, c0 in zip(gspecs, initial):
+            s["C"].copy_(c0)
+        plan.execute()
+        torch.cuda.synchronize()
+        assert all(torch.equal(s["B"], f) for s, f in zip(tspecs, first)), "replay differs"
+    finally:
+        plan.close()
+    assert sum(info["tiles"]) == len(gops)
+    panels = sum((s["n"] if s["left"] else s["m"]) // 16 for s in tspecs)
+    assert info["items"] >= panels
+
+
+def test_grouped_trsm_workspace_launch(lib):
+    """mublis_sgrouped_launch_ws (the serving loop's form) on the TRSM batch:
+    the same bits as the captured plan."""
+    torch = pytest.importorskip("torch")
+    tops, tspecs = trsm_batch(torch, 6)
+    arr = (M.SbatchOp * len(tops))(*tops)
+    plan = M.GroupedPlan(tops)
+    try:
+        plan.execute()
+        torch.cuda.synchronize()
+    finally:
+        plan.close()
+    via_plan = [s["B"].clone() for s in tspecs]
+    for s in tspecs:
+        s["B"].copy_(torch.from_numpy(np.ascontiguousarray(s["b0"].T)).cuda())
+    nbytes = lib.mublis_sgrouped_ws_bytes(ctypes.cast(arr, ctypes.c_void_p), len(tops))
+    assert nbytes > 0
+    host = torch.empty(nbytes, dtype=torch.uint8, pin_memory=True)
+    dev = torch.empty(nbytes, dtype=torch.uint8, device="cuda")
+    assert lib.mublis_sgrouped_launch_ws(ctypes.cast(arr, ctypes.c_void_p), len(tops),
+                                         host.data_ptr(), dev.data_ptr(), nbytes, None) == 0
+    torch.cuda.synchronize()
+    assert lib.mublis_rt_sync(lib.mublis_rt_stream()) == 0
+    assert all(torch.equal(s["B"], w) for s, w in zip(tspecs, via_plan))
+    check_trsm(lib, torch, tspecs)
+
+
+def test_grouped_trsm_rejects_bad_arguments(lib):
+    torch = pytest.importorskip("torch")
+    A = torch.zeros((8, 8), device="cuda")
+    B = torch.zeros((8, 8), device="cuda")
+    good = dict(op=M.BATCH_TRSM, trans_a=0, trans_b=0, uplo=1, m=8, n=8, k=0, alpha=1.0,
+                beta=0.0, a=M.ptr(A), lda=8, b=None, ldb=1, c=M.ptr(B), ldc=8, side=0, diag=0)
+    for field, value in [("uplo", 0), ("side", 2), ("diag", 3), ("lda", 4), ("ldc", 2),
+                         ("m", -1)]:
+        o = M.SbatchOp(**dict(good, **{field: value}))
+        h = ctypes.c_void_p()
+        st = lib.mublis_sgrouped_plan_create(ctypes.byref(o), 1, 0, ctypes.byref(h))
+        assert st == M.EINVAL, (field, st)
