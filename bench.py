@@ -318,7 +318,7 @@ def run_batched(args, torch, M, lib, extra):
 def run_batched_trsm(torch, M, lib, extra):
     """Batched STRSM: 1024 problems with m, n log-uniform [16, 512], side,
     uplo and transpose random, diagonally dominant A, in one grouped plan
-    (FP64 inverses of every diagonal block in one launch, then 16-column
+    (FP64 inverses of every diagonal block in one launch, then 16- or 32-column
     panels) replayed from its hipGraph; against the same problems as serial
     cblas_strsm calls.  Each replay solves in place again."""
     from mublis_amd.workloads import grouped_trsm_problems

@@ -234,7 +234,7 @@ int mublis_get_sgemm_mode(void);
  * trans_a, `diag` MUBLIS_UNIT_DIAG or MUBLIS_NON_UNIT_DIAG); b, k, beta
  * unused.  The 64-row diagonal blocks are inverted in FP64 and the block
  * updates run on FP64 MFMA, as in cblas_strsm; problems whose triangle has
- * order up to 1024 run as 16-column panels inside the grouped launches,
+ * order up to 1024 run as 16- or 32-column panels inside the grouped launches,
  * larger ones (and alpha = 0) as their own solve in the same plan or
  * launch.  Problems of one batch must not write what another reads or
  * writes.  side and diag are ignored by the other ops (zero them).

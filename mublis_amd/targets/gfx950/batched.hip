@@ -11,7 +11,7 @@
 // to what cblas_sgemm / cblas_ssyrk compute for it.  The 128-tile items and
 // the smaller ones are two launches (different LDS and occupancy), captured
 // once in a hipGraph and replayed.  TRSM problems run as independent
-// 16-column panels after one launch that inverts all their diagonal blocks
+// column panels after one launch that inverts all their diagonal blocks
 // (trsm.hip, gfx950_trsm_grouped), in the same graph.
 //
 // The reference has no batched interface; this is the batched addition on
@@ -187,11 +187,11 @@ struct Work {
   int ncls[3] = {0, 0, 0};       // items per tile shape (index 0: 32, 1: 64, 2: 128)
   double flops = 0, bytes = 0;
   int hist[3] = {0, 0, 0};
-  // TRSM: descriptors, the six work lists of gfx950_trsm_grouped back to
-  // back, FP64 inverse workspace, and the problems solved by themselves
+  // TRSM: descriptors, the work lists of gfx950_trsm_grouped back to back,
+  // FP64 inverse workspace, and the problems solved by themselves
   std::vector<TrsmGroupProb<float>> tprobs;
   std::vector<TrsmGroupItem> titems;
-  int tcounts[6] = {0, 0, 0, 0, 0, 0};
+  int tcounts[kGroupTrsmLists] = {};
   int64_t inv_doubles = 0;
   std::vector<SoloTrsm> solo;
   size_t solo_bytes = 0;
@@ -240,9 +240,9 @@ int add_trsm(const mublis_sbatch_op_t &o, Work &w,
   const int nbk = (mm + 63) / 64;
   w.inv_doubles += (int64_t)nbk * 64 * 64;
   for (int p = 0; p < nbk; ++p) lists[lower ? 0 : 1].push_back({pid, p});
-  const int cls = (mm <= kGroupTrsmSmall ? 2 : 4) + (lower ? 0 : 1);
-  for (int c = 0; c < (nn + kGroupTrsmCols - 1) / kGroupTrsmCols; ++c)
-    lists[cls].push_back({pid, c});
+  const int cls = (mm <= kGroupTrsmWide ? 2 : mm <= kGroupTrsmSmall ? 4 : 6) + (lower ? 0 : 1);
+  const int cols = mm <= kGroupTrsmWide ? kGroupTrsmWideCols : kGroupTrsmCols;
+  for (int c = 0; c < (nn + cols - 1) / cols; ++c) lists[cls].push_back({pid, c});
   return MUBLIS_OK;
 }
 
@@ -250,7 +250,7 @@ int add_trsm(const mublis_sbatch_op_t &o, Work &w,
 int build_work(const mublis_sbatch_op_t *ops, int count, int flags, Work &w) {
   std::vector<GroupItem> items;
   std::vector<double> cost;
-  std::vector<std::vector<TrsmGroupItem>> tlists(6);
+  std::vector<std::vector<TrsmGroupItem>> tlists(kGroupTrsmLists);
   w.probs.reserve(count);
   const int forced = (flags >> 4) & 3;  // MUBLIS_GROUPED_FORCE_TILE(t) stores t + 1
   for (int i = 0; i < count; ++i) {
@@ -371,12 +371,12 @@ int build_work(const mublis_sbatch_op_t *ops, int count, int flags, Work &w) {
   }
   // TRSM panels run longest first (a panel's work grows with the square of
   // its problem's order)
-  for (int c = 2; c < 6; ++c)
+  for (int c = 2; c < kGroupTrsmLists; ++c)
     std::stable_sort(tlists[c].begin(), tlists[c].end(),
                      [&](const TrsmGroupItem &x, const TrsmGroupItem &y) {
                        return w.tprobs[x.prob].m > w.tprobs[y.prob].m;
                      });
-  for (int c = 0; c < 6; ++c) {
+  for (int c = 0; c < kGroupTrsmLists; ++c) {
     w.tcounts[c] = (int)tlists[c].size();
     w.titems.insert(w.titems.end(), tlists[c].begin(), tlists[c].end());
   }
@@ -471,8 +471,8 @@ int mublis_sgrouped_plan_create(const mublis_sbatch_op_t *ops, int count, int fl
   for (int i = 0; i < 3; ++i) plan->tile_hist[i] = w.hist[i];
   const std::vector<GemmArgs<float>> &probs = w.probs;
   const std::vector<GroupItem> &sorted = w.items;
-  plan->nitems = (int)sorted.size() + w.tcounts[2] + w.tcounts[3] + w.tcounts[4] +
-                 w.tcounts[5] + (int)w.solo.size();
+  plan->nitems = (int)sorted.size() + (int)w.solo.size();
+  for (int c = 2; c < kGroupTrsmLists; ++c) plan->nitems += w.tcounts[c];
   for (int c = 0; c < 3; ++c) plan->ncls[c] = w.ncls[c];
   st = MUBLIS_OK;
   if (!probs.empty() && hipMalloc(&plan->d_probs, sizeof(GemmArgs<float>) * probs.size()))
@@ -496,8 +496,7 @@ int mublis_sgrouped_plan_create(const mublis_sbatch_op_t *ops, int count, int fl
   }
   // Capture the launches once; executions replay the graph.
   if (!st && hipStreamCreateWithFlags(&plan->cap, hipStreamNonBlocking)) st = MUBLIS_EHIP;
-  if (!st && (w.tcounts[1] + w.tcounts[3] + w.tcounts[5]) > 0 &&
-      (w.tcounts[0] + w.tcounts[2] + w.tcounts[4]) > 0 &&
+  if (!st && w.tcounts[0] > 0 && w.tcounts[1] > 0 &&  // lower and upper problems
       (hipStreamCreateWithFlags(&plan->side, hipStreamNonBlocking) ||
        hipEventCreateWithFlags(&plan->fork, hipEventDisableTiming) ||
        hipEventCreateWithFlags(&plan->join, hipEventDisableTiming)))

@@ -72,19 +72,25 @@ struct TrsmGroupProb {
   int64_t inv;
 };
 // Work item: problem, and its diagonal block (inverse launch) or column
-// panel of kGroupTrsmCols columns (solve launch).
+// panel (solve launch).
 struct TrsmGroupItem {
   int prob, idx;
 };
+// Panel classes by the triangle's order: up to kGroupTrsmWide, panels of
+// kGroupTrsmWideCols columns; up to kGroupTrsmSmall, of kGroupTrsmCols
+// columns (both two workgroups per CU); up to kGroupTrsmMax, of
+// kGroupTrsmCols columns at one workgroup per CU.  Larger problems are
+// solved by themselves.
 constexpr int kGroupTrsmCols = 16;
-// Orders up to kGroupTrsmSmall run two workgroups per CU, up to
-// kGroupTrsmMax one; larger problems are solved by themselves.
+constexpr int kGroupTrsmWideCols = 32;
+constexpr int kGroupTrsmWide = 256;
 constexpr int kGroupTrsmSmall = 512;
 constexpr int kGroupTrsmMax = 1024;
-// Launches over six work lists stored back to back in `items` (device
-// memory), counts[c] items each: the inverse items of lower, then of upper
-// problems; the panels of lower, then upper problems of order at most
-// kGroupTrsmSmall; the panels of the larger lower, then upper problems.
+constexpr int kGroupTrsmLists = 8;
+// Launches over kGroupTrsmLists work lists stored back to back in `items`
+// (device memory), counts[c] items each: the inverse items of lower, then
+// of upper problems; then, for each panel class in the order above, the
+// panels of its lower, then of its upper problems.
 // Asynchronous on s, nothing allocated.  Given a second stream s2 and two
 // events, the upper problems' launches run on s2, forked from s and joined
 // back into s (concurrent graph branches when s is being captured).

@@ -503,15 +503,16 @@ __global__ __launch_bounds__(256) void trsm_inv_apply(int nb, int n, T alpha,
 constexpr int kPanelRows = 512;
 constexpr int PW = 16;
 
-// Columns c0 .. c0 + PW - 1 of the solve, by one workgroup; xs holds at
-// least m rows.
-template <typename T, bool LOWER>
+// Columns c0 .. c0 + W - 1 of the solve, by one workgroup (each wave 16 rows
+// of a block row times W columns); xs holds at least m rows.
+template <typename T, bool LOWER, int W = PW>
 __device__ __forceinline__ void panel_solve(int m, int n, T alpha, const T *__restrict__ a,
-                                   int64_t rs_a, int64_t cs_a, T *b, int64_t rs_b,
-                                   int64_t cs_b, const double *__restrict__ inv, int c0,
-                                   double (*xs)[PW + 1]) {
+                                            int64_t rs_a, int64_t cs_a, T *b, int64_t rs_b,
+                                            int64_t cs_b, const double *__restrict__ inv,
+                                            int c0, double (*xs)[W + 1]) {
   typedef double d4 __attribute__((ext_vector_type(4)));
-  constexpr int TV = PB * PW / 256;
+  constexpr int TV = PB * W / 256;  // tile elements per thread
+  constexpr int TN = W / 16;        // 16-column MFMA tiles per wave
   const int nbk = (m + PB - 1) / PB;
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   const int lr = lane & 15, lk = lane >> 4;
@@ -522,13 +523,15 @@ __device__ __forceinline__ void panel_solve(int m, int n, T alpha, const T *__re
     const int arow = p * PB + 16 * wave + lr;
     T breg[TV];
     double ireg[16];
-    load_tile<T, PW>(breg, b, rs_b, cs_b, m, n, p, c0);
+    load_tile<T, W>(breg, b, rs_b, cs_b, m, n, p, c0);
     {
       const double *ip = inv + (int64_t)p * PB * PB + (16 * wave + lr) * PB;
 #pragma unroll
       for (int kk = 0; kk < 16; ++kk) ireg[kk] = ip[4 * kk + lk];
     }
-    d4 acc = d4{0, 0, 0, 0};
+    d4 acc[TN];
+#pragma unroll
+    for (int t = 0; t < TN; ++t) acc[t] = d4{0, 0, 0, 0};
     T av[16], an[16];
     const bool prow = (p + 1) * PB <= m;
     if (j > 0) {
@@ -543,8 +546,10 @@ __device__ __forceinline__ void panel_solve(int m, int n, T alpha, const T *__re
       }
 #pragma unroll
       for (int kk = 0; kk < 16; ++kk)
-        acc = __builtin_amdgcn_mfma_f64_16x16x4f64((double)av[kk], xs[q * PB + 4 * kk + lk][lr],
-                                                   acc, 0, 0, 0);
+#pragma unroll
+        for (int t = 0; t < TN; ++t)
+          acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(
+              (double)av[kk], xs[q * PB + 4 * kk + lk][16 * t + lr], acc[t], 0, 0, 0);
 #pragma unroll
       for (int kk = 0; kk < 16; ++kk) av[kk] = an[kk];
     }
@@ -552,28 +557,36 @@ __device__ __forceinline__ void panel_solve(int m, int n, T alpha, const T *__re
 #pragma unroll
     for (int it = 0; it < TV; ++it) {
       int r, cc;
-      tile_rc<PW>(it * 256 + tid, cm, r, cc);
+      tile_rc<W>(it * 256 + tid, cm, r, cc);
       xs[p * PB + r][cc] = al * (double)breg[it];
     }
     __syncthreads();
 #pragma unroll
-    for (int r = 0; r < 4; ++r) xs[p * PB + 16 * wave + lk + 4 * r][lr] -= acc[r];
+    for (int t = 0; t < TN; ++t)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) xs[p * PB + 16 * wave + lk + 4 * r][16 * t + lr] -= acc[t][r];
     __syncthreads();
-    d4 out = d4{0, 0, 0, 0};
+    d4 out[TN];
+#pragma unroll
+    for (int t = 0; t < TN; ++t) out[t] = d4{0, 0, 0, 0};
 #pragma unroll
     for (int kk = 0; kk < 16; ++kk)
-      out = __builtin_amdgcn_mfma_f64_16x16x4f64(ireg[kk], xs[p * PB + 4 * kk + lk][lr], out,
-                                                 0, 0, 0);
+#pragma unroll
+      for (int t = 0; t < TN; ++t)
+        out[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(
+            ireg[kk], xs[p * PB + 4 * kk + lk][16 * t + lr], out[t], 0, 0, 0);
     __syncthreads();
 #pragma unroll
-    for (int r = 0; r < 4; ++r) xs[p * PB + 16 * wave + lk + 4 * r][lr] = out[r];
+    for (int t = 0; t < TN; ++t)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) xs[p * PB + 16 * wave + lk + 4 * r][16 * t + lr] = out[t][r];
     __syncthreads();
   }
   for (int j = 0; j < nbk; ++j) {
 #pragma unroll
     for (int it = 0; it < TV; ++it) {
       int r, cc;
-      tile_rc<PW>(it * 256 + tid, cm, r, cc);
+      tile_rc<W>(it * 256 + tid, cm, r, cc);
       const int gr = j * PB + r, gc = c0 + cc;
       if (gr < m && gc < n) b[(int64_t)gr * rs_b + (int64_t)gc * cs_b] = (T)xs[j * PB + r][cc];
     }
@@ -597,6 +610,7 @@ __global__ __launch_bounds__(256) void trsm_panel(int m, int n, T alpha,
 // persistent loop over the items made these register-heavy bodies spill.)
 static_assert(kGroupTrsmCols == PW, "grouped panels are the column panels");
 static_assert(kGroupTrsmSmall == kPanelRows, "small grouped panels fit trsm_panel's LDS");
+static_assert(kGroupTrsmLists == 8, "two inverse lists and three panel classes");
 
 // One launch per triangle orientation (both in one kernel spill).
 template <typename T, bool LOWER>
@@ -610,15 +624,17 @@ __global__ __launch_bounds__(64) void grouped_inv_prep(const TrsmGroupProb<T> *_
   inv_prep_block<T, LOWER>(w.idx, d.m, d.a, d.rs_a, d.cs_a, d.unit, inv + d.inv, LT, rd);
 }
 
-template <typename T, bool LOWER, int ROWS>
-__global__ __launch_bounds__(256) void grouped_panels(const TrsmGroupProb<T> *__restrict__ probs,
+// Two workgroups per CU where LDS allows (the 32-column lower body needs the
+// bound to fit 256 registers; without it, one per CU).
+template <typename T, bool LOWER, int ROWS, int W>
+__global__ __launch_bounds__(256, ROWS <= kGroupTrsmSmall ? 2 : 1) void grouped_panels(const TrsmGroupProb<T> *__restrict__ probs,
                                                       const TrsmGroupItem *__restrict__ items,
                                                       const double *inv) {
-  __shared__ double xs[ROWS][PW + 1];
+  __shared__ double xs[ROWS][W + 1];
   const TrsmGroupItem w = items[blockIdx.x];
   const TrsmGroupProb<T> &d = probs[w.prob];
-  panel_solve<T, LOWER>(d.m, d.n, d.alpha, d.a, d.rs_a, d.cs_a, d.b, d.rs_b, d.cs_b,
-                        inv + d.inv, w.idx * PW, xs);
+  panel_solve<T, LOWER, W>(d.m, d.n, d.alpha, d.a, d.rs_a, d.cs_a, d.b, d.rs_b, d.cs_b,
+                           inv + d.inv, w.idx * W, xs);
 }
 
 
@@ -1026,17 +1042,17 @@ int gfx950_trsm_grouped(const TrsmGroupProb<T> *probs, const TrsmGroupItem *item
                         const int *counts, double *inv, hipStream_t s, hipStream_t s2,
                         hipEvent_t fork, hipEvent_t join) {
   // The inverse kernel keeps a 64-double column per lane in registers (one
-  // wavefront per SIMD); the panels hold 512 (1024) rows of X in LDS, two
-  // (one) workgroups per CU.  Lists run longest items first.  Lower and
-  // upper problems are independent: with a second stream they run as two
-  // concurrent branches, each launch alone leaving CUs idle.
+  // wavefront per SIMD); the panels hold 256 x 32, 512 x 16 or 1024 x 16
+  // doubles of X in LDS, two, two and one workgroups per CU.  Lists run
+  // longest items first.  Lower and upper problems are independent: with a
+  // second stream they run as two concurrent branches.
   const bool two = s2 && fork && join;
   if (two && (hipEventRecord(fork, s) != hipSuccess || hipStreamWaitEvent(s2, fork, 0) != hipSuccess))
     return MUBLIS_EHIP;
   hipStream_t su = two ? s2 : s;
-  const TrsmGroupItem *it[6];
+  const TrsmGroupItem *it[kGroupTrsmLists];
   it[0] = items;
-  for (int c = 1; c < 6; ++c) it[c] = it[c - 1] + counts[c - 1];
+  for (int c = 1; c < kGroupTrsmLists; ++c) it[c] = it[c - 1] + counts[c - 1];
   if (counts[0] > 0)
     hipLaunchKernelGGL((grouped_inv_prep<T, true>), dim3(counts[0]), dim3(64), 0, s, probs,
                        it[0], inv);
@@ -1044,28 +1060,32 @@ int gfx950_trsm_grouped(const TrsmGroupProb<T> *probs, const TrsmGroupItem *item
     hipLaunchKernelGGL((grouped_inv_prep<T, false>), dim3(counts[1]), dim3(64), 0, su, probs,
                        it[1], inv);
   if (counts[2] > 0)
-    hipLaunchKernelGGL((grouped_panels<T, true, kGroupTrsmSmall>), dim3(counts[2]), dim3(256),
-                       0, s, probs, it[2], inv);
+    hipLaunchKernelGGL((grouped_panels<T, true, kGroupTrsmWide, kGroupTrsmWideCols>),
+                       dim3(counts[2]), dim3(256), 0, s, probs, it[2], inv);
   if (counts[3] > 0)
-    hipLaunchKernelGGL((grouped_panels<T, false, kGroupTrsmSmall>), dim3(counts[3]), dim3(256),
-                       0, su, probs, it[3], inv);
+    hipLaunchKernelGGL((grouped_panels<T, false, kGroupTrsmWide, kGroupTrsmWideCols>),
+                       dim3(counts[3]), dim3(256), 0, su, probs, it[3], inv);
   if (counts[4] > 0)
-    hipLaunchKernelGGL((grouped_panels<T, true, kGroupTrsmMax>), dim3(counts[4]), dim3(256), 0,
-                       s, probs, it[4], inv);
+    hipLaunchKernelGGL((grouped_panels<T, true, kGroupTrsmSmall, PW>), dim3(counts[4]),
+                       dim3(256), 0, s, probs, it[4], inv);
   if (counts[5] > 0)
-    hipLaunchKernelGGL((grouped_panels<T, false, kGroupTrsmMax>), dim3(counts[5]), dim3(256), 0,
-                       su, probs, it[5], inv);
+    hipLaunchKernelGGL((grouped_panels<T, false, kGroupTrsmSmall, PW>), dim3(counts[5]),
+                       dim3(256), 0, su, probs, it[5], inv);
+  if (counts[6] > 0)
+    hipLaunchKernelGGL((grouped_panels<T, true, kGroupTrsmMax, PW>), dim3(counts[6]),
+                       dim3(256), 0, s, probs, it[6], inv);
+  if (counts[7] > 0)
+    hipLaunchKernelGGL((grouped_panels<T, false, kGroupTrsmMax, PW>), dim3(counts[7]),
+                       dim3(256), 0, su, probs, it[7], inv);
   if (hipGetLastError() != hipSuccess) return MUBLIS_EHIP;
   if (two && (hipEventRecord(join, s2) != hipSuccess || hipStreamWaitEvent(s, join, 0) != hipSuccess))
     return MUBLIS_EHIP;
   return MUBLIS_OK;
 }
+// Instantiated for the batched interface's FP32 problems.
 template int gfx950_trsm_grouped<float>(const TrsmGroupProb<float> *, const TrsmGroupItem *,
                                         const int *, double *, hipStream_t, hipStream_t,
                                         hipEvent_t, hipEvent_t);
-template int gfx950_trsm_grouped<double>(const TrsmGroupProb<double> *, const TrsmGroupItem *,
-                                         const int *, double *, hipStream_t, hipStream_t,
-                                         hipEvent_t, hipEvent_t);
 
 extern "C" int mublis_gfx950_dtrsm_inverted_dev(int lower, int m, int n,
                                                 const double *a, int64_t rs_a,

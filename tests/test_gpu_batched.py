@@ -142,9 +142,10 @@ def test_autotuner_writes_and_uses_table(lib, tmp_path):
 
 
 # Grouped TRSM problems: (side, uplo, trans_a, diag, m, n, alpha).  Orders
-# (m for Left, n for Right) cover one partial 64-row block, the two panel
-# classes (<= 512 and <= 1024 rows of X in LDS) at their limits, problems
-# solved by themselves (order > 1024) and alpha = 0 (B := 0, A unread).
+# (m for Left, n for Right) cover one partial 64-row block, the three panel
+# classes (32-column panels up to order 256, 16-column panels up to 512 and
+# 1024) at their limits, problems solved by themselves (order > 1024) and
+# alpha = 0 (B := 0, A unread).
 TRSM_CASES = [
     ("L", "lo", 0, 0, 1, 1, 1.0),
     ("L", "up", 1, 1, 17, 33, -1.5),
