@@ -546,7 +546,7 @@ def main():
     args = parse()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for {args.gpus}); "
-              "multi-GPU sharding is described in DESIGN.md but not run", file=sys.stderr)
+              "multi-GPU runs are not part of it", file=sys.stderr)
         sys.exit(2)
     # The serving loop keeps 4 streams busy at once; HIP spreads a process's
     # streams over GPU_MAX_HW_QUEUES hardware queues (4 by default), so with

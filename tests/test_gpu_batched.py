@@ -1,12 +1,12 @@
-"""GPU checks of the grouped persistent batched kernel (SURVEY-named batched
-workload: uneven FP32 GEMM and SYRK problems in one hipGraph-captured
-launch), through the C ABI.
+"""GPU checks of the grouped persistent batched kernel (uneven FP32 GEMM,
+SYRK and TRSM problems in one hipGraph-captured plan), through the C ABI.
 
-Bars: every problem's result is bit-identical to the corresponding single
-cblas_sgemm / cblas_ssyrk call on the same device operands (the grouped
-kernel runs the same per-tile MFMA routine and the same K order); forced
-tile shapes give identical bits; graph replays accumulate exactly like
-repeated calls; a sample is within the normwise FP32 bound of the oracle.
+Bars: every GEMM/SYRK problem's result is bit-identical to the
+corresponding single cblas_sgemm / cblas_ssyrk call on the same device
+operands (the grouped kernel runs the same per-tile MFMA routine and the
+same K order); forced tile shapes give identical bits; graph replays
+accumulate exactly like repeated calls; a sample is within the normwise
+FP32 bound of the oracle.  TRSM problems: the bars of check_trsm below.
 """
 import ctypes
 import os
