@@ -218,8 +218,9 @@ def check_trsm(lib, torch, specs):
 def test_grouped_trsm_mixed_with_gemm_and_syrk(lib):
     """TRSM problems in one plan with GEMM and SYRK problems: the TRSM
     results meet the residual and single-call bars, the GEMM/SYRK results
-    stay bit-identical to single calls, and a graph replay solves again
-    (B restored first) to the same bits."""
+    stay bit-identical to single calls, and three more graph replays, each
+    after eager work that restores the operands and a synchronisation, solve
+    again to the same bits."""
     torch = pytest.importorskip("torch")
     gops, keep, gspecs = grouped_problems(torch, seed=11, n_gemm=60, n_syrk=16,
                                           syrk_n=(32, 300), vary_flags=True)
@@ -238,13 +239,17 @@ def test_grouped_trsm_mixed_with_gemm_and_syrk(lib):
         check_trsm(lib, torch, tspecs)
         bad = [i for i, (s, w) in enumerate(zip(gspecs, want)) if not torch.equal(s["C"], w)]
         assert not bad, f"{len(bad)} GEMM/SYRK problems differ"
-        for s in tspecs:
-            s["B"].copy_(torch.from_numpy(np.ascontiguousarray(s["b0"].T)).cuda())
-        for s, c0 in zip(gspecs, initial):
-            s["C"].copy_(c0)
-        plan.execute()
-        torch.cuda.synchronize()
-        assert all(torch.equal(s["B"], f) for s, f in zip(tspecs, first)), "replay differs"
+        gfirst = [s["C"].clone() for s in gspecs]
+        for rep in range(3):
+            for s in tspecs:
+                s["B"].copy_(torch.from_numpy(np.ascontiguousarray(s["b0"].T)).cuda())
+            for s, c0 in zip(gspecs, initial):
+                s["C"].copy_(c0)
+            torch.cuda.synchronize()
+            plan.execute()
+            torch.cuda.synchronize()
+            assert all(torch.equal(s["B"], f) for s, f in zip(tspecs, first)), f"replay {rep}"
+            assert all(torch.equal(s["C"], f) for s, f in zip(gspecs, gfirst)), f"replay {rep}"
     finally:
         plan.close()
     assert sum(info["tiles"]) == len(gops)
