@@ -640,12 +640,12 @@ __global__ __launch_bounds__(256, ROWS <= kGroupTrsmSmall ? 2 : 1) void grouped_
 
 // Widest B solved by the pipelined kernel, at most 4096; 0 disables it.
 // Below that cap a whole solve takes the pipeline for B up to 128 columns,
-// up to m * n = 2^21, or up to 768 columns and m * n = 2^23; beyond, the
-// recursive schedule's large GEMMs fill the GPU better than one chain of
-// block rows (measured on MI355X, FP64 with the folded pipeline: pipelined
-// 8192 x 512 1.54 ms vs 1.65, 6000 x 700 1.11 vs 1.48, 8192 x 200 1.27 vs
-// 2.05; recursive 8192 x 1024 2.03 ms vs 2.42, 4096 x 2048 1.04 vs 1.30,
-// 4096 x 1024 0.77 vs 0.79).  Float solves wider than 128 columns prefer
+// up to 2048 columns and m * n = 2^22, or up to 768 columns and m * n =
+// 2^23; beyond, the recursive schedule's large GEMMs fill the GPU better
+// than one chain of block rows (measured on MI355X, FP64: pipelined 8192 x
+// 512 1.38 ms vs 1.62, 8192 x 768 1.74 vs 1.89, 4096 x 1024 0.71 vs 0.79,
+// 2048 x 2048 0.41 vs 0.46, 6000 x 700 1.17 vs 1.48; recursive 8192 x 1024
+// 2.03 ms vs 2.18, 4096 x 2048 1.04 vs 1.18, 1024 x 4096 0.22 vs 0.26).  Float solves wider than 128 columns prefer
 // the recursion's GEMM leaves whenever those apply (float 8192 x 256: 0.93
 // ms vs 1.09 pipelined; 8192 x 128: pipelined 0.82 vs 0.93).  The recursive
 // schedule in turn ends in GEMM leaves (B from 256 columns for double, from
@@ -662,7 +662,8 @@ constexpr int kPanelMinN = 2048;
 bool use_pipe(int m, int n) {
   const int64_t mn = (int64_t)m * n;
   return m > NB && n <= g_pipe_max_n &&
-         (n <= 128 || mn <= (int64_t(1) << 21) || (n <= 768 && mn <= (int64_t(1) << 23)));
+         (n <= 128 || (n <= 2048 && mn <= (int64_t(1) << 22)) ||
+          (n <= 768 && mn <= (int64_t(1) << 23)));
 }
 
 int pipe_cw(int n) { return n <= 32 ? 16 : (n <= 128 ? 32 : 64); }

@@ -12,7 +12,7 @@ Bars:
 * ragged shapes: m not a multiple of the 64-row block, n not a multiple of
   the column group (16, 32 or 64 columns), a single right-hand side; the
   SHAPES are ones the default dispatch sends to the pipeline (n <= 128,
-  m n <= 2^21, or n <= 768 and m n <= 2^23 for double; float past 128
+  n <= 2048 and m n <= 2^22, or n <= 768 and m n <= 2^23 for double; float past 128
   columns takes GEMM leaves wherever they apply, and 2100 x 100 / 1100 x 72
   are float solves that the pipeline takes from the GEMM leaves), the wider
   ones go through the recursive schedule with GEMM leaves (1024-row
@@ -86,7 +86,7 @@ def residual(a, x, b0, alpha, right, lower, trans, unit, seed=0):
 
 
 SHAPES = [(65, 1), (200, 16), (257, 17), (1000, 33), (640, 128), (1100, 72), (2100, 100),
-          (1536, 200), (3000, 300)]
+          (1536, 200), (3000, 300), (2100, 1500)]
 
 
 @pytest.mark.parametrize("dt", L.DTYPES)
@@ -187,28 +187,30 @@ def test_pipe_threshold_knob(lib):
 GEMM_LEAF_CASES = [
     # (m, n, order, right): the solve's order is n on the right side; the
     # inverted blocks have 1024 rows, so the doubling's last pairs are short.
-    # Float takes GEMM leaves in all of these; double in the last two (B
-    # wider than 768 and m n > 2^21) and the pipeline in the others.
-    (1500, 300, "F", False),  # leaves 1024 + 476
-    (2600, 256, "F", False),  # 2048 + 552
-    (777, 513, "C", False),   # row-major B, pipelined leaves (m < 1024)
-    (300, 1500, "F", True),   # right side: B's transposed view, 1500-row solve
-    (257, 1111, "C", True),   # row-major, 1024 + 87
-    (128, 2100, "F", True),   # 1024 + 1024 + 52: a 52-row last block
-    (1100, 64, "F", False),   # float: GEMM leaves from n = 64 (double: pipeline)
-    (2050, 100, "C", False),  # float: 2048 + 2 rows
-    (2600, 900, "F", False),  # 2048 + 552
-    (1100, 2000, "C", True),  # right side, row-major: 1024 + 976
+    # Float takes GEMM leaves where the solve's B is wider than 128 columns
+    # and its order above 1024, the pipeline where B is narrower; double
+    # takes GEMM leaves in the last two (m n > 2^22 beyond 768 columns) and
+    # the pipeline in the others.
+    (1500, 300, "F", False),   # leaves 1024 + 476
+    (2600, 256, "F", False),   # 2048 + 552
+    (777, 513, "C", False),    # row-major B, pipelined leaves (m < 1024)
+    (300, 1500, "F", True),    # right side: B's transposed view, 1500-row solve
+    (257, 1111, "C", True),    # row-major, 1024 + 87
+    (128, 2100, "F", True),    # 1024 + 1024 + 52: a 52-row last block
+    (1100, 64, "F", False),    # pipeline for both types (64 columns)
+    (2050, 100, "C", False),   # pipeline for both types, 2048 + 2 rows
+    (2600, 1700, "F", False),  # GEMM leaves for both: 2048 + 552
+    (2100, 2100, "C", True),   # right side, row-major: 2048 + 52
 ]
 
 
 @pytest.mark.parametrize("dt", L.DTYPES)
 @pytest.mark.parametrize("m,n,order,right", GEMM_LEAF_CASES)
 def test_gemm_leaves_ragged(lib, prod, dt, m, n, order, right):
-    """GEMM leaves on ragged shapes: short last 512-row blocks (their
-    identity and pipeline rows past the end), row-major storage and the
-    right side; every uplo / trans / diag combination, against the
-    recursive schedule down to 64-row leaves and the residual probe."""
+    """The cases above on ragged shapes: short last 1024-row blocks (the
+    doubling's short last pairs), row-major storage and the right side;
+    every uplo / trans / diag combination, against the recursive schedule
+    down to 64-row leaves and the residual probe."""
     co = L.ROW if order == "C" else L.COL
     s_ = n if right else m
     rng = np.random.default_rng(m * 7 + n)
