@@ -666,7 +666,25 @@ bool use_pipe(int m, int n) {
           (n <= 768 && mn <= (int64_t(1) << 23)));
 }
 
+// Column-group width of the pipelined leaves of the recursive schedule
+// (they share the whole solve's flags, laid out for this width).
 int pipe_cw(int n) { return n <= 32 ? 16 : (n <= 128 ? 32 : 64); }
+
+// Column-group width of a whole pipelined solve: the narrowest of 16 (B up
+// to 128 columns), 32 (up to 1024) and 64 whose grid stays within 1024
+// workgroups, except that 16 also needs more than 256 of them (or B of at
+// most 32 columns).  Narrower groups shorten each step and add parallelism
+// until the A blocks every group reloads outweigh it (measured on MI355X,
+// FP64, one workgroup per CU: 8192 x 128 0.74 ms at 16 columns vs 0.80 at
+// 32; 4096 x 64 0.34 at 16 vs 0.32 at 32; 8192 x 256 0.92 at 32 vs 1.17 at
+// 64; 4096 x 512 0.51 at 32 vs 0.60; 8192 x 512 1.38 at 64 vs 1.45 at 32;
+// 2048 x 2048 0.41 at 64 vs 0.49).
+int pipe_cw_whole(int m, int n) {
+  const int64_t nbk = (m + NB - 1) / NB, g16 = nbk * ((n + 15) / 16);
+  if (n <= 32 || (n <= 128 && g16 > 256 && g16 <= 1024)) return 16;
+  if (n <= 1024 && nbk * ((n + 31) / 32) <= 1024) return 32;
+  return 64;
+}
 
 // Workspace of a solve: the inverted diagonal blocks (NB * NB doubles each,
 // row-major), then the pipeline's flags: flags[0] is the abort flag,
@@ -982,12 +1000,15 @@ int solve_ws(bool lower, int diag_mode, int m, int n, T alpha, const T *a, int64
   const bool gemm_leaves = !whole && gemm_ok;
   int *flags = (piped || gemm_leaves) ? reinterpret_cast<int *>(ad + (int64_t)nbk * NB * NB)
                                       : nullptr;
-  const int cg = (n + pipe_cw(n) - 1) / pipe_cw(n);
+  const int cw = whole ? pipe_cw_whole(m, n) : pipe_cw(n);
+  const int cg = (n + cw - 1) / cw;
   int64_t nflags = piped ? 1 + (int64_t)nbk * cg : 0;
   if (gemm_leaves) nflags = std::max(nflags, inv_flag_count<T>(m));
   int st = prep<T>(lower, m, a, rs_a, cs_a, diag_mode, ad, flags, (int)nflags, s);
   if (st) return st;
-  if (whole) return pipe<T>(lower, m, n, alpha, a, rs_a, cs_a, b, rs_b, cs_b, ad, flags, 1, s);
+  if (whole)
+    return pipe<T>(lower, m, n, alpha, a, rs_a, cs_a, b, rs_b, cs_b, ad, flags, 1, s, nullptr,
+                   cw);
   Leaves<T> lv{flags, 0, nullptr};
   if (gemm_leaves) {
     if (tmp && (rs_a == 1 || cs_a == 1))
