@@ -645,9 +645,11 @@ __global__ __launch_bounds__(256, ROWS <= kGroupTrsmSmall ? 2 : 1) void grouped_
 // than one chain of block rows (measured on MI355X, FP64: pipelined 8192 x
 // 512 1.38 ms vs 1.62, 8192 x 768 1.74 vs 1.89, 4096 x 1024 0.71 vs 0.79,
 // 2048 x 2048 0.41 vs 0.46, 6000 x 700 1.17 vs 1.48; recursive 8192 x 1024
-// 2.03 ms vs 2.18, 4096 x 2048 1.04 vs 1.18, 1024 x 4096 0.22 vs 0.26).  Float solves wider than 128 columns prefer
-// the recursion's GEMM leaves whenever those apply (float 8192 x 256: 0.93
-// ms vs 1.09 pipelined; 8192 x 128: pipelined 0.82 vs 0.93).  The recursive
+// 2.03 ms vs 2.18, 4096 x 2048 1.04 vs 1.18, 1024 x 4096 0.22 vs 0.26).
+// Float solves wider than 128 columns with m * n from 2^21 prefer the
+// recursion's GEMM leaves whenever those apply (float, leaves vs pipelined:
+// 8192 x 512 1.00 ms vs 1.33, 4096 x 512 0.47 vs 0.49; below: 8192 x 192
+// 0.80 pipelined vs 0.93, 2048 x 512 0.24 vs 0.27).  The recursive
 // schedule in turn ends in GEMM leaves (B from 256 columns for double, from
 // 64 for float; see use_inv_leaves), else hands its diagonal sub-problems of
 // at most kPipeLeaf rows to the pipeline (narrower B) or to column-panel
@@ -996,7 +998,9 @@ int solve_ws(bool lower, int diag_mode, int m, int n, T alpha, const T *a, int64
   const int nbk = (m + NB - 1) / NB;
   const bool piped = m > NB && n <= g_pipe_max_n && flags_fit(m, n);
   const bool gemm_ok = iv && w && use_inv_leaves<T>(m, n);
-  const bool whole = piped && use_pipe(m, n) && !(sizeof(T) == 4 && gemm_ok && n > 128);
+  const bool f32_leaves = sizeof(T) == 4 && gemm_ok && n > 128 &&
+                          (int64_t)m * n >= (int64_t(1) << 21);
+  const bool whole = piped && use_pipe(m, n) && !f32_leaves;
   const bool gemm_leaves = !whole && gemm_ok;
   int *flags = (piped || gemm_leaves) ? reinterpret_cast<int *>(ad + (int64_t)nbk * NB * NB)
                                       : nullptr;

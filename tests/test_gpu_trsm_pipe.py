@@ -12,12 +12,12 @@ Bars:
 * ragged shapes: m not a multiple of the 64-row block, n not a multiple of
   the column group (16, 32 or 64 columns), a single right-hand side; the
   SHAPES are ones the default dispatch sends to the pipeline (n <= 128,
-  n <= 2048 and m n <= 2^22, or n <= 768 and m n <= 2^23 for double; float past 128
-  columns takes GEMM leaves wherever they apply, and 2100 x 100 / 1100 x 72
-  are float solves that the pipeline takes from the GEMM leaves), the wider
-  ones go through the recursive schedule with GEMM leaves (1024-row
-  diagonal blocks inverted by doubling the 64-row inverses with batched
-  GEMMs).
+  n <= 2048 and m n <= 2^22, or n <= 768 and m n <= 2^23 for double; float
+  past 128 columns with m n >= 2^21 takes GEMM leaves wherever they apply,
+  so 1536 x 200 and 3000 x 300 are float pipelines too, while float
+  2100 x 1500 takes the GEMM leaves); the wider ones go through the
+  recursive schedule with GEMM leaves (1024-row diagonal blocks inverted by
+  doubling the 64-row inverses with batched GEMMs).
 """
 import ctypes
 import itertools
@@ -143,13 +143,13 @@ def test_pipelined_right_side(lib, prod, dt):
 
 
 @pytest.mark.parametrize("dt", L.DTYPES)
-@pytest.mark.parametrize("m,n", [(2000, 1100), (1100, 2100)])
+@pytest.mark.parametrize("m,n", [(2000, 2200), (1100, 2100)])
 def test_recursive_schedule_with_gemm_leaves_wide(lib, prod, dt, m, n):
     """Wide B (n >= 256): the 1024-row diagonal blocks are inverted by
     doubling the 64-row inverses with batched GEMMs and the recursion, split
-    at multiples of 1024 (2000 -> 1024 + 976; 1100 -> 1024 + 76; B wide
-    enough that double solves take this schedule too), ends in
-    GEMM leaves; compared with the recursive schedule down to 64-row leaves
+    at multiples of 1024 (2000 -> 1024 + 976; 1100 -> 1024 + 76; B wider
+    than 2048 columns, so that double solves take this schedule too), ends
+    in GEMM leaves; compared with the recursive schedule down to 64-row leaves
     (pipeline disabled) and checked by the residual probe."""
     rng = np.random.default_rng(21)
     failures = []
@@ -187,10 +187,9 @@ def test_pipe_threshold_knob(lib):
 GEMM_LEAF_CASES = [
     # (m, n, order, right): the solve's order is n on the right side; the
     # inverted blocks have 1024 rows, so the doubling's last pairs are short.
-    # Float takes GEMM leaves where the solve's B is wider than 128 columns
-    # and its order above 1024, the pipeline where B is narrower; double
-    # takes GEMM leaves in the last two (m n > 2^22 beyond 768 columns) and
-    # the pipeline in the others.
+    # Both types take GEMM leaves in the last two (float: B wider than 128
+    # columns with m n >= 2^21 and order above 1024; double: m n > 2^22
+    # beyond 768 columns) and the pipeline in the others.
     (1500, 300, "F", False),   # leaves 1024 + 476
     (2600, 256, "F", False),   # 2048 + 552
     (777, 513, "C", False),    # row-major B, pipelined leaves (m < 1024)
