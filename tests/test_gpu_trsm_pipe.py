@@ -271,3 +271,47 @@ def test_gemm_leaves_without_unit_stride(lib, dt, lower):
     diff = np.linalg.norm(outs[0].astype(np.float64) - outs[1]) / np.linalg.norm(outs[1])
     assert np.isfinite(outs[0]).all() and res <= lim, (res, lim)
     assert diff <= 50 * m * EPS[np.dtype(dt)], diff
+
+
+# Shapes at the edges of the whole-solve column-group rule (16 columns when
+# B has at most 32 columns, or at most 128 with a 16-column grid of more than
+# 256 and at most 1024 workgroups; then 32 columns up to 1024 with a grid of
+# at most 1024; else 64) and of the widened pipeline regions: (m, n, width
+# the rule picks).
+WIDTH_CASES = [
+    (4096, 64, 32),    # 16-column grid of exactly 256: 32 columns
+    (4096, 80, 16),    # 320 workgroups at 16 columns
+    (8192, 128, 16),   # exactly 1024 at 16 columns
+    (8192, 130, 32),   # past 128 columns: 32 (640 workgroups)
+    (8192, 300, 64),   # 32-column grid of 1280: 64
+    (4096, 512, 32),   # 32-column grid of exactly 1024
+    (2100, 1500, 64),  # widened double pipeline (m n <= 2^22, n <= 2048)
+    (3000, 555, 32),   # float below m n = 2^21 but wider than 128: pipeline
+]
+
+
+@pytest.mark.parametrize("dt", L.DTYPES)
+@pytest.mark.parametrize("m,n,width", WIDTH_CASES)
+def test_column_group_widths(lib, prod, dt, m, n, width):
+    """One uplo / trans / diag combination per shape (rotating), against the
+    recursive schedule down to 64-row leaves and the residual probe."""
+    i = WIDTH_CASES.index((m, n, width))
+    lower, trans, unit = bool(i & 1), bool(i & 2), bool(i & 4)
+    rng = np.random.default_rng(m + 7 * n)
+    a = L.conditioned_triangular(rng, m, dt, "F", lower, unit)
+    b0 = L.rand(rng, (m, n), dt, "F")
+    outs = []
+    for max_n in (-1, 0):
+        old = lib.mublis_gfx950_trsm_pipe_max_n(max_n)
+        try:
+            x = np.array(b0, copy=True, order="F")
+            prod.call(dt, "trsm", L.COL, L.LEFT, L.LO if lower else L.UP,
+                      L.TR if trans else L.NOTR, L.UNIT if unit else L.NONUNIT, m, n, 1.5,
+                      a, m, x, m)
+        finally:
+            lib.mublis_gfx950_trsm_pipe_max_n(old)
+        outs.append(x)
+    res, lim = residual(a, outs[0], b0, 1.5, False, lower, trans, unit)
+    diff = np.linalg.norm(outs[0].astype(np.float64) - outs[1]) / np.linalg.norm(outs[1])
+    assert np.isfinite(outs[0]).all() and res <= lim, f"residual {res:.3e} > {lim:.3e}"
+    assert diff <= 50 * m * EPS[np.dtype(dt)], f"vs 64-row leaves {diff:.3e}"
