@@ -135,6 +135,13 @@ size_t mublis_gfx950_trsm_ws_bytes(int m);
  * diagonal blocks), so 0 leaves the plain recursion with 64-row leaves.
  * Returns the previous value.  Tuning knob; the frame's calls share it. */
 int mublis_gfx950_trsm_pipe_max_n(int max_n);
+/* Polls a pipelined solve's wait makes before it gives up (default 2^20,
+ * about a second; set with spins > 0, returns the previous value).  A wait
+ * that gives up raises the solve's abort flag: every other wait then
+ * returns, and the workgroups store NaN for their blocks of X, so the call
+ * completes and the failure shows in B.  No counterpart in the reference;
+ * tests use a tiny limit to exercise that path. */
+int mublis_gfx950_trsm_pipe_spin_limit(int spins);
 int mublis_gfx950_dtrsm_ws_dev(int lower, int unit, int m, int n, double alpha,
                                const double *a, int64_t rs_a, int64_t cs_a, double *b,
                                int64_t rs_b, int64_t cs_b, void *ws, size_t ws_bytes,

@@ -144,16 +144,17 @@ __global__ __launch_bounds__(64) void trsm_inv_prep(int m, const T *__restrict__
   inv_prep_block<T, LOWER>(p, m, a, rs_a, cs_a, diag_mode, inv, LT, rd);
 }
 
-// Spin until *f == epoch (or the abort flag is up).  The loads are relaxed:
-// an acquiring load invalidates the L1 and the XCD's L2 on every iteration,
-// which would evict everyone's operands while hundreds of workgroups wait.
-// Callers order their data loads after the wait with one acquire fence.
-__device__ inline void wait_flag(int *f, int epoch, int *abort_flag) {
+// Spin until *f == epoch (or the abort flag is up; after max_spins polls the
+// wait gives up and raises it).  The loads are relaxed: an acquiring load
+// invalidates the L1 and the XCD's L2 on every iteration, which would evict
+// everyone's operands while hundreds of workgroups wait.  Callers order
+// their data loads after the wait with one acquire fence.
+__device__ inline void wait_flag(int *f, int epoch, int *abort_flag, int max_spins) {
   int spins = 0;
   while (__hip_atomic_load(f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != epoch) {
     if (__hip_atomic_load(abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
     __builtin_amdgcn_s_sleep(2);
-    if (++spins > (1 << 20)) {
+    if (++spins > max_spins) {
       __hip_atomic_store(abort_flag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       return;
     }
@@ -248,7 +249,8 @@ __global__ __launch_bounds__(256) void trsm_pipe(int m, int n, T alpha,
                                                  const T *__restrict__ a, int64_t rs_a,
                                                  int64_t cs_a, T *b, int64_t rs_b,
                                                  int64_t cs_b, const double *__restrict__ inv,
-                                                 int *flags, int cg, int epoch, PipeBatch bt) {
+                                                 int *flags, int cg, int epoch, int max_spins,
+                                                 PipeBatch bt) {
   typedef double d4 __attribute__((ext_vector_type(4)));
   constexpr int TN = CW / 16;          // 16-column MFMA tiles per wave
   constexpr int TV = PB * CW / 256;    // tile elements per thread
@@ -349,7 +351,7 @@ __global__ __launch_bounds__(256) void trsm_pipe(int m, int n, T alpha,
       load_ablk<T>(av, a, rs_a, cs_a, m, arow, q, lk, prow && (q + 1) * PB <= m);
     }
     if (tid == 0) {
-      wait_flag(&flags[i * cg + cgi], epoch, abort_flag);
+      wait_flag(&flags[i * cg + cgi], epoch, abort_flag, max_spins);
       int f = i + 1;
       while (f < jend &&
              __hip_atomic_load(&flags[f * cg + cgi], __ATOMIC_RELAXED,
@@ -407,7 +409,7 @@ __global__ __launch_bounds__(256) void trsm_pipe(int m, int n, T alpha,
                                                     out[t], 0, 0, 0);
   if (fold) {
     // X(p) = Y - M X(ql): wait for X(ql), the only step after the wait
-    if (tid == 0) wait_flag(&flags[(j - 1) * cg + cgi], epoch, abort_flag);
+    if (tid == 0) wait_flag(&flags[(j - 1) * cg + cgi], epoch, abort_flag, max_spins);
     __syncthreads();  // also: every wave is past its Y MFMAs on xs[0]
     // acquire: X(ql) was published with a release store
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
@@ -426,13 +428,19 @@ __global__ __launch_bounds__(256) void trsm_pipe(int m, int n, T alpha,
 #pragma unroll
     for (int r = 0; r < 4; ++r) xs[1][16 * wave + lk + 4 * r][16 * t + lr] = out[t][r];
   __syncthreads();
-  // store X(p) along B's unit stride, then publish it
+  // store X(p) along B's unit stride, then publish it.  After a timed-out
+  // wait (abort flag up) X is not valid: NaN is stored instead, so that the
+  // failure shows in the result rather than as plausible wrong values.
+  const bool aborted =
+      __hip_atomic_load(abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0;
 #pragma unroll
   for (int it = 0; it < TV; ++it) {
     int r, cc;
     tile_rc<CW>(it * 256 + tid, cm, r, cc);
     const int gr = p * PB + r, gc = c0 + cc;
-    if (gr < m && gc < n) b[(int64_t)gr * rs_b + (int64_t)gc * cs_b] = (T)xs[1][r][cc];
+    if (gr < m && gc < n)
+      b[(int64_t)gr * rs_b + (int64_t)gc * cs_b] =
+          aborted ? T(__builtin_nan("")) : (T)xs[1][r][cc];
   }
   // Every wave's stores are complete at the barrier; thread 0's release
   // store then publishes them all (release is cumulative over the
@@ -656,6 +664,10 @@ __global__ __launch_bounds__(256, ROWS <= kGroupTrsmSmall ? 2 : 1) void grouped_
 // workgroups (wider B, when the solve brings no room for the GEMM leaves'
 // inverses).
 int g_pipe_max_n = 4096;
+// Polls a pipeline wait makes before it gives up (about a second; the
+// dispatch order makes every wait end long before, this only bounds a
+// fault).
+int g_pipe_spins = 1 << 20;
 constexpr int kPipeLeaf = 512;
 // From this many columns up, the pipelined leaves of at most kPanelRows
 // rows are column panels (n / 16 independent workgroups) instead.
@@ -736,7 +748,8 @@ int pipe_launch(int m, int n, T alpha, const T *a, int64_t rs_a, int64_t cs_a, T
   // the kernel's own occupancy.
   const size_t lds = nz == 1 ? kLdsOneWg - sizeof(double) * pipe_lds_doubles(CW) : 0;
   hipLaunchKernelGGL((trsm_pipe<T, LOWER, CW>), dim3(nbk * cg, nz), dim3(256), lds, s, m, n,
-                     alpha, a, rs_a, cs_a, b, rs_b, cs_b, inv, flags, cg, epoch, bt);
+                     alpha, a, rs_a, cs_a, b, rs_b, cs_b, inv, flags, cg, epoch, g_pipe_spins,
+                     bt);
   return hipGetLastError() == hipSuccess ? MUBLIS_OK : MUBLIS_EHIP;
 }
 
@@ -1177,6 +1190,12 @@ extern "C" int mublis_gfx950_strsm_ws_dev(int lower, int unit, int m, int n, flo
                                           void *stream) {
   return trsm_ws_dev<float>(lower, unit, m, n, alpha, a, rs_a, cs_a, b, rs_b, cs_b, ws,
                             ws_bytes, stream);
+}
+
+extern "C" int mublis_gfx950_trsm_pipe_spin_limit(int spins) {
+  const int old = g_pipe_spins;
+  if (spins > 0) g_pipe_spins = spins;
+  return old;
 }
 
 extern "C" int mublis_gfx950_trsm_pipe_max_n(int max_n) {

@@ -315,3 +315,31 @@ def test_column_group_widths(lib, prod, dt, m, n, width):
     diff = np.linalg.norm(outs[0].astype(np.float64) - outs[1]) / np.linalg.norm(outs[1])
     assert np.isfinite(outs[0]).all() and res <= lim, f"residual {res:.3e} > {lim:.3e}"
     assert diff <= 50 * m * EPS[np.dtype(dt)], f"vs 64-row leaves {diff:.3e}"
+
+
+def test_timed_out_wait_leaves_nan(lib, prod):
+    """Failure handling of the pipeline's bounded waits, forced from the host:
+    with the spin limit at 2 polls the 64-step chain of a 4096 x 64 solve
+    gives up (every wait still ends and the kernel exits normally), the
+    call completes and B holds NaN instead of plausible values; the next
+    solve with the default limit meets the residual bound again."""
+    m, n = 4096, 64
+    rng = np.random.default_rng(99)
+    a = L.conditioned_triangular(rng, m, np.float64, "F", True, False)
+    b0 = L.rand(rng, (m, n), np.float64, "F")
+
+    def solve():
+        x = np.array(b0, copy=True, order="F")
+        prod.call(np.float64, "trsm", L.COL, L.LEFT, L.LO, L.NOTR, L.NONUNIT, m, n, 1.0, a,
+                  m, x, m)
+        return x
+
+    old = lib.mublis_gfx950_trsm_pipe_spin_limit(2)
+    try:
+        x = solve()
+    finally:
+        assert lib.mublis_gfx950_trsm_pipe_spin_limit(old) == 2
+    assert np.isnan(x).any(), "a given-up wait must show in the result"
+    x = solve()
+    res, lim = residual(a, x, b0, 1.0, False, True, False, False)
+    assert np.isfinite(x).all() and res <= lim
