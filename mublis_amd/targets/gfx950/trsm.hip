@@ -1007,7 +1007,7 @@ int trsm_rec(bool lower, const double *ad, const T *iv, int m, int n, T alpha, c
 template <typename T>
 int solve_ws(bool lower, int diag_mode, int m, int n, T alpha, const T *a, int64_t rs_a,
              int64_t cs_a, T *b, int64_t rs_b, int64_t cs_b, double *ad, T *iv, T *w, T *tmp,
-             hipStream_t s) {
+             hipStream_t s, bool *inv_ready = nullptr) {
   const int nbk = (m + NB - 1) / NB;
   const bool piped = m > NB && n <= g_pipe_max_n && flags_fit(m, n);
   const bool gemm_ok = iv && w && use_inv_leaves<T>(m, n);
@@ -1021,6 +1021,12 @@ int solve_ws(bool lower, int diag_mode, int m, int n, T alpha, const T *a, int64
   const int cg = (n + cw - 1) / cw;
   int64_t nflags = piped ? 1 + (int64_t)nbk * cg : 0;
   if (gemm_leaves) nflags = std::max(nflags, inv_flag_count<T>(m));
+  if (gemm_leaves && inv_ready && *inv_ready) {
+    // the inverses of this triangle are in iv from an earlier solve: GEMM
+    // leaves read nothing else
+    Leaves<T> lv{flags, 1, w};
+    return trsm_rec<T>(lower, ad, iv, m, n, alpha, a, rs_a, cs_a, b, rs_b, cs_b, lv, s);
+  }
   int st = prep<T>(lower, m, a, rs_a, cs_a, diag_mode, ad, flags, (int)nflags, s);
   if (st) return st;
   if (whole)
@@ -1035,6 +1041,7 @@ int solve_ws(bool lower, int diag_mode, int m, int n, T alpha, const T *a, int64
     if (st) return st;
     lv.epoch = 1;
     lv.w = w;
+    if (inv_ready) *inv_ready = true;
   }
   return trsm_rec<T>(lower, ad, gemm_leaves ? iv : nullptr, m, n, alpha, a, rs_a, cs_a, b,
                      rs_b, cs_b, lv, s);
@@ -1190,6 +1197,43 @@ extern "C" int mublis_gfx950_strsm_ws_dev(int lower, int unit, int m, int n, flo
                                           void *stream) {
   return trsm_ws_dev<float>(lower, unit, m, n, alpha, a, rs_a, cs_a, b, rs_b, cs_b, ws,
                             ws_bytes, stream);
+}
+
+// FP32 solves against one triangle that keep its inverted diagonal blocks
+// in caller workspace between calls (the refined STRSM's three solves).
+// Layout: the 64-row inverses and flags, then, when the solve takes GEMM
+// leaves, the 1024-row inverses, the leaf copy of B and the doubling's
+// scratch, each 256-byte aligned.
+static size_t keep_part(size_t bytes) { return (bytes + 255) / 256 * 256; }
+
+size_t gfx950_strsm_keep_ws_bytes(int m, int n) {
+  size_t bytes = keep_part(sizeof(double) * (size_t)ws_doubles(m));
+  if (use_inv_leaves<float>(m, n))
+    bytes += keep_part(sizeof(float) * (size_t)inv_leaf_elems<float>(m)) +
+             keep_part(sizeof(float) * (size_t)kInvLeaf<float> * (size_t)n) +
+             keep_part(sizeof(float) * (size_t)m * kInvLeaf<float> / 4);
+  return bytes;
+}
+
+int gfx950_strsm_keep(bool lower, bool unit, int m, int n, float alpha, const float *a,
+                      int64_t rs_a, int64_t cs_a, float *b, int64_t rs_b, int64_t cs_b,
+                      void *ws, bool *inv_ready, hipStream_t s) {
+  if (m < 0 || n < 0 || !ws) return MUBLIS_EINVAL;
+  if (m == 0 || n == 0) return MUBLIS_OK;
+  if (alpha == 0.0f) return gfx950_fill<float>(m, n, 0.0f, b, rs_b, cs_b, s);
+  char *p = (char *)ws;
+  double *ad = (double *)p;
+  p += keep_part(sizeof(double) * (size_t)ws_doubles(m));
+  float *iv = nullptr, *w = nullptr, *tmp = nullptr;
+  if (use_inv_leaves<float>(m, n)) {
+    iv = (float *)p;
+    p += keep_part(sizeof(float) * (size_t)inv_leaf_elems<float>(m));
+    w = (float *)p;
+    p += keep_part(sizeof(float) * (size_t)kInvLeaf<float> * (size_t)n);
+    tmp = (float *)p;
+  }
+  return solve_ws<float>(lower, unit ? 1 : 0, m, n, alpha, a, rs_a, cs_a, b, rs_b, cs_b, ad,
+                         iv, w, tmp, s, inv_ready);
 }
 
 extern "C" int mublis_gfx950_trsm_pipe_spin_limit(int spins) {
