@@ -33,9 +33,17 @@ namespace {
 
 using namespace gfx950;
 
+// K slice depth and workgroups per CU of the one-problem kernel.
+template <typename T, int TILE>
+struct KernelShape {
+  static constexpr int BK = Mfma<T>::BK;
+  static constexpr int WG_PER_CU = Tile<TILE>::MIN_WG_PER_CU;
+};
+
 template <typename T, int TILE, bool A_KC, bool B_NC>
-__global__ __launch_bounds__(NT, Tile<TILE>::MIN_WG_PER_CU) void gemm_kernel(GemmArgs<T> p) {
-  __shared__ T lds[LdsSize<T, TILE>::value];
+__global__ __launch_bounds__(NT, (KernelShape<T, TILE>::WG_PER_CU)) void gemm_kernel(GemmArgs<T> p) {
+  constexpr int BK = KernelShape<T, TILE>::BK;
+  __shared__ T lds[LdsSize<T, TILE, BK>::value];
   if (gridDim.y > 1) {
     const int64_t zi = blockIdx.y % p.bat, zo = blockIdx.y / p.bat;
     p.a += zi * p.bsa[0] + zo * p.bsa[1];
@@ -49,7 +57,7 @@ __global__ __launch_bounds__(NT, Tile<TILE>::MIN_WG_PER_CU) void gemm_kernel(Gem
     map_tile(blockIdx.x, p.ntiles, p.tiles_m, p.tiles_n, p.tri, p.ji, tm, tn);
     if (tm >= p.tiles_m || tn >= p.tiles_n) return;  // mode 3 past an odd last tile
   }
-  gemm_tile<T, TILE, A_KC, B_NC>(p, tm, tn, lds);
+  gemm_tile<T, TILE, A_KC, B_NC, BK>(p, tm, tn, lds);
 }
 
 // beta-only update on the ji part of C (alpha == 0 or k == 0): the

@@ -160,6 +160,8 @@ template <typename T, int LC, int LO, int S>
 struct Slice {
   static constexpr int V = 16 / sizeof(T);
   static constexpr int NV = LC * LO / (V * NT);
+  static_assert(NV >= 1 && NV * V * NT == LC * LO, "every thread loads whole vectors");
+  static_assert(LC % V == 0, "a vector stays within one row");
   T v[NV][V];
 
   __device__ inline void load(const T *__restrict__ x, int64_t ld, int c0, int o0,
@@ -212,9 +214,8 @@ struct Slice {
   }
 };
 
-template <typename T, int TILE, bool A_KC, bool B_NC>
+template <typename T, int TILE, bool A_KC, bool B_NC, int BK>
 struct Operands {
-  static constexpr int BK = Mfma<T>::BK;
   static constexpr int BM = Tile<TILE>::BM, BN = Tile<TILE>::BN, S = Tile<TILE>::S;
   // A is m x k: K-contiguous -> (c = k, o = m); else (c = m, o = k).
   typedef Slice<T, A_KC ? BK : BM, A_KC ? BM : BK, S> SA;
@@ -222,9 +223,9 @@ struct Operands {
   typedef Slice<T, B_NC ? BN : BK, B_NC ? BK : BN, S> SB;
 };
 
-template <typename T, int TILE, bool A_KC, bool B_NC>
-__device__ inline void load_slices(typename Operands<T, TILE, A_KC, B_NC>::SA &ra,
-                                   typename Operands<T, TILE, A_KC, B_NC>::SB &rb,
+template <typename T, int TILE, bool A_KC, bool B_NC, int BK>
+__device__ inline void load_slices(typename Operands<T, TILE, A_KC, B_NC, BK>::SA &ra,
+                                   typename Operands<T, TILE, A_KC, B_NC, BK>::SB &rb,
                                    const T *a, int64_t lda, const T *b,
                                    int64_t ldb, int m0, int n0, int k0, int m,
                                    int n, int k, bool va, bool vb, int tid) {
@@ -234,23 +235,27 @@ __device__ inline void load_slices(typename Operands<T, TILE, A_KC, B_NC>::SA &r
   else rb.load(b, ldb, k0, n0, k, n, vb, tid);
 }
 
-// LDS elements one workgroup needs for a TILE x TILE tile of type T.
-template <typename T, int TILE>
+// LDS elements one workgroup needs for a TILE x TILE tile of type T staged
+// in K slices of BK.
+template <typename T, int TILE, int BK = Mfma<T>::BK>
 struct LdsSize {
-  static constexpr int value = 4 * Mfma<T>::BK * Tile<TILE>::S;
+  static constexpr int value = 4 * BK * Tile<TILE>::S;
 };
 
 // One TILE x TILE output tile (tm, tn) of the product described by p,
 // computed by the calling workgroup of NT threads with `lds` (LdsSize
 // elements) as staging memory.  The caller synchronises before reusing lds.
-template <typename T, int TILE, bool A_KC, bool B_NC>
+// BK, the K depth of one staged slice, sets only how K is cut into slices:
+// every depth runs the same MFMA steps in the same k order (a slice past k
+// adds exact zeros), so the result does not depend on it.
+template <typename T, int TILE, bool A_KC, bool B_NC, int BK = Mfma<T>::BK>
 __device__ inline void gemm_tile(const GemmArgs<T> &p, int tm, int tn, T *lds) {
   typedef Mfma<T> MF;
   typedef typename MF::acc_t acc_t;
-  constexpr int BK = MF::BK;
+  static_assert(BK % 4 == 0 && BK <= Mfma<T>::BK, "whole MFMA k steps, swizzle range");
   constexpr int BM = Tile<TILE>::BM, BN = Tile<TILE>::BN, S = Tile<TILE>::S;
   constexpr int TI = Tile<TILE>::TI, WT = TILE / 2;
-  typedef Operands<T, TILE, A_KC, B_NC> Ops;
+  typedef Operands<T, TILE, A_KC, B_NC, BK> Ops;
   T(*As)[BK * S] = reinterpret_cast<T(*)[BK * S]>(lds);
   T(*Bs)[BK * S] = reinterpret_cast<T(*)[BK * S]>(lds + 2 * BK * S);
   const int m0 = tm * BM, n0 = tn * BN;
@@ -286,7 +291,7 @@ __device__ inline void gemm_tile(const GemmArgs<T> &p, int tm, int tn, T *lds) {
     const int ktiles = kend > kbeg ? (kend + BK - 1) / BK - kt0 : 0;
     if (ktiles <= 0) continue;
 
-    load_slices<T, TILE, A_KC, B_NC>(ra, rb, a, lda, b, ldb, m0, n0, kt0 * BK, p.m, p.n,
+    load_slices<T, TILE, A_KC, B_NC, BK>(ra, rb, a, lda, b, ldb, m0, n0, kt0 * BK, p.m, p.n,
                                      p.k, va, vb, tid);
     __syncthreads();  // previous segment's readers are done with buffer 0
     ra.template store<A_KC>(As[0], tid);
@@ -298,7 +303,7 @@ __device__ inline void gemm_tile(const GemmArgs<T> &p, int tm, int tn, T *lds) {
       const int cur = kt & 1;
       const bool more = kt + 1 < ktiles;
       if (more)
-        load_slices<T, TILE, A_KC, B_NC>(ra, rb, a, lda, b, ldb, m0, n0,
+        load_slices<T, TILE, A_KC, B_NC, BK>(ra, rb, a, lda, b, ldb, m0, n0,
                                          (kt0 + kt + 1) * BK, p.m, p.n, p.k, va, vb, tid);
       const T *as = As[cur];
       const T *bs = Bs[cur];
