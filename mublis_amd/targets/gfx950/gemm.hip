@@ -33,16 +33,28 @@ namespace {
 
 using namespace gfx950;
 
-// K slice depth and workgroups per CU of the one-problem kernel.
-template <typename T, int TILE>
+// K slice depth and workgroups per CU of the one-problem kernel.  SHALLOW
+// (FP64 64-tiles only) halves the slice, which halves the LDS ring to
+// 20 KiB and the body to 78 VGPRs, for 6 workgroups per CU instead of 4.
+// Measured on MI355X: plain mid-size DGEMMs 2-10 % faster (2048 x 3072 x
+// 1024: 49.4 -> 54.4 TFLOP/s; 1536 x 4096 x 1536: 52.0 -> 56.3) and DSYRK
+// 3072 47.0 -> 52.7 TFLOP/s, but launches with a triangular operand (long-
+// first walk, the TRSM leaves) 13 % slower, and grids of fewer than four
+// workgroups per CU (the few long SYRK diagonal tiles, the smaller updates
+// of a TRSM recursion), which gain no residency, slower too: those keep the
+// deeper slice.
+constexpr int64_t kShallowMinWgs = 1024;
+template <typename T, int TILE, bool SHALLOW>
 struct KernelShape {
-  static constexpr int BK = Mfma<T>::BK;
-  static constexpr int WG_PER_CU = Tile<TILE>::MIN_WG_PER_CU;
+  static_assert(!SHALLOW || (TILE == 64 && sizeof(T) == 8), "FP64 64-tiles only");
+  static constexpr int BK = SHALLOW ? Mfma<T>::BK / 2 : Mfma<T>::BK;
+  static constexpr int WG_PER_CU = SHALLOW ? 6 : Tile<TILE>::MIN_WG_PER_CU;
 };
 
-template <typename T, int TILE, bool A_KC, bool B_NC>
-__global__ __launch_bounds__(NT, (KernelShape<T, TILE>::WG_PER_CU)) void gemm_kernel(GemmArgs<T> p) {
-  constexpr int BK = KernelShape<T, TILE>::BK;
+template <typename T, int TILE, bool A_KC, bool B_NC, bool SHALLOW>
+__global__ __launch_bounds__(NT, (KernelShape<T, TILE, SHALLOW>::WG_PER_CU)) void gemm_kernel(
+    GemmArgs<T> p) {
+  constexpr int BK = KernelShape<T, TILE, SHALLOW>::BK;
   __shared__ T lds[LdsSize<T, TILE, BK>::value];
   if (gridDim.y > 1) {
     const int64_t zi = blockIdx.y % p.bat, zo = blockIdx.y / p.bat;
@@ -82,6 +94,19 @@ bool aligned16(const T *x, int64_t ld) {
   return ((uintptr_t)x % 16 == 0) && ((ld * (int64_t)sizeof(T)) % 16 == 0);
 }
 
+template <typename T, int TILE, bool SHALLOW>
+void launch_layout(const GemmArgs<T> &p, bool akc, bool bnc, dim3 grid, hipStream_t s) {
+  const dim3 block(NT);
+  if (akc && bnc)
+    hipLaunchKernelGGL((gemm_kernel<T, TILE, true, true, SHALLOW>), grid, block, 0, s, p);
+  else if (akc)
+    hipLaunchKernelGGL((gemm_kernel<T, TILE, true, false, SHALLOW>), grid, block, 0, s, p);
+  else if (bnc)
+    hipLaunchKernelGGL((gemm_kernel<T, TILE, false, true, SHALLOW>), grid, block, 0, s, p);
+  else
+    hipLaunchKernelGGL((gemm_kernel<T, TILE, false, false, SHALLOW>), grid, block, 0, s, p);
+}
+
 template <typename T, int TILE>
 hipError_t launch(GemmArgs<T> p, bool akc, bool bnc, hipStream_t s, int nbatch = 1) {
   p.tiles_m = (p.m + TILE - 1) / TILE;
@@ -100,11 +125,14 @@ hipError_t launch(GemmArgs<T> p, bool akc, bool bnc, hipStream_t s, int nbatch =
     const int64_t units = (int64_t)lines * ((per + p.lpt_g - 1) / p.lpt_g);
     wgs = (int)(8 * ((units + 7) / 8) * p.lpt_g);
   }
-  const dim3 grid(wgs, nbatch), block(NT);
-  if (akc && bnc) hipLaunchKernelGGL((gemm_kernel<T, TILE, true, true>), grid, block, 0, s, p);
-  else if (akc) hipLaunchKernelGGL((gemm_kernel<T, TILE, true, false>), grid, block, 0, s, p);
-  else if (bnc) hipLaunchKernelGGL((gemm_kernel<T, TILE, false, true>), grid, block, 0, s, p);
-  else hipLaunchKernelGGL((gemm_kernel<T, TILE, false, false>), grid, block, 0, s, p);
+  const dim3 grid(wgs, nbatch);
+  if constexpr (TILE == 64 && sizeof(T) == 8) {
+    if (!p.lpt && p.tri != 3 && (int64_t)wgs * nbatch >= kShallowMinWgs) {
+      launch_layout<T, TILE, true>(p, akc, bnc, grid, s);
+      return hipGetLastError();
+    }
+  }
+  launch_layout<T, TILE, false>(p, akc, bnc, grid, s);
   return hipGetLastError();
 }
 
