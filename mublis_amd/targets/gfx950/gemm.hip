@@ -34,19 +34,21 @@ namespace {
 using namespace gfx950;
 
 // K slice depth and workgroups per CU of the one-problem kernel.  SHALLOW
-// (FP64 64-tiles only) halves the slice, which halves the LDS ring to
-// 20 KiB and the body to 78 VGPRs, for 6 workgroups per CU instead of 4.
-// Measured on MI355X: plain mid-size DGEMMs 2-10 % faster (2048 x 3072 x
-// 1024: 49.4 -> 54.4 TFLOP/s; 1536 x 4096 x 1536: 52.0 -> 56.3) and DSYRK
-// 3072 47.0 -> 52.7 TFLOP/s, but launches with a triangular operand (long-
-// first walk, the TRSM leaves) 13 % slower, and grids of fewer than four
-// workgroups per CU (the few long SYRK diagonal tiles, the smaller updates
-// of a TRSM recursion), which gain no residency, slower too: those keep the
-// deeper slice.
-constexpr int64_t kShallowMinWgs = 1024;
+// (64-tiles only) halves the slice, which halves the LDS ring to 20 KiB and
+// the body to 78 (FP64) / 57 (FP32) VGPRs, for 6 workgroups per CU instead
+// of 4.  Measured on MI355X: plain mid-size DGEMMs 2-10 % faster (2048 x
+// 3072 x 1024: 49.4 -> 54.4 TFLOP/s; 1536 x 4096 x 1536: 52.0 -> 56.3),
+// DSYRK 3072 47.0 -> 52.7, SGEMM 2048 x 3072 x 1024 85.9 -> 92.2, SSYRK
+// 3072 83.4 -> 93.5; but launches with a triangular operand (long-first
+// walk, the TRSM leaves) 13 % slower, and grids that fit four workgroups
+// per CU (FP64: fewer than four; the few long SYRK diagonal tiles, the
+// smaller updates of a TRSM recursion), which gain no residency, slower
+// too: those keep the deeper slice.
+template <typename T>
+constexpr int64_t shallow_min_wgs() { return sizeof(T) == 8 ? 1024 : 1025; }
 template <typename T, int TILE, bool SHALLOW>
 struct KernelShape {
-  static_assert(!SHALLOW || (TILE == 64 && sizeof(T) == 8), "FP64 64-tiles only");
+  static_assert(!SHALLOW || TILE == 64, "64-tiles only");
   static constexpr int BK = SHALLOW ? Mfma<T>::BK / 2 : Mfma<T>::BK;
   static constexpr int WG_PER_CU = SHALLOW ? 6 : Tile<TILE>::MIN_WG_PER_CU;
 };
@@ -126,8 +128,8 @@ hipError_t launch(GemmArgs<T> p, bool akc, bool bnc, hipStream_t s, int nbatch =
     wgs = (int)(8 * ((units + 7) / 8) * p.lpt_g);
   }
   const dim3 grid(wgs, nbatch);
-  if constexpr (TILE == 64 && sizeof(T) == 8) {
-    if (!p.lpt && p.tri != 3 && (int64_t)wgs * nbatch >= kShallowMinWgs) {
+  if constexpr (TILE == 64) {
+    if (!p.lpt && p.tri != 3 && (int64_t)wgs * nbatch >= shallow_min_wgs<T>()) {
       launch_layout<T, TILE, true>(p, akc, bnc, grid, s);
       return hipGetLastError();
     }
