@@ -34,23 +34,32 @@ namespace {
 using namespace gfx950;
 
 // K slice depth and workgroups per CU of the one-problem kernel.  SHALLOW
-// (64-tiles only) halves the slice, which halves the LDS ring to 20 KiB and
-// the body to 78 (FP64) / 57 (FP32) VGPRs, for 6 workgroups per CU instead
-// of 4.  Measured on MI355X: plain mid-size DGEMMs 2-10 % faster (2048 x
-// 3072 x 1024: 49.4 -> 54.4 TFLOP/s; 1536 x 4096 x 1536: 52.0 -> 56.3),
-// DSYRK 3072 47.0 -> 52.7, SGEMM 2048 x 3072 x 1024 85.9 -> 92.2, SSYRK
-// 3072 83.4 -> 93.5; but launches with a triangular operand (long-first
-// walk, the TRSM leaves) 13 % slower, and grids that fit four workgroups
-// per CU (FP64: fewer than four; the few long SYRK diagonal tiles, the
-// smaller updates of a TRSM recursion), which gain no residency, slower
-// too: those keep the deeper slice.
-template <typename T>
-constexpr int64_t shallow_min_wgs() { return sizeof(T) == 8 ? 1024 : 1025; }
+// halves the slice, which halves the LDS ring and frees the operand
+// registers for more resident workgroups: 64-tiles 20 KiB and 78 (FP64) / 57
+// (FP32) VGPRs at 6 per CU instead of 4; FP32 128-tiles 36 KiB and 127 VGPRs
+// at 4 per CU instead of 2 (FP64 128-tiles would spill at 3).  Measured on
+// MI355X (TFLOP/s): SGEMM 8192 op(A) = A^T 126.3 -> 129.8, NN 123.5 ->
+// 129.1, 4096 119.8 -> 125, 3072 x 3072 x 1024 74 -> 81, 8192 x 8192 x
+// 1024 112 -> 121; DGEMM 2048 x 3072 x 1024 49.4 -> 54.4, 1536 x 4096 x
+// 1536 52.0 -> 56.3; DSYRK 3072 47.0 -> 52.7; SSYRK 3072 83.4 -> 93.5.  But
+// launches with a triangular operand (long-first walk, the TRSM leaves) ran
+// 13 % slower, and grids that the deep kernel already holds in one round
+// (FP64 64-tiles: fewer than four per CU; FP32: four per CU or two for
+// 128-tiles; the few long SYRK diagonal tiles, the smaller updates of a
+// TRSM recursion) gain no residency and lose 1-2 %: those keep the deep
+// slice.  The slice depth never changes the k order of the MFMAs.
+template <typename T, int TILE>
+bool use_shallow(const GemmArgs<T> &p, int64_t wgs) {
+  if (p.lpt) return false;
+  if (TILE == 64) return p.tri != 3 && wgs >= (sizeof(T) == 8 ? 1024 : 1025);
+  return sizeof(T) == 4 && wgs > 512;
+}
 template <typename T, int TILE, bool SHALLOW>
 struct KernelShape {
-  static_assert(!SHALLOW || TILE == 64, "64-tiles only");
+  static_assert(!SHALLOW || TILE == 64 || sizeof(T) == 4, "FP64 128-tiles: deep only");
   static constexpr int BK = SHALLOW ? Mfma<T>::BK / 2 : Mfma<T>::BK;
-  static constexpr int WG_PER_CU = SHALLOW ? 6 : Tile<TILE>::MIN_WG_PER_CU;
+  static constexpr int WG_PER_CU =
+      SHALLOW ? (TILE == 64 ? 6 : 4) : Tile<TILE>::MIN_WG_PER_CU;
 };
 
 template <typename T, int TILE, bool A_KC, bool B_NC, bool SHALLOW>
@@ -128,8 +137,8 @@ hipError_t launch(GemmArgs<T> p, bool akc, bool bnc, hipStream_t s, int nbatch =
     wgs = (int)(8 * ((units + 7) / 8) * p.lpt_g);
   }
   const dim3 grid(wgs, nbatch);
-  if constexpr (TILE == 64) {
-    if (!p.lpt && p.tri != 3 && (int64_t)wgs * nbatch >= shallow_min_wgs<T>()) {
+  if constexpr (TILE == 64 || sizeof(T) == 4) {
+    if (use_shallow<T, TILE>(p, (int64_t)wgs * nbatch)) {
       launch_layout<T, TILE, true>(p, akc, bnc, grid, s);
       return hipGetLastError();
     }
