@@ -7,9 +7,10 @@
 //   pc + packm         -> the K loop: 128 x BK slices of A and B are loaded
 //                         from HBM with 16-byte vector loads and staged in
 //                         LDS ([k][m] and [k][n], double-buffered);
-//   jr / ir            -> 4 wavefronts in a 2 x 2 layout, each owning 64 x 64;
+//   jr / ir            -> 4 wavefronts in a 2 x 2 layout, each owning 64 x 64
+//                         (FP64 128-tiles: 8 in 2 x 4, each 64 x 32);
 //   micro-kernel p loop-> chains of v_mfma_f64_16x16x4_f64 (FP64) or
-//                         v_mfma_f32_16x16x4_f32 (FP32), 16 tiles per wave;
+//                         v_mfma_f32_16x16x4_f32 (FP32), 16 (8) tiles per wave;
 //   beta pre-pass, c_buf edge tiles, ji masks
 //                      -> one predicated epilogue: C = alpha*acc + beta*C,
 //                         beta == 0 never reads C, elements outside the ji
@@ -54,16 +55,26 @@ bool use_shallow(const GemmArgs<T> &p, int64_t wgs) {
   if (TILE == 64) return p.tri != 3 && wgs >= (sizeof(T) == 8 ? 1024 : 1025);
   return sizeof(T) == 4 && wgs > 512;
 }
+//
+// FP64 128-tiles run 8 waves (512 threads, each wave 64 x 32) instead of 4:
+// the same 2 workgroups per CU hold 4 waves per SIMD instead of 2, at 116-128
+// VGPRs.  Measured on MI355X (TFLOP/s): DGEMM 8192 op(A) = A^T 63.4 -> 65.5,
+// NN 63.0 -> 65.5, 4096 62.2 -> 64.5, DSYRK 8192 60.0 -> 63-64; DTRSM 8192 x
+// 4096 5.22 -> 5.09 ms.
 template <typename T, int TILE, bool SHALLOW>
 struct KernelShape {
   static_assert(!SHALLOW || TILE == 64 || sizeof(T) == 4, "FP64 128-tiles: deep only");
   static constexpr int BK = SHALLOW ? Mfma<T>::BK / 2 : Mfma<T>::BK;
   static constexpr int WG_PER_CU =
       SHALLOW ? (TILE == 64 ? 6 : 4) : Tile<TILE>::MIN_WG_PER_CU;
+  static constexpr int NTH = (sizeof(T) == 8 && TILE == 128) ? 512 : NT;
+  // __launch_bounds__'s second argument: minimum waves per SIMD
+  static constexpr int WAVES_PER_SIMD = WG_PER_CU * NTH / 256;
 };
 
 template <typename T, int TILE, bool A_KC, bool B_NC, bool SHALLOW>
-__global__ __launch_bounds__(NT, (KernelShape<T, TILE, SHALLOW>::WG_PER_CU)) void gemm_kernel(
+__global__ __launch_bounds__((KernelShape<T, TILE, SHALLOW>::NTH),
+                             (KernelShape<T, TILE, SHALLOW>::WAVES_PER_SIMD)) void gemm_kernel(
     GemmArgs<T> p) {
   constexpr int BK = KernelShape<T, TILE, SHALLOW>::BK;
   __shared__ T lds[LdsSize<T, TILE, BK>::value];
@@ -80,7 +91,7 @@ __global__ __launch_bounds__(NT, (KernelShape<T, TILE, SHALLOW>::WG_PER_CU)) voi
     map_tile(blockIdx.x, p.ntiles, p.tiles_m, p.tiles_n, p.tri, p.ji, tm, tn);
     if (tm >= p.tiles_m || tn >= p.tiles_n) return;  // mode 3 past an odd last tile
   }
-  gemm_tile<T, TILE, A_KC, B_NC, BK>(p, tm, tn, lds);
+  gemm_tile<T, TILE, A_KC, B_NC, BK, KernelShape<T, TILE, SHALLOW>::NTH>(p, tm, tn, lds);
 }
 
 // beta-only update on the ji part of C (alpha == 0 or k == 0): the
@@ -107,7 +118,7 @@ bool aligned16(const T *x, int64_t ld) {
 
 template <typename T, int TILE, bool SHALLOW>
 void launch_layout(const GemmArgs<T> &p, bool akc, bool bnc, dim3 grid, hipStream_t s) {
-  const dim3 block(NT);
+  const dim3 block(KernelShape<T, TILE, SHALLOW>::NTH);
   if (akc && bnc)
     hipLaunchKernelGGL((gemm_kernel<T, TILE, true, true, SHALLOW>), grid, block, 0, s, p);
   else if (akc)

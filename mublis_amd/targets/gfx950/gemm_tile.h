@@ -40,16 +40,16 @@ template <> struct Mfma<float> {
   __device__ static inline int swz(int k) { return k & 28; }
 };
 
-constexpr int NT = 256;       // threads per workgroup (4 waves, 2 x 2)
+constexpr int NT = 256;       // threads per workgroup (4 waves, 2 x 2) unless stated
 constexpr int LDS_PAD = 16;   // elements; shifts successive k rows by half the banks
 
 // Workgroup tile TILE x TILE: 128 for large problems, 64 when the 128-tile
 // grid would leave most CUs idle (the skinny GEMMs of the TRSM recursion,
-// small batched problems).  Each wave owns (TILE/2)^2 = TI x TI MFMA tiles.
+// small batched problems).  A workgroup of NTH threads has 2 waves along M
+// and NTH / 128 along N; each wave owns (TILE / 2) x (TILE / (NTH / 128)).
 template <int TILE>
 struct Tile {
   static constexpr int BM = TILE, BN = TILE, S = TILE + LDS_PAD;
-  static constexpr int TI = TILE / 32;  // 16 x 16 MFMA tiles per wave and dimension
   static constexpr int MIN_WG_PER_CU = TILE == 128 ? 2 : 4;
 };
 
@@ -156,11 +156,11 @@ __host__ __device__ inline void map_tile(int w, int ntiles, int tiles_m, int til
 
 // One operand slice (A: BM x BK, B: BK x BN) in registers.  `c` runs along the
 // operand's unit-stride dimension, `o` along the other; V elements per vector.
-template <typename T, int LC, int LO, int S>
+template <typename T, int LC, int LO, int S, int NTH>
 struct Slice {
   static constexpr int V = 16 / sizeof(T);
-  static constexpr int NV = LC * LO / (V * NT);
-  static_assert(NV >= 1 && NV * V * NT == LC * LO, "every thread loads whole vectors");
+  static constexpr int NV = LC * LO / (V * NTH);
+  static_assert(NV >= 1 && NV * V * NTH == LC * LO, "every thread loads whole vectors");
   static_assert(LC % V == 0, "a vector stays within one row");
   T v[NV][V];
 
@@ -170,7 +170,7 @@ struct Slice {
     const bool full = vec_ok && (c0 + LC <= c_lim) && (o0 + LO <= o_lim);
 #pragma unroll
     for (int i = 0; i < NV; ++i) {
-      const int lin = i * NT + tid;
+      const int lin = i * NTH + tid;
       const int c = (lin % (LC / V)) * V;
       const int o = lin / (LC / V);
       const T *p = x + (int64_t)(o0 + o) * ld + (c0 + c);
@@ -202,7 +202,7 @@ struct Slice {
   __device__ inline void store(T *lds, int tid) const {
 #pragma unroll
     for (int i = 0; i < NV; ++i) {
-      const int lin = i * NT + tid;
+      const int lin = i * NTH + tid;
       const int c = (lin % (LC / V)) * V;
       const int o = lin / (LC / V);
 #pragma unroll
@@ -214,18 +214,18 @@ struct Slice {
   }
 };
 
-template <typename T, int TILE, bool A_KC, bool B_NC, int BK>
+template <typename T, int TILE, bool A_KC, bool B_NC, int BK, int NTH>
 struct Operands {
   static constexpr int BM = Tile<TILE>::BM, BN = Tile<TILE>::BN, S = Tile<TILE>::S;
   // A is m x k: K-contiguous -> (c = k, o = m); else (c = m, o = k).
-  typedef Slice<T, A_KC ? BK : BM, A_KC ? BM : BK, S> SA;
+  typedef Slice<T, A_KC ? BK : BM, A_KC ? BM : BK, S, NTH> SA;
   // B is k x n: N-contiguous -> (c = n, o = k); else (c = k, o = n).
-  typedef Slice<T, B_NC ? BN : BK, B_NC ? BK : BN, S> SB;
+  typedef Slice<T, B_NC ? BN : BK, B_NC ? BK : BN, S, NTH> SB;
 };
 
-template <typename T, int TILE, bool A_KC, bool B_NC, int BK>
-__device__ inline void load_slices(typename Operands<T, TILE, A_KC, B_NC, BK>::SA &ra,
-                                   typename Operands<T, TILE, A_KC, B_NC, BK>::SB &rb,
+template <typename T, int TILE, bool A_KC, bool B_NC, int BK, int NTH>
+__device__ inline void load_slices(typename Operands<T, TILE, A_KC, B_NC, BK, NTH>::SA &ra,
+                                   typename Operands<T, TILE, A_KC, B_NC, BK, NTH>::SB &rb,
                                    const T *a, int64_t lda, const T *b,
                                    int64_t ldb, int m0, int n0, int k0, int m,
                                    int n, int k, bool va, bool vb, int tid) {
@@ -243,32 +243,34 @@ struct LdsSize {
 };
 
 // One TILE x TILE output tile (tm, tn) of the product described by p,
-// computed by the calling workgroup of NT threads with `lds` (LdsSize
+// computed by the calling workgroup of NTH threads with `lds` (LdsSize
 // elements) as staging memory.  The caller synchronises before reusing lds.
 // BK, the K depth of one staged slice, sets only how K is cut into slices:
 // every depth runs the same MFMA steps in the same k order (a slice past k
 // adds exact zeros), so the result does not depend on it.
-template <typename T, int TILE, bool A_KC, bool B_NC, int BK = Mfma<T>::BK>
+template <typename T, int TILE, bool A_KC, bool B_NC, int BK = Mfma<T>::BK, int NTH = NT>
 __device__ inline void gemm_tile(const GemmArgs<T> &p, int tm, int tn, T *lds) {
   typedef Mfma<T> MF;
   typedef typename MF::acc_t acc_t;
   static_assert(BK % 4 == 0 && BK <= Mfma<T>::BK, "whole MFMA k steps, swizzle range");
   constexpr int BM = Tile<TILE>::BM, BN = Tile<TILE>::BN, S = Tile<TILE>::S;
-  constexpr int TI = Tile<TILE>::TI, WT = TILE / 2;
-  typedef Operands<T, TILE, A_KC, B_NC, BK> Ops;
+  constexpr int WNW = NTH / 128, WTM = TILE / 2, WTN = TILE / WNW;
+  constexpr int TIM = WTM / 16, TIN = WTN / 16;  // 16 x 16 MFMA tiles per wave
+  static_assert(NTH % 128 == 0 && WTN >= 16, "2 x (NTH / 128) waves of whole MFMA tiles");
+  typedef Operands<T, TILE, A_KC, B_NC, BK, NTH> Ops;
   T(*As)[BK * S] = reinterpret_cast<T(*)[BK * S]>(lds);
   T(*Bs)[BK * S] = reinterpret_cast<T(*)[BK * S]>(lds + 2 * BK * S);
   const int m0 = tm * BM, n0 = tn * BN;
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
-  const int wm = (wave & 1) * WT, wn = (wave >> 1) * WT;
+  const int wm = (wave & 1) * WTM, wn = (wave >> 1) * WTN;
   const int lr = lane & 15, lk = lane >> 4;
 
-  acc_t acc[TI][TI];
+  acc_t acc[TIM][TIN];
 #pragma unroll
-  for (int i = 0; i < TI; ++i)
+  for (int i = 0; i < TIM; ++i)
 #pragma unroll
-    for (int j = 0; j < TI; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
+    for (int j = 0; j < TIN; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
 
   typename Ops::SA ra;
   typename Ops::SB rb;
@@ -291,7 +293,7 @@ __device__ inline void gemm_tile(const GemmArgs<T> &p, int tm, int tn, T *lds) {
     const int ktiles = kend > kbeg ? (kend + BK - 1) / BK - kt0 : 0;
     if (ktiles <= 0) continue;
 
-    load_slices<T, TILE, A_KC, B_NC, BK>(ra, rb, a, lda, b, ldb, m0, n0, kt0 * BK, p.m, p.n,
+    load_slices<T, TILE, A_KC, B_NC, BK, NTH>(ra, rb, a, lda, b, ldb, m0, n0, kt0 * BK, p.m, p.n,
                                      p.k, va, vb, tid);
     __syncthreads();  // previous segment's readers are done with buffer 0
     ra.template store<A_KC>(As[0], tid);
@@ -303,7 +305,7 @@ __device__ inline void gemm_tile(const GemmArgs<T> &p, int tm, int tn, T *lds) {
       const int cur = kt & 1;
       const bool more = kt + 1 < ktiles;
       if (more)
-        load_slices<T, TILE, A_KC, B_NC, BK>(ra, rb, a, lda, b, ldb, m0, n0,
+        load_slices<T, TILE, A_KC, B_NC, BK, NTH>(ra, rb, a, lda, b, ldb, m0, n0,
                                          (kt0 + kt + 1) * BK, p.m, p.n, p.k, va, vb, tid);
       const T *as = As[cur];
       const T *bs = Bs[cur];
@@ -315,17 +317,28 @@ __device__ inline void gemm_tile(const GemmArgs<T> &p, int tm, int tn, T *lds) {
       if (PRIO) __builtin_amdgcn_s_setprio(1);
 #pragma unroll
       for (int kk = 0; kk < BK / 4; ++kk) {
-        T af[TI], bf[TI];
+        T af[TIM], bf[TIN];
         const int kr = kk * 4 + lk, row = kr * S;
         const int swa = A_KC ? MF::swz(kr) : 0, swb = B_NC ? 0 : MF::swz(kr);
+        // (x + lr) ^ s == (x ^ (s & ~15)) + (lr ^ (s & 15)) for x a multiple
+        // of 16 and lr < 16; s & ~15 depends on kk alone (lk < 4 sets only
+        // bits 0-1 of kr), so each tile's part is an immediate offset.  This
+        // form saves address registers (it took the FP64 8-wave kernel with
+        // op(A) = A^T out of spilling).  For B when only B is swizzled
+        // (column-major A and B) the plain form measured faster.
+        const int hia = A_KC ? MF::swz(4 * kk) & ~15 : 0;
+        const int hib = B_NC ? 0 : MF::swz(4 * kk) & ~15;
+        const int rowa = row + (lr ^ (swa & 15)), rowb = row + (lr ^ (swb & 15));
+        constexpr bool SPLIT_B = A_KC || B_NC;
 #pragma unroll
-        for (int i = 0; i < TI; ++i) af[i] = as[row + ((wm + i * 16 + lr) ^ swa)];
+        for (int i = 0; i < TIM; ++i) af[i] = as[rowa + ((wm + i * 16) ^ hia)];
 #pragma unroll
-        for (int j = 0; j < TI; ++j) bf[j] = bs[row + ((wn + j * 16 + lr) ^ swb)];
+        for (int j = 0; j < TIN; ++j)
+          bf[j] = SPLIT_B ? bs[rowb + ((wn + j * 16) ^ hib)] : bs[row + ((wn + j * 16 + lr) ^ swb)];
 #pragma unroll
-        for (int i = 0; i < TI; ++i)
+        for (int i = 0; i < TIM; ++i)
 #pragma unroll
-          for (int j = 0; j < TI; ++j) acc[i][j] = MF::step(af[i], bf[j], acc[i][j]);
+          for (int j = 0; j < TIN; ++j) acc[i][j] = MF::step(af[i], bf[j], acc[i][j]);
       }
       if (PRIO) __builtin_amdgcn_s_setprio(0);
       if (more) {
@@ -348,11 +361,11 @@ __device__ inline void gemm_tile(const GemmArgs<T> &p, int tm, int tn, T *lds) {
     return row < p.m && col < p.n && !(p.ji == 1 && col > row) && !(p.ji == 2 && col < row);
   };
 #pragma unroll
-  for (int i = 0; i < TI; ++i) {
-    T cv[TI][4];
+  for (int i = 0; i < TIM; ++i) {
+    T cv[TIN][4];
     if (beta != T(0)) {
 #pragma unroll
-      for (int j = 0; j < TI; ++j) {
+      for (int j = 0; j < TIN; ++j) {
         const int col = n0 + wn + j * 16 + lr;
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
@@ -362,7 +375,7 @@ __device__ inline void gemm_tile(const GemmArgs<T> &p, int tm, int tn, T *lds) {
       }
     }
 #pragma unroll
-    for (int j = 0; j < TI; ++j) {
+    for (int j = 0; j < TIN; ++j) {
       const int col = n0 + wn + j * 16 + lr;
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
