@@ -309,12 +309,12 @@ __device__ inline void gemm_tile(const GemmArgs<T> &p, int tm, int tn, T *lds) {
                                          (kt0 + kt + 1) * BK, p.m, p.n, p.k, va, vb, tid);
       const T *as = As[cur];
       const T *bs = Bs[cur];
-      // A wave raises its priority for its MFMA block, so that of the two
-      // waves on a SIMD the one with MFMAs ready keeps issuing them while the
-      // other runs its loads and LDS stores (measured: FP64 NN +2.6 %, FP32
-      // +1.3 %; FP64 with op(A) = A^T and column-major B, -0.5 %: left out).
-      constexpr bool PRIO = !(sizeof(T) == 8 && A_KC && !B_NC);
-      if (PRIO) __builtin_amdgcn_s_setprio(1);
+      // A wave raises its priority for its MFMA block, so that of the waves
+      // on a SIMD the one with MFMAs ready keeps issuing them while the
+      // others run their loads and LDS stores (measured: FP64 NN +2.6 %, FP32
+      // +1.3 %; FP64 with op(A) = A^T and column-major B -0.5 % at 2 waves
+      // per SIMD, +3.1 % at 4).
+      __builtin_amdgcn_s_setprio(1);
 #pragma unroll
       for (int kk = 0; kk < BK / 4; ++kk) {
         T af[TIM], bf[TIN];
@@ -340,7 +340,7 @@ __device__ inline void gemm_tile(const GemmArgs<T> &p, int tm, int tn, T *lds) {
 #pragma unroll
           for (int j = 0; j < TIN; ++j) acc[i][j] = MF::step(af[i], bf[j], acc[i][j]);
       }
-      if (PRIO) __builtin_amdgcn_s_setprio(0);
+      __builtin_amdgcn_s_setprio(0);
       if (more) {
         ra.template store<A_KC>(As[cur ^ 1], tid);
         rb.template store<!B_NC>(Bs[cur ^ 1], tid);
