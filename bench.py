@@ -125,7 +125,10 @@ def profile_mode(args, torch, M, lib, dev, g):
         torch.cuda.synchronize()
         what, calls = "dgemm", 1
     else:
-        what, calls = args.only, 3
+        # The GEMMs run bench.py's own warm-up + timed call count, so that a
+        # traced average compares with the timed figure; the rest a few calls.
+        what = args.only
+        calls = args.warmup + args.steps if what in ("dgemm", "sgemm") else 3
     if what == "sgemm_bf16x3":
         nn = args.precision_n or n
         A = torch.randn((nn, nn), device=dev, generator=g)
