@@ -8,7 +8,8 @@
 //                         from HBM with 16-byte vector loads and staged in
 //                         LDS ([k][m] and [k][n], double-buffered);
 //   jr / ir            -> 4 wavefronts in a 2 x 2 layout, each owning 64 x 64
-//                         (FP64: 8 in 2 x 4, each 64 x 32 or 32 x 16);
+//                         (FP64 128-tiles and full-depth 64-tiles: 8 in
+//                         2 x 4, each 64 x 32 or 32 x 16);
 //   micro-kernel p loop-> chains of v_mfma_f64_16x16x4_f64 (FP64) or
 //                         v_mfma_f32_16x16x4_f32 (FP32), 16 (8) tiles per wave;
 //   beta pre-pass, c_buf edge tiles, ji masks
@@ -64,14 +65,15 @@ bool use_shallow(const GemmArgs<T> &p, int64_t wgs) {
 // smaller levels, SYRK diagonal blocks) likewise run 8 waves of 32 x 16 at 4
 // workgroups per CU, 8 waves per SIMD at 53-64 VGPRs: DTRSM 8192 x 4096 5.07
 // -> 4.99 ms, 8192 x 1024 2.00 -> 1.94, 4096 x 4096 1.54 -> 1.51; DSYRK 4096
-// / 2048 +2.6 %.
+// / 2048 +2.6 %.  FP32 full-depth 64-tiles too (SSYRK 4096 / 2048 +3.6 %,
+// SGEMM within noise).
 template <typename T, int TILE, bool SHALLOW>
 struct KernelShape {
   static_assert(!SHALLOW || TILE == 64 || sizeof(T) == 4, "FP64 128-tiles: deep only");
   static constexpr int BK = SHALLOW ? Mfma<T>::BK / 2 : Mfma<T>::BK;
   static constexpr int WG_PER_CU =
       SHALLOW ? (TILE == 64 ? 6 : 4) : Tile<TILE>::MIN_WG_PER_CU;
-  static constexpr int NTH = (sizeof(T) == 8 && (TILE == 128 || !SHALLOW)) ? 512 : NT;
+  static constexpr int NTH = ((sizeof(T) == 8 && TILE == 128) || (TILE == 64 && !SHALLOW)) ? 512 : NT;
   // __launch_bounds__'s second argument: minimum waves per SIMD
   static constexpr int WAVES_PER_SIMD = WG_PER_CU * NTH / 256;
 };
