@@ -44,23 +44,26 @@ struct GroupItem {
 
 constexpr int kTiles[3] = {32, 64, 128};
 
-template <int TILE, int BK>
+template <int TILE, int BK, int NTH>
 __device__ inline void run_tile(const GemmArgs<float> &p, int tm, int tn, int layout,
                                 float *lds) {
   switch (layout) {
-    case 0: gemm_tile<float, TILE, false, false, BK>(p, tm, tn, lds); break;
-    case 1: gemm_tile<float, TILE, true, false, BK>(p, tm, tn, lds); break;
-    case 2: gemm_tile<float, TILE, false, true, BK>(p, tm, tn, lds); break;
-    default: gemm_tile<float, TILE, true, true, BK>(p, tm, tn, lds); break;
+    case 0: gemm_tile<float, TILE, false, false, BK, NTH>(p, tm, tn, lds); break;
+    case 1: gemm_tile<float, TILE, true, false, BK, NTH>(p, tm, tn, lds); break;
+    case 2: gemm_tile<float, TILE, false, true, BK, NTH>(p, tm, tn, lds); break;
+    default: gemm_tile<float, TILE, true, true, BK, NTH>(p, tm, tn, lds); break;
   }
 }
 
 // MAXT: the tile shape of every item of the launch, which sizes LDS (and
 // registers) and so the workgroups per CU: 128-tile items run 2 per CU,
-// 64-tile items 4 and 32-tile items 6, so the many small items of a batch
-// of small problems hide more of their short K loops' latency.
-template <int MAXT, int WG_PER_CU, int BK>
-__global__ __launch_bounds__(NT, WG_PER_CU) void grouped_kernel(
+// 64-tile items 3 (of 8 waves: 6 per SIMD) and 32-tile items 6, so the many
+// small items of a batch of small problems hide more of their short K
+// loops' latency.  NTH threads per workgroup; __launch_bounds__ takes the
+// minimum waves per SIMD.  (64-tile items at 4 per CU of 4 waves: 1.597 ms
+// per replay; 3 per CU of 8 waves: 1.589; 4 of 8 waves spill: 1.643.)
+template <int MAXT, int WG_PER_CU, int BK, int NTH = NT>
+__global__ __launch_bounds__(NTH, WG_PER_CU * NTH / 256) void grouped_kernel(
     const GemmArgs<float> *__restrict__ probs, const GroupItem *__restrict__ items,
     int nitems) {
   __shared__ float lds[LdsSize<float, MAXT, BK>::value];
@@ -69,7 +72,7 @@ __global__ __launch_bounds__(NT, WG_PER_CU) void grouped_kernel(
     const GemmArgs<float> p = probs[w.prob];
     const int tile = w.kind >> 2, layout = w.kind & 3;
     (void)tile;  // every item of a launch has the launch's tile shape
-    run_tile<MAXT, BK>(p, w.tm, w.tn, layout, lds);
+    run_tile<MAXT, BK, NTH>(p, w.tm, w.tn, layout, lds);
     __syncthreads();  // the next item restages lds
   }
 }
@@ -439,7 +442,7 @@ int persistent_grid(int nitems, int per_cu) {
 // Measured on the batched workload, back-to-back launches beat parallel
 // graph branches (1.86 vs 2.69 ms for two classes) and one launch for all
 // items at 2 workgroups per CU (2.27 ms).
-constexpr int W128 = 2, BK128 = 32, W64 = 4, BK64 = 32, W32 = 6, BK32 = 32;
+constexpr int W128 = 2, BK128 = 32, W64 = 3, BK64 = 32, W32 = 6, BK32 = 32;
 int launch_items(const GemmArgs<float> *probs, const GroupItem *items, const int *ncls,
                  hipStream_t s) {
   const int n128 = ncls[2], n64 = ncls[1], n32 = ncls[0];
@@ -447,8 +450,8 @@ int launch_items(const GemmArgs<float> *probs, const GroupItem *items, const int
     hipLaunchKernelGGL((grouped_kernel<128, W128, BK128>), dim3(persistent_grid(n128, W128)),
                        dim3(NT), 0, s, probs, items, n128);
   if (n64 > 0)
-    hipLaunchKernelGGL((grouped_kernel<64, W64, BK64>), dim3(persistent_grid(n64, W64)),
-                       dim3(NT), 0, s, probs, items + n128, n64);
+    hipLaunchKernelGGL((grouped_kernel<64, W64, BK64, 512>), dim3(persistent_grid(n64, W64)),
+                       dim3(512), 0, s, probs, items + n128, n64);
   if (n32 > 0)
     hipLaunchKernelGGL((grouped_kernel<32, W32, BK32>), dim3(persistent_grid(n32, W32)),
                        dim3(NT), 0, s, probs, items + n128 + n64, n32);
