@@ -353,21 +353,7 @@ def run_batched_trsm(torch, M, lib, extra):
 def serial_call(lib, S, r):
     """One serving request as the plain synchronous cblas_* call on the same
     device memory."""
-    a, b, c = (ctypes.c_void_p(r[x].ptr) if r.get(x) is not None else None for x in "abc")
-    la, lb, lc = (int(r[x].ld) if r.get(x) is not None else 0 for x in "abc")
-    op = r["op"]
-    if op == S.SGEMM:
-        lib.cblas_sgemm(COL, NOTR, NOTR, r["m"], r["n"], r["k"], r["alpha"], a, la, b, lb,
-                        r["beta"], c, lc)
-    elif op == S.SSYRK:
-        lib.cblas_ssyrk(COL, LO if r["uplo"] == S.LOWER else UP, NOTR, r["n"], r["k"],
-                        r["alpha"], a, la, r["beta"], c, lc)
-    elif op == S.SSYR2K:
-        lib.cblas_ssyr2k(COL, LO if r["uplo"] == S.LOWER else UP, NOTR, r["n"], r["k"],
-                         r["alpha"], a, la, b, lb, r["beta"], c, lc)
-    else:
-        lib.cblas_dtrsm(COL, LEFT, LO, NOTR, NONUNIT, r["m"], r["n"], r["alpha"], a, la, b,
-                        lb)
+    S.plain_call(lib, r)
 
 
 def fifo_latencies(arrivals, service):

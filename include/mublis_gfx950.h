@@ -83,16 +83,21 @@ int mublis_gfx950_sgemm_dev(int m, int n, int k, float alpha,
  * (leading dimension ld): struc 0 GENERAL, 1 SYMMETRIC (read the reflected
  * element outside `uplo`), 2 TRIANGULAR (zero outside `uplo`; diag 1 UNIT
  * writes 1 without reading, 2 INVERT writes 1/a).  diagoff = row - col offset
- * of the view's (0,0) from the diagonal.  Unreferenced storage is never
+ * of the view's (0,0) from the diagonal.  mask (0 all, 1 lower, 2 upper)
+ * then zeroes, without reading, the elements on the other side of that
+ * diagonal (the driver's jp / pi domains).  Unreferenced storage is never
  * read, so NaN there cannot leak.  Replaces packm at whole-operand
- * granularity (frame/l1m/packm.c:18-84).
+ * granularity (frame/l1m/packm.c:18-84).  Codes outside 0..2 give
+ * MUBLIS_EINVAL.
  */
 int mublis_gfx950_dpack_dev(int m, int n, const double *src, int64_t rs,
                             int64_t cs, int struc, int uplo, int diag,
-                            int diagoff, double *dst, int64_t ld, void *stream);
+                            int diagoff, int mask, double *dst, int64_t ld,
+                            void *stream);
 int mublis_gfx950_spack_dev(int m, int n, const float *src, int64_t rs,
                             int64_t cs, int struc, int uplo, int diag,
-                            int diagoff, float *dst, int64_t ld, void *stream);
+                            int diagoff, int mask, float *dst, int64_t ld,
+                            void *stream);
 
 /*
  * In-place triangular solve, Left side only: B := alpha * A^{-1} * B, A m x m
@@ -123,10 +128,17 @@ int mublis_gfx950_strsm_inverted_dev(int lower, int m, int n, const float *a,
                                      int64_t rs_a, int64_t cs_a, float *b,
                                      int64_t rs_b, int64_t cs_b, void *stream);
 
-/* Left-side TRSM with caller workspace (mublis_gfx950_trsm_ws_bytes(m)
- * bytes of device memory), fully asynchronous on `stream`: the form the
- * serving loop uses so its scheduler never waits on a solve. */
+/* Left-side TRSM with caller workspace, fully asynchronous on `stream`: the
+ * form the serving loop uses so its scheduler never waits on a solve.  At
+ * least mublis_gfx950_trsm_ws_bytes(m) bytes of device memory are needed
+ * (the 64-row diagonal-block inverses and the pipeline's flags).  With
+ * mublis_gfx950_?trsm_sched_ws_bytes(m, n) bytes or more the solve runs the
+ * same schedule as the plain mublis_gfx950_?trsm_dev call (GEMM leaves on
+ * inverted 1024-row blocks where they apply) and gives the same bits; with
+ * less it ends its recursion in pipelined or panel leaves instead. */
 size_t mublis_gfx950_trsm_ws_bytes(int m);
+size_t mublis_gfx950_dtrsm_sched_ws_bytes(int m, int n);
+size_t mublis_gfx950_strsm_sched_ws_bytes(int m, int n);
 /* Widest B (columns after the side transformation) that the left-side
  * solves hand to the pipelined GEMMTRSM kernel (one launch, block rows
  * chained by device-side flags) instead of the recursive schedule; set it

@@ -78,7 +78,7 @@ def _declare(lib):
         f.argtypes = [_I, _I, _I, _I, ct, _VP, _I64, _I64, _VP, _I64, _I64, _VP]
         f.restype = _I
         f = getattr(lib, f"mublis_gfx950_{p}pack_dev")
-        f.argtypes = [_I, _I, _VP, _I64, _I64, _I, _I, _I, _I, _VP, _I64, _VP]
+        f.argtypes = [_I, _I, _VP, _I64, _I64, _I, _I, _I, _I, _I, _VP, _I64, _VP]
         f.restype = _I
         f = getattr(lib, f"mublis_gfx950_{p}trsm_inverted_dev")
         f.argtypes = [_I, _I, _I, _VP, _I64, _I64, _VP, _I64, _I64, _VP]

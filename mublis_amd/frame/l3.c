@@ -54,12 +54,37 @@ static int finish(int st, void *stream, stage_t *out, stage_t **tmp, int ntmp) {
 /* ------------------------------------------------------ generic driver -- */
 /*
  * Driver contract (frame/l3/l3_driver.c:49-249): C := alpha * Aview * Bview +
- * beta * C on the elements with ji(j, i).  SYMMETRIC and TRIANGULAR operands
- * are materialised densely first (square operands only: their reflected or
- * masked elements must stay inside the staged footprint).  jp and pi do not
- * change the value: the shims set them only for TRIANGULAR operands whose
- * excluded part is zero.
+ * beta * C on the elements with ji(j, i).  The views put the diagonal at
+ * i == p for A and p == j for B whatever the shape, as packm's diagoff = ic -
+ * pc / jc - pc does (l3_driver.c:133-161, frame/l1m/packm.c:41): an m x k
+ * TRIANGULAR A is a trapezoid when m != k, a SYMMETRIC one reads (p, i) for
+ * an unstored (i, p).  jp / pi are applied per term (term p of C(i,j) only
+ * where pi(p, i) and jp(j, p) hold) by zeroing the excluded part of the
+ * operand while it is materialised.  The reference skips whole cache blocks
+ * instead (l3_driver.c:27-47,130,148); the two agree whenever the relation
+ * excludes only structural zeros, which is every descriptor the shims build.
+ * Structured or masked operands are materialised densely first (pack_dev);
+ * GENERAL unmasked ones are read in place.
  */
+
+/* Extent of the storage an operand view reads (rows x cols of its strided
+ * footprint).  A SYMMETRIC view also reads reflected elements, which lie
+ * past the m x k box on the stored side when m != k. */
+static void view_extent(const mublis_l3_operand_t *o, int m, int k, int *rows,
+                        int *cols) {
+  *rows = m;
+  *cols = k;
+  if (o->struc == MUBLIS_STRUC_SYMMETRIC && m != k && m > 0 && k > 0) {
+    const int lo = m < k ? m : k, hi = m < k ? k : m;
+    if (o->uplo == MUBLIS_LOWER) { *rows = hi; *cols = lo; }
+    if (o->uplo == MUBLIS_UPPER) { *rows = lo; *cols = hi; }
+  }
+}
+
+static int operand_ok(const mublis_l3_operand_t *o) {
+  return (unsigned)o->struc <= 2u && (unsigned)o->uplo <= 2u && (unsigned)o->diag <= 2u;
+}
+
 #define DRIVER_IMPL(T, name, gemm_dev, pack_dev)                                \
   int name(int m, int n, int k, T alpha, const T *a, int rs_a, int cs_a,       \
            const T *b, int rs_b, int cs_b, T beta, T *c, int rs_c, int cs_c,   \
@@ -67,48 +92,58 @@ static int finish(int st, void *stream, stage_t *out, stage_t **tmp, int ntmp) {
     if (m == 0 || n == 0) return 0;                                            \
     if (m < 0 || n < 0 || k < 0 || !product) return MUBLIS_EINVAL;             \
     const mublis_l3_operand_t *oa = &product->a, *ob = &product->b;            \
-    if ((oa->struc != MUBLIS_STRUC_GENERAL && m != k) ||                       \
-        (ob->struc != MUBLIS_STRUC_GENERAL && k != n))                         \
+    const mublis_l3_domain_t *dom = &product->domain;                          \
+    if (!operand_ok(oa) || !operand_ok(ob) || (unsigned)dom->jp > 2u ||        \
+        (unsigned)dom->ji > 2u || (unsigned)dom->pi > 2u)                      \
       return MUBLIS_EINVAL;                                                    \
     void *s;                                                                   \
     int st = begin(&s);                                                        \
     if (st) return st;                                                         \
     const int use = alpha != (T)0 && k > 0;                                    \
+    /* pi(p, i) LOWER keeps p <= i of A(i, p); jp(j, p) LOWER keeps j <= p   \
+     * of B(p, j): both are "column <= row", the pack's lower mask. */        \
+    const int mask_a = (int)dom->pi, mask_b = (int)dom->jp;                    \
+    const int pack_a = oa->struc != MUBLIS_STRUC_GENERAL || mask_a;            \
+    const int pack_b = ob->struc != MUBLIS_STRUC_GENERAL || mask_b;            \
+    int ra_rows, ra_cols, rb_rows, rb_cols;                                    \
+    view_extent(oa, m, k, &ra_rows, &ra_cols);                                 \
+    view_extent(ob, k, n, &rb_rows, &rb_cols);                                 \
     stage_t sa, sb, sc;                                                        \
     memset(&sa, 0, sizeof sa);                                                 \
     memset(&sb, 0, sizeof sb);                                                 \
     T *pa = NULL, *pb = NULL;                                                  \
     stage_t *tmp[2] = {&sa, &sb};                                              \
-    st = mublis_stage_in(c, m, n, rs_c, cs_c, sizeof(T), 1, &sc, s);                  \
+    st = mublis_stage_in(c, m, n, rs_c, cs_c, sizeof(T), 1, &sc, s);           \
     if (st == MUBLIS_OK && use)                                                \
-      st = mublis_stage_in(a, m, k, rs_a, cs_a, sizeof(T), 1, &sa, s);                \
+      st = mublis_stage_in(a, ra_rows, ra_cols, rs_a, cs_a, sizeof(T), 1, &sa, s); \
     if (st == MUBLIS_OK && use) {                                              \
-      if ((const void *)b == (const void *)a && sa.dev) {                      \
+      if ((const void *)b == (const void *)a && sa.dev && rb_rows == ra_cols &&\
+          rb_cols == ra_rows && rs_b == cs_a && cs_b == rs_a) {                \
         sb = sa;                                                               \
-        sb.owned = 0; /* shared with sa (SYRK passes A twice) */               \
+        sb.owned = 0; /* the same storage (SYRK passes A twice) */             \
       } else                                                                   \
-        st = mublis_stage_in(b, k, n, rs_b, cs_b, sizeof(T), 1, &sb, s);              \
+        st = mublis_stage_in(b, rb_rows, rb_cols, rs_b, cs_b, sizeof(T), 1, &sb, s); \
     }                                                                          \
     const T *da = (const T *)sa.dev, *db = (const T *)sb.dev;                  \
     int64_t ra = rs_a, ca = cs_a, rb = rs_b, cb = cs_b;                        \
-    if (st == MUBLIS_OK && use && oa->struc != MUBLIS_STRUC_GENERAL) {         \
+    if (st == MUBLIS_OK && use && pack_a) {                                    \
       pa = (T *)mublis_rt_alloc(sizeof(T) * (size_t)m * (size_t)k);            \
       if (!pa) st = MUBLIS_ENOMEM;                                             \
       else st = pack_dev(m, k, da, rs_a, cs_a, oa->struc, oa->uplo, oa->diag,  \
-                         0, pa, m, s);                                         \
+                         0, mask_a, pa, m, s);                                 \
       da = pa; ra = 1; ca = m;                                                 \
     }                                                                          \
-    if (st == MUBLIS_OK && use && ob->struc != MUBLIS_STRUC_GENERAL) {         \
+    if (st == MUBLIS_OK && use && pack_b) {                                    \
       pb = (T *)mublis_rt_alloc(sizeof(T) * (size_t)k * (size_t)n);            \
       if (!pb) st = MUBLIS_ENOMEM;                                             \
       else st = pack_dev(k, n, db, rs_b, cs_b, ob->struc, ob->uplo, ob->diag,  \
-                         0, pb, k, s);                                         \
+                         0, mask_b, pb, k, s);                                 \
       db = pb; rb = 1; cb = k;                                                 \
     }                                                                          \
     if (st == MUBLIS_OK)                                                       \
       st = gemm_dev(m, n, use ? k : 0, use ? alpha : (T)0, da, ra, ca, db, rb, \
                     cb, NULL, 0, 0, NULL, 0, 0, beta, (T *)sc.dev, rs_c, cs_c, \
-                    (int)product->domain.ji, s);                               \
+                    (int)dom->ji, s);                                          \
     st = finish(st, s, &sc, tmp, 2);                                           \
     mublis_rt_free(pa);                                                        \
     mublis_rt_free(pb);                                                        \
@@ -271,7 +306,7 @@ SYR2K_IMPL(double, mublis_dsyr2k, mublis_gfx950_dgemm_dev)
                     MUBLIS_STRUC_TRIANGULAR, uplo,                             \
                     diag == MUBLIS_UNIT_DIAG ? MUBLIS_PACKM_DIAG_UNIT          \
                                              : MUBLIS_PACKM_DIAG_NONUNIT,      \
-                    0, tri, sz, s);                                            \
+                    0, 0, tri, sz, s);                                         \
     if (st == MUBLIS_OK)                                                       \
       st = copy_dev(m, n, (const T *)sb.dev, rs_b, cs_b, bw, 1, m, s);         \
     /* K slices outside the triangle are skipped (tri codes of            \

@@ -259,9 +259,15 @@ struct mublis_server_s {
       }
       // a launch of its own: TRSM (with workspace), DGEMM, or a large FP32
       // request on the single-problem kernel
+      // A TRSM gets the full schedule's workspace, so it runs the schedule of
+      // the plain call (the same bits, alone or in any mix).
       size_t ws = 0;
-      if (r->r.op == MUBLIS_REQ_STRSM || r->r.op == MUBLIS_REQ_DTRSM)
-        ws = mublis_gfx950_trsm_ws_bytes(r->r.side == MUBLIS_RIGHT ? r->r.n : r->r.m);
+      if (r->r.op == MUBLIS_REQ_STRSM || r->r.op == MUBLIS_REQ_DTRSM) {
+        const bool right = r->r.side == MUBLIS_RIGHT;
+        const int mm = right ? r->r.n : r->r.m, nn = right ? r->r.m : r->r.n;
+        ws = r->r.op == MUBLIS_REQ_DTRSM ? mublis_gfx950_dtrsm_sched_ws_bytes(mm, nn)
+                                         : mublis_gfx950_strsm_sched_ws_bytes(mm, nn);
+      }
       if (!ws_alloc(ws, *s)) {
         if (inflight == 0 && out.empty() && ws_arena.used() == 0) {
           // can never fit: fail it rather than wait forever

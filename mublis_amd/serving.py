@@ -144,3 +144,33 @@ class Server:
 
     def __exit__(self, *exc):
         self.close()
+
+
+def plain_call(lib, r):
+    """Run one serving request (a dict of Server.submit keyword arguments
+    plus "op") as the plain synchronous cblas_* call on the same device
+    memory -- the serial path the serving loop is measured and checked
+    against."""
+    a, b, c = (ctypes.c_void_p(r[x].ptr) if r.get(x) is not None else None for x in "abc")
+    la, lb, lc = (int(r[x].ld) if r.get(x) is not None else 0 for x in "abc")
+    op = r["op"]
+    col, notr, tr, lo, up = 102, 111, 112, 122, 121
+    uplo = lo if r.get("uplo", LOWER) == LOWER else up
+    ta = tr if r.get("trans_a", NO_TRANS) else notr
+    tb = tr if r.get("trans_b", NO_TRANS) else notr
+    if op == SGEMM:
+        lib.cblas_sgemm(col, ta, tb, r["m"], r["n"], r["k"], r["alpha"], a, la, b, lb,
+                        r["beta"], c, lc)
+    elif op == DGEMM:
+        lib.cblas_dgemm(col, ta, tb, r["m"], r["n"], r["k"], r["alpha"], a, la, b, lb,
+                        r["beta"], c, lc)
+    elif op == SSYRK:
+        lib.cblas_ssyrk(col, uplo, ta, r["n"], r["k"], r["alpha"], a, la, r["beta"], c, lc)
+    elif op == SSYR2K:
+        lib.cblas_ssyr2k(col, uplo, ta, r["n"], r["k"], r["alpha"], a, la, b, lb, r["beta"],
+                         c, lc)
+    else:
+        side = 142 if r.get("side", LEFT) == RIGHT else 141
+        diag = 132 if r.get("diag", NON_UNIT) == UNIT else 131
+        f = lib.cblas_strsm if op == STRSM else lib.cblas_dtrsm
+        f(col, side, uplo, ta, diag, r["m"], r["n"], r["alpha"], a, la, b, lb)

@@ -231,7 +231,8 @@ int add_trsm(const mublis_sbatch_op_t &o, Work &w,
   w.bytes += 4.0 * ((double)mm * (mm + 1) / 2 + 2.0 * mm * nn);
   if (mm == 0 || nn == 0) return MUBLIS_OK;
   if (mm > kGroupTrsmMax || o.alpha == 0.0f) {
-    const size_t ws = o.alpha == 0.0f ? 0 : align256(mublis_gfx950_trsm_ws_bytes(mm));
+    // the full schedule's workspace: the bits of the single cblas_strsm call
+    const size_t ws = o.alpha == 0.0f ? 0 : align256(mublis_gfx950_strsm_sched_ws_bytes(mm, nn));
     w.solo.push_back({lower, o.diag, mm, nn, o.alpha, o.a, rsa, csa, o.c, rsb, csb,
                       w.solo_bytes, ws});
     w.solo_bytes += ws;
@@ -267,6 +268,17 @@ int build_work(const mublis_sbatch_op_t *ops, int count, int flags, Work &w) {
     const bool sym = syrk || syr2k;
     const int m = sym ? o.n : o.m, n = o.n, k = o.k;
     if (m < 0 || n < 0 || k < 0 || (!sym && o.op != MUBLIS_BATCH_GEMM)) return MUBLIS_EINVAL;
+    if ((o.trans_a & ~1) || (!sym && (o.trans_b & ~1)) ||
+        (sym && o.uplo != MUBLIS_LOWER && o.uplo != MUBLIS_UPPER))
+      return MUBLIS_EINVAL;
+    // The CBLAS leading-dimension rules (frame/compat/cblas/cblas.c:150-171,
+    // 306-325, 380-404, column-major): every stored column of every operand
+    // fits its ld, so no tile reads or writes outside the caller's storage.
+    const int rows_a = o.trans_a ? k : m;
+    const int rows_b = sym ? rows_a : (o.trans_b ? n : k);
+    if (o.lda < std::max(1, rows_a) || o.ldc < std::max(1, m) ||
+        ((syr2k || !sym) && o.ldb < std::max(1, rows_b)))
+      return MUBLIS_EINVAL;
     GemmArgs<float> p;
     memset(&p, 0, sizeof p);
     // alpha == 0 reads neither A nor B (frame/l3/l3_driver.c:123-124): K = 0
@@ -294,8 +306,15 @@ int build_work(const mublis_sbatch_op_t *ops, int count, int flags, Work &w) {
       p.ji = 0;
     }
     p.c = o.c; p.rs_c = 1; p.cs_c = o.ldc;
-    const bool akc = p.cs_a == 1, bnc = p.rs_b != 1;
-    if ((p.rs_a != 1 && p.cs_a != 1) || (p.rs_b != 1 && p.cs_b != 1)) return MUBLIS_EINVAL;
+    // the same layout rule as the one-problem kernel (gemm.hip), so both
+    // stage the same tiles; SYR2K's second product shares the first's
+    const bool pair = syr2k && p.k > 0;
+    const int la = common_layout(m, k, p.rs_a, p.cs_a, pair ? &p.rs_a2 : nullptr,
+                                 pair ? &p.cs_a2 : nullptr, true);
+    const int lb = common_layout(k, n, p.rs_b, p.cs_b, pair ? &p.rs_b2 : nullptr,
+                                 pair ? &p.cs_b2 : nullptr, false);
+    if (la < 0 || lb < 0) return MUBLIS_EINVAL;
+    const bool akc = la == 1, bnc = lb == 1;
     p.vec_a = aligned16f(p.a, akc ? p.rs_a : p.cs_a);
     p.vec_b = aligned16f(p.b, bnc ? p.rs_b : p.cs_b);
     if (syr2k) {

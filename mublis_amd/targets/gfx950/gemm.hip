@@ -208,15 +208,23 @@ int gemm_dev(int m, int n, int k, T alpha, const T *a, int64_t rs_a, int64_t cs_
   if (st == MUBLIS_OK) {
     // A: K-contiguous whenever cs_a == 1, else M-contiguous (rs_a == 1).
     // B: K-contiguous whenever rs_b == 1, else N-contiguous (cs_b == 1).
-    // When a dimension has extent 1 both readings address the same data.
-    const bool akc = (cs_a == 1);
-    const bool bnc = (rs_b != 1);
+    // A second product is read with the same layout when one suits both
+    // (always so for CBLAS SYR2K, whose operands share trans; strides of
+    // extent-1 dimensions are normalised for it), else it is added in a
+    // launch of its own with beta = 1.
     bool same2 = true;
+    int la = -1, lb = -1;
     if (a2) {
-      const bool akc2 = akc ? (cs_a2 == 1) : (rs_a2 == 1);
-      const bool bnc2 = bnc ? (cs_b2 == 1) : (rs_b2 == 1);
-      same2 = akc2 && bnc2;
+      la = common_layout(m, k, rs_a, cs_a, &rs_a2, &cs_a2, true);
+      lb = common_layout(k, n, rs_b, cs_b, &rs_b2, &cs_b2, false);
+      same2 = la >= 0 && lb >= 0;
     }
+    if (!same2 || !a2) {
+      la = common_layout(m, k, rs_a, cs_a, nullptr, nullptr, true);
+      lb = common_layout(k, n, rs_b, cs_b, nullptr, nullptr, false);
+    }
+    const bool akc = la == 1;
+    const bool bnc = lb == 1;
     GemmArgs<T> p;
     p.m = m; p.n = n; p.k = k; p.alpha = alpha; p.beta = beta;
     p.a = a; p.rs_a = rs_a; p.cs_a = cs_a;

@@ -2,7 +2,9 @@
 //
 // Every FP32 operand element is split exactly as x = hi + mid + lo with hi,
 // mid and lo BF16 (8 significant bits each, round to nearest even; the
-// remainders x - hi and x - hi - mid are exact in FP32).  Then
+// remainders x - hi and x - hi - mid are exact in FP32; Inf and NaN are
+// carried by hi alone, and values past the largest finite BF16 round hi
+// toward zero: bf16_split.h).  Then
 //   A*B ~= Ah*Bh + (Ah*Bm + Am*Bh) + (Ah*Bl + Al*Bh + Am*Bm)
 // keeps every partial product down to 2^-16 relative; the three dropped
 // terms are below FP32 rounding.  Products of two BF16 values are exact in
@@ -24,6 +26,7 @@
 
 #include <algorithm>
 
+#include "bf16_split.h"
 #include "gfx950_common.h"
 #include "mublis_gfx950.h"
 
@@ -39,12 +42,7 @@ constexpr int XTN = 128;   // output tile columns
 constexpr int XBK = 16;    // k per stage (one MFMA k step)
 constexpr int kWaves = 8;  // waves per workgroup (4 or 8)
 
-__device__ inline u16 bf16_rne(float x) {
-  const uint32_t u = __float_as_uint(x);
-  const uint32_t bias = 0x7FFFu + ((u >> 16) & 1u);
-  return (u16)((u + bias) >> 16);
-}
-__device__ inline float bf16_to_f(u16 h) { return __uint_as_float((uint32_t)h << 16); }
+using gfx950::bf16_split3;
 
 // Split an m x k FP32 view (strides rs, cs) into three K-contiguous BF16
 // planes of mp x kp.  A 64 x 64 tile goes through LDS so that reads run
@@ -74,12 +72,8 @@ __global__ __launch_bounds__(256) void split3_kernel(int m, int k, const float *
   u16x8 vh[2], vm[2], vl[2];
 #pragma unroll
   for (int e = 0; e < 16; ++e) {
-    const float v = t[r][c0 + e];
-    const u16 h = bf16_rne(v);
-    const float r1 = v - bf16_to_f(h);
-    const u16 md = bf16_rne(r1);
-    const float r2 = r1 - bf16_to_f(md);
-    const u16 l = bf16_rne(r2);
+    u16 h, md, l;
+    bf16_split3(t[r][c0 + e], h, md, l);
     vh[e / 8][e % 8] = h;
     vm[e / 8][e % 8] = md;
     vl[e / 8][e % 8] = l;

@@ -87,6 +87,39 @@ struct GemmArgs {
   int64_t bsa[2], bsb[2], bsc[2];
 };
 
+// Staging layout of an operand read through strides (r1, c1): 1 when it is
+// read contiguous along its second index (c1 == 1), 0 along its first (r1 ==
+// 1), -1 when neither stride is 1.  `prefer_second` breaks the tie when both
+// are 1 (A prefers K = its second index, B prefers K = its first).  With a
+// second operand of the same rows x cols shape read in the same tile loop
+// through (*r2, *c2) (SYR2K's second product), the first operand's layout is
+// kept when the second can be read that way too, else the other layout is
+// taken when it suits both; a dimension of extent 1 allows either reading
+// and its stride is then set to 1 so that the reading holds.  -1 when no
+// layout suits both (nothing is changed then).
+__host__ inline int common_layout(int rows, int cols, int64_t &r1, int64_t &c1, int64_t *r2,
+                                  int64_t *c2, bool prefer_second) {
+  const bool f1 = r1 == 1, s1 = c1 == 1;
+  if (!f1 && !s1) return -1;
+  const int own = (s1 && (prefer_second || !f1)) ? 1 : 0;
+  if (!r2) return own;
+  auto ok = [&](int lay, int64_t r, int64_t c) {
+    return lay ? (c == 1 || cols <= 1) : (r == 1 || rows <= 1);
+  };
+  for (int lay : {own, 1 - own}) {
+    if (!ok(lay, r1, c1) || !ok(lay, *r2, *c2)) continue;
+    if (lay) {
+      c1 = 1;
+      *c2 = 1;
+    } else {
+      r1 = 1;
+      *r2 = 1;
+    }
+    return lay;
+  }
+  return -1;
+}
+
 // Long-first walk: with a triangular operand, a tile's K extent grows
 // linearly along one axis (a "line" of tiles shares one K), so one launch
 // mixes tiles of K = BK .. k.  Units of g tiles -- whole lines when there

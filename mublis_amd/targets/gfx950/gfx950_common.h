@@ -17,7 +17,8 @@ int gfx950_copy(int m, int n, const T *src, int64_t rs_s, int64_t cs_s, T *dst,
 // Materialise an operand view (GENERAL / SYMMETRIC / TRIANGULAR) densely.
 template <typename T>
 int gfx950_pack(int m, int n, const T *src, int64_t rs, int64_t cs, int struc,
-                int uplo, int diag, int diagoff, T *dst, int64_t ld, hipStream_t s);
+                int uplo, int diag, int diagoff, int mask, T *dst, int64_t ld,
+                hipStream_t s);
 
 template <typename T>
 int gfx950_fill(int m, int n, T value, T *x, int64_t rs, int64_t cs, hipStream_t s);
@@ -55,15 +56,15 @@ int gfx950_pack_f2d(int m, int n, const float *src, int64_t rs, int64_t cs, int 
 int gfx950_refine_update(int m, int n, const float *d, double *x64, float *out,
                          int64_t rs_o, int64_t cs_o, hipStream_t s);
 
-// Left-side FP32 TRSM in caller workspace (gfx950_strsm_keep_ws_bytes(m, n)
-// bytes), asynchronous on s.  With *inv_ready false the solve inverts the
-// triangle's diagonal blocks and sets it when they stay valid for more
-// solves against the same triangle, n and triangle flags; with *inv_ready
-// true it reuses them.
+// Left-side FP32 TRSM in caller workspace of ws_bytes (the full schedule
+// with gfx950_strsm_keep_ws_bytes(m, n) bytes), asynchronous on s.  With
+// *inv_ready false the solve inverts the triangle's diagonal blocks and sets
+// it when they stay valid for more solves against the same triangle, n and
+// triangle flags; with *inv_ready true it reuses them.
 size_t gfx950_strsm_keep_ws_bytes(int m, int n);
 int gfx950_strsm_keep(bool lower, bool unit, int m, int n, float alpha, const float *a,
                       int64_t rs_a, int64_t cs_a, float *b, int64_t rs_b, int64_t cs_b,
-                      void *ws, bool *inv_ready, hipStream_t s);
+                      void *ws, size_t ws_bytes, bool *inv_ready, hipStream_t s);
 
 // Grouped triangular solves (the batched interface's TRSM problems), each in
 // the left form B := alpha * inv(A) * B: A m x m lower (lower = 1) or upper

@@ -25,17 +25,22 @@ __device__ inline bool stored(int uplo, int64_t delta) {
   return uplo == 1 ? delta >= 0 : (uplo == 2 ? delta <= 0 : true);
 }
 
+// `mask` (0 all, 1 lower, 2 upper) keeps only the elements on its side of
+// the same diagonal and writes zero elsewhere without reading: the driver's
+// jp / pi read domains applied per element.
 template <typename T, typename D = T>
 __global__ void pack_kernel(int m, int n, const T *__restrict__ src, int64_t rs,
                             int64_t cs, int struc, int uplo, int diag, int diagoff,
-                            D *__restrict__ dst, int64_t ld) {
+                            int mask, D *__restrict__ dst, int64_t ld) {
   const int64_t total = (int64_t)m * n;
   for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
        t += (int64_t)gridDim.x * blockDim.x) {
     const int64_t i = t % m, j = t / m;
     const int64_t d = diagoff + i - j;
     D v;
-    if (struc == 1) {  // SYMMETRIC: reflect when outside the stored triangle
+    if (!stored(mask, d)) {
+      v = D(0);
+    } else if (struc == 1) {  // SYMMETRIC: reflect when outside the stored triangle
       v = (D)(stored(uplo, d) ? src[i * rs + j * cs]
                               : src[(j - diagoff) * rs + (i + diagoff) * cs]);
     } else if (struc == 2) {  // TRIANGULAR
@@ -101,10 +106,14 @@ int gfx950_copy(int m, int n, const T *src, int64_t rs_s, int64_t cs_s, T *dst,
 
 template <typename T>
 int gfx950_pack(int m, int n, const T *src, int64_t rs, int64_t cs, int struc,
-                int uplo, int diag, int diagoff, T *dst, int64_t ld, hipStream_t s) {
+                int uplo, int diag, int diagoff, int mask, T *dst, int64_t ld,
+                hipStream_t s) {
   if (m <= 0 || n <= 0) return MUBLIS_OK;
+  if (struc < 0 || struc > 2 || uplo < 0 || uplo > 2 || diag < 0 || diag > 2 || mask < 0 ||
+      mask > 2)
+    return MUBLIS_EINVAL;
   hipLaunchKernelGGL(pack_kernel<T>, dim3(grid_for((int64_t)m * n)), dim3(256), 0, s,
-                     m, n, src, rs, cs, struc, uplo, diag, diagoff, dst, ld);
+                     m, n, src, rs, cs, struc, uplo, diag, diagoff, mask, dst, ld);
   return hipGetLastError() == hipSuccess ? MUBLIS_OK : MUBLIS_EHIP;
 }
 
@@ -134,7 +143,7 @@ int gfx950_pack_f2d(int m, int n, const float *src, int64_t rs, int64_t cs, int 
                     int uplo, int diag, double *dst, int64_t ld, hipStream_t s) {
   if (m <= 0 || n <= 0) return MUBLIS_OK;
   hipLaunchKernelGGL((pack_kernel<float, double>), dim3(grid_for((int64_t)m * n)),
-                     dim3(256), 0, s, m, n, src, rs, cs, struc, uplo, diag, 0, dst, ld);
+                     dim3(256), 0, s, m, n, src, rs, cs, struc, uplo, diag, 0, 0, dst, ld);
   return hipGetLastError() == hipSuccess ? MUBLIS_OK : MUBLIS_EHIP;
 }
 int gfx950_refine_update(int m, int n, const float *d, double *x64, float *out,
@@ -150,9 +159,9 @@ template int gfx950_copy<float>(int, int, const float *, int64_t, int64_t, float
 template int gfx950_copy<double>(int, int, const double *, int64_t, int64_t, double *,
                                  int64_t, int64_t, hipStream_t);
 template int gfx950_pack<float>(int, int, const float *, int64_t, int64_t, int, int,
-                                int, int, float *, int64_t, hipStream_t);
+                                int, int, int, float *, int64_t, hipStream_t);
 template int gfx950_pack<double>(int, int, const double *, int64_t, int64_t, int, int,
-                                 int, int, double *, int64_t, hipStream_t);
+                                 int, int, int, double *, int64_t, hipStream_t);
 template int gfx950_fill<float>(int, int, float, float *, int64_t, int64_t,
                                 hipStream_t);
 template int gfx950_fill<double>(int, int, double, double *, int64_t, int64_t,
@@ -164,15 +173,15 @@ static hipStream_t pick(void *stream) {
 
 extern "C" {
 int mublis_gfx950_dpack_dev(int m, int n, const double *src, int64_t rs, int64_t cs,
-                            int struc, int uplo, int diag, int diagoff, double *dst,
-                            int64_t ld, void *stream) {
-  return gfx950_pack<double>(m, n, src, rs, cs, struc, uplo, diag, diagoff, dst, ld,
+                            int struc, int uplo, int diag, int diagoff, int mask,
+                            double *dst, int64_t ld, void *stream) {
+  return gfx950_pack<double>(m, n, src, rs, cs, struc, uplo, diag, diagoff, mask, dst, ld,
                              pick(stream));
 }
 int mublis_gfx950_spack_dev(int m, int n, const float *src, int64_t rs, int64_t cs,
-                            int struc, int uplo, int diag, int diagoff, float *dst,
-                            int64_t ld, void *stream) {
-  return gfx950_pack<float>(m, n, src, rs, cs, struc, uplo, diag, diagoff, dst, ld,
+                            int struc, int uplo, int diag, int diagoff, int mask,
+                            float *dst, int64_t ld, void *stream) {
+  return gfx950_pack<float>(m, n, src, rs, cs, struc, uplo, diag, diagoff, mask, dst, ld,
                             pick(stream));
 }
 int mublis_gfx950_dcopy_dev(int m, int n, const double *src, int64_t rs_s,

@@ -34,7 +34,8 @@ extern "C" int mublis_gfx950_strsm_refined_dev(int lower, int unit, int m, int n
   float *r32 = (float *)mublis_rt_alloc(sizeof(float) * mn);
   // one TRSM workspace for the three FP32 solves: the triangle's inverted
   // diagonal blocks are computed by the first and reused by the others
-  void *tws = mublis_rt_alloc(gfx950_strsm_keep_ws_bytes(m, n));
+  const size_t tws_bytes = gfx950_strsm_keep_ws_bytes(m, n);
+  void *tws = mublis_rt_alloc(tws_bytes);
   bool inv_ready = false;
   int st = (t64 && b64 && x64 && r64 && r32 && tws) ? MUBLIS_OK : MUBLIS_ENOMEM;
   // the right-hand side and the triangle, exactly, in FP64
@@ -45,7 +46,7 @@ extern "C" int mublis_gfx950_strsm_refined_dev(int lower, int unit, int m, int n
   // X0 in FP32, in place
   if (st == MUBLIS_OK)
     st = gfx950_strsm_keep(lower, unit, m, n, alpha, a, rs_a, cs_a, b, rs_b, cs_b, tws,
-                           &inv_ready, s);
+                           tws_bytes, &inv_ready, s);
   if (st == MUBLIS_OK) st = gfx950_convert_f2d(m, n, b, rs_b, cs_b, x64, 1, m, s);
   for (int sw = 0; sw < sweeps && st == MUBLIS_OK; ++sw) {
     st = gfx950_copy<double>(m, n, b64, 1, m, r64, 1, m, s);
@@ -55,7 +56,7 @@ extern "C" int mublis_gfx950_strsm_refined_dev(int lower, int unit, int m, int n
     if (st == MUBLIS_OK) st = gfx950_convert_d2f(m, n, r64, 1, m, r32, 1, m, s);
     if (st == MUBLIS_OK)
       st = gfx950_strsm_keep(lower, unit, m, n, 1.0f, a, rs_a, cs_a, r32, 1, m, tws,
-                             &inv_ready, s);
+                             tws_bytes, &inv_ready, s);
     if (st == MUBLIS_OK) st = gfx950_refine_update(m, n, r32, x64, b, rs_b, cs_b, s);
   }
   if (hipStreamSynchronize(s) != hipSuccess && st == MUBLIS_OK) st = MUBLIS_EHIP;

@@ -24,6 +24,7 @@
 #include <stdint.h>
 
 #include <algorithm>
+#include <atomic>
 
 #include "gfx950_common.h"
 
@@ -663,11 +664,32 @@ __global__ __launch_bounds__(256, ROWS <= kGroupTrsmSmall ? 2 : 1) void grouped_
 // at most kPipeLeaf rows to the pipeline (narrower B) or to column-panel
 // workgroups (wider B, when the solve brings no room for the GEMM leaves'
 // inverses).
-int g_pipe_max_n = 4096;
+// Both knobs are set from any thread (mublis_gfx950_trsm_pipe_max_n /
+// _spin_limit, used by tests to force schedules) and read by solves on
+// others: they are atomics, and a solve reads them once, at its outermost
+// entry, into a per-thread snapshot that everything it enqueues uses (the
+// KnobScope below), so one solve never mixes two settings.
+std::atomic<int> g_pipe_max_n_set{4096};
 // Polls a pipeline wait makes before it gives up (about a second; the
 // dispatch order makes every wait end long before, this only bounds a
 // fault).
-int g_pipe_spins = 1 << 20;
+std::atomic<int> g_pipe_spins_set{1 << 20};
+struct PipeKnobs {
+  int max_n, spins, depth;
+};
+thread_local PipeKnobs t_knobs = {4096, 1 << 20, 0};
+struct KnobScope {
+  KnobScope() {
+    if (t_knobs.depth++ == 0) {
+      t_knobs.max_n = g_pipe_max_n_set.load(std::memory_order_relaxed);
+      t_knobs.spins = g_pipe_spins_set.load(std::memory_order_relaxed);
+    }
+  }
+  ~KnobScope() { --t_knobs.depth; }
+  KnobScope(const KnobScope &) = delete;
+  KnobScope &operator=(const KnobScope &) = delete;
+};
+
 constexpr int kPipeLeaf = 512;
 // From this many columns up, the pipelined leaves of at most kPanelRows
 // rows are column panels (n / 16 independent workgroups) instead.
@@ -675,7 +697,7 @@ constexpr int kPanelMinN = 2048;
 
 bool use_pipe(int m, int n) {
   const int64_t mn = (int64_t)m * n;
-  return m > NB && n <= g_pipe_max_n &&
+  return m > NB && n <= t_knobs.max_n &&
          (n <= 128 || (n <= 2048 && mn <= (int64_t(1) << 22)) ||
           (n <= 768 && mn <= (int64_t(1) << 23)));
 }
@@ -748,7 +770,7 @@ int pipe_launch(int m, int n, T alpha, const T *a, int64_t rs_a, int64_t cs_a, T
   // the kernel's own occupancy.
   const size_t lds = nz == 1 ? kLdsOneWg - sizeof(double) * pipe_lds_doubles(CW) : 0;
   hipLaunchKernelGGL((trsm_pipe<T, LOWER, CW>), dim3(nbk * cg, nz), dim3(256), lds, s, m, n,
-                     alpha, a, rs_a, cs_a, b, rs_b, cs_b, inv, flags, cg, epoch, g_pipe_spins,
+                     alpha, a, rs_a, cs_a, b, rs_b, cs_b, inv, flags, cg, epoch, t_knobs.spins,
                      bt);
   return hipGetLastError() == hipSuccess ? MUBLIS_OK : MUBLIS_EHIP;
 }
@@ -800,7 +822,7 @@ constexpr int kEyeCw = 32;
 
 template <typename T>
 bool use_inv_leaves(int m, int n) {
-  return m > kInvLeaf<T> && n >= kInvMinN<T> && g_pipe_max_n >= kInvLeaf<T>;
+  return m > kInvLeaf<T> && n >= kInvMinN<T> && t_knobs.max_n >= kInvLeaf<T>;
 }
 
 template <typename T>
@@ -1008,8 +1030,9 @@ template <typename T>
 int solve_ws(bool lower, int diag_mode, int m, int n, T alpha, const T *a, int64_t rs_a,
              int64_t cs_a, T *b, int64_t rs_b, int64_t cs_b, double *ad, T *iv, T *w, T *tmp,
              hipStream_t s, bool *inv_ready = nullptr) {
+  KnobScope knobs;
   const int nbk = (m + NB - 1) / NB;
-  const bool piped = m > NB && n <= g_pipe_max_n && flags_fit(m, n);
+  const bool piped = m > NB && n <= t_knobs.max_n && flags_fit(m, n);
   const bool gemm_ok = iv && w && use_inv_leaves<T>(m, n);
   const bool f32_leaves = sizeof(T) == 4 && gemm_ok && n > 128 &&
                           (int64_t)m * n >= (int64_t(1) << 21);
@@ -1051,6 +1074,7 @@ int solve_ws(bool lower, int diag_mode, int m, int n, T alpha, const T *a, int64
 template <typename T>
 int solve(bool lower, int diag_mode, int m, int n, T alpha, const T *a, int64_t rs_a,
           int64_t cs_a, T *b, int64_t rs_b, int64_t cs_b, hipStream_t s) {
+  KnobScope knobs;
   double *ad = (double *)mublis_rt_alloc(sizeof(double) * (size_t)ws_doubles(m));
   if (!ad) return MUBLIS_ENOMEM;
   T *iv = nullptr, *w = nullptr, *tmp = nullptr;
@@ -1167,6 +1191,42 @@ extern "C" int mublis_gfx950_strsm_dev(int lower, int unit, int m, int n, float 
                          stream);
 }
 
+// Workspace of the full schedule, which is the one a call with arena
+// workspace runs (solve): the 64-row inverses and flags, then, when the
+// solve takes GEMM leaves, the 1024-row inverses, the leaf copy of B and the
+// doubling's scratch, each 256-byte aligned.  With at least this much
+// workspace a workspace call runs exactly the schedule of the plain call and
+// gives its bits; with less (at least mublis_gfx950_trsm_ws_bytes(m)) it
+// runs the schedule that needs only the 64-row inverses.
+static size_t ws_part(size_t bytes) { return (bytes + 255) / 256 * 256; }
+
+template <typename T>
+size_t sched_ws_bytes(int m, int n) {
+  KnobScope knobs;
+  size_t bytes = ws_part(sizeof(double) * (size_t)ws_doubles(m));
+  if (use_inv_leaves<T>(m, n))
+    bytes += ws_part(sizeof(T) * (size_t)inv_leaf_elems<T>(m)) +
+             ws_part(sizeof(T) * (size_t)kInvLeaf<T> * (size_t)n) +
+             ws_part(sizeof(T) * (size_t)m * kInvLeaf<T> / 4);
+  return bytes;
+}
+
+// Carve `ws` as sched_ws_bytes<T>(m, n) lays it out; iv / w / tmp stay null
+// when the solve takes no GEMM leaves or ws_bytes is too small for them.
+template <typename T>
+void carve_ws(void *ws, size_t ws_bytes, int m, int n, double **ad, T **iv, T **w, T **tmp) {
+  char *p = (char *)ws;
+  *ad = (double *)p;
+  *iv = *w = *tmp = nullptr;
+  if (!use_inv_leaves<T>(m, n) || ws_bytes < sched_ws_bytes<T>(m, n)) return;
+  p += ws_part(sizeof(double) * (size_t)ws_doubles(m));
+  *iv = (T *)p;
+  p += ws_part(sizeof(T) * (size_t)inv_leaf_elems<T>(m));
+  *w = (T *)p;
+  p += ws_part(sizeof(T) * (size_t)kInvLeaf<T> * (size_t)n);
+  *tmp = (T *)p;
+}
+
 extern "C" size_t mublis_gfx950_trsm_ws_bytes(int m) {
   return m <= 0 ? 0 : sizeof(double) * (size_t)ws_doubles(m);
 }
@@ -1180,8 +1240,12 @@ static int trsm_ws_dev(int lower, int unit, int m, int n, T alpha, const T *a, i
   hipStream_t s = (hipStream_t)(stream ? stream : mublis_rt_stream());
   if (alpha == T(0)) return gfx950_fill<T>(m, n, T(0), b, rs_b, cs_b, s);
   if (!ws || ws_bytes < mublis_gfx950_trsm_ws_bytes(m)) return MUBLIS_EINVAL;
-  return solve_ws<T>(lower != 0, unit ? 1 : 0, m, n, alpha, a, rs_a, cs_a, b, rs_b, cs_b,
-                     (double *)ws, (T *)nullptr, (T *)nullptr, (T *)nullptr, s);
+  KnobScope knobs;
+  double *ad;
+  T *iv, *w, *tmp;
+  carve_ws<T>(ws, ws_bytes, m, n, &ad, &iv, &w, &tmp);
+  return solve_ws<T>(lower != 0, unit ? 1 : 0, m, n, alpha, a, rs_a, cs_a, b, rs_b, cs_b, ad,
+                     iv, w, tmp, s);
 }
 
 extern "C" int mublis_gfx950_dtrsm_ws_dev(int lower, int unit, int m, int n, double alpha,
@@ -1201,49 +1265,37 @@ extern "C" int mublis_gfx950_strsm_ws_dev(int lower, int unit, int m, int n, flo
 
 // FP32 solves against one triangle that keep its inverted diagonal blocks
 // in caller workspace between calls (the refined STRSM's three solves).
-// Layout: the 64-row inverses and flags, then, when the solve takes GEMM
-// leaves, the 1024-row inverses, the leaf copy of B and the doubling's
-// scratch, each 256-byte aligned.
-static size_t keep_part(size_t bytes) { return (bytes + 255) / 256 * 256; }
-
-size_t gfx950_strsm_keep_ws_bytes(int m, int n) {
-  size_t bytes = keep_part(sizeof(double) * (size_t)ws_doubles(m));
-  if (use_inv_leaves<float>(m, n))
-    bytes += keep_part(sizeof(float) * (size_t)inv_leaf_elems<float>(m)) +
-             keep_part(sizeof(float) * (size_t)kInvLeaf<float> * (size_t)n) +
-             keep_part(sizeof(float) * (size_t)m * kInvLeaf<float> / 4);
-  return bytes;
-}
+size_t gfx950_strsm_keep_ws_bytes(int m, int n) { return sched_ws_bytes<float>(m, n); }
 
 int gfx950_strsm_keep(bool lower, bool unit, int m, int n, float alpha, const float *a,
                       int64_t rs_a, int64_t cs_a, float *b, int64_t rs_b, int64_t cs_b,
-                      void *ws, bool *inv_ready, hipStream_t s) {
+                      void *ws, size_t ws_bytes, bool *inv_ready, hipStream_t s) {
   if (m < 0 || n < 0 || !ws) return MUBLIS_EINVAL;
   if (m == 0 || n == 0) return MUBLIS_OK;
   if (alpha == 0.0f) return gfx950_fill<float>(m, n, 0.0f, b, rs_b, cs_b, s);
-  char *p = (char *)ws;
-  double *ad = (double *)p;
-  p += keep_part(sizeof(double) * (size_t)ws_doubles(m));
-  float *iv = nullptr, *w = nullptr, *tmp = nullptr;
-  if (use_inv_leaves<float>(m, n)) {
-    iv = (float *)p;
-    p += keep_part(sizeof(float) * (size_t)inv_leaf_elems<float>(m));
-    w = (float *)p;
-    p += keep_part(sizeof(float) * (size_t)kInvLeaf<float> * (size_t)n);
-    tmp = (float *)p;
-  }
+  if (ws_bytes < mublis_gfx950_trsm_ws_bytes(m)) return MUBLIS_EINVAL;
+  KnobScope knobs;
+  double *ad;
+  float *iv, *w, *tmp;
+  carve_ws<float>(ws, ws_bytes, m, n, &ad, &iv, &w, &tmp);
   return solve_ws<float>(lower, unit ? 1 : 0, m, n, alpha, a, rs_a, cs_a, b, rs_b, cs_b, ad,
                          iv, w, tmp, s, inv_ready);
 }
 
+extern "C" size_t mublis_gfx950_dtrsm_sched_ws_bytes(int m, int n) {
+  return m <= 0 || n <= 0 ? 0 : sched_ws_bytes<double>(m, n);
+}
+extern "C" size_t mublis_gfx950_strsm_sched_ws_bytes(int m, int n) {
+  return m <= 0 || n <= 0 ? 0 : sched_ws_bytes<float>(m, n);
+}
+
 extern "C" int mublis_gfx950_trsm_pipe_spin_limit(int spins) {
-  const int old = g_pipe_spins;
-  if (spins > 0) g_pipe_spins = spins;
-  return old;
+  return spins > 0 ? g_pipe_spins_set.exchange(spins, std::memory_order_relaxed)
+                   : g_pipe_spins_set.load(std::memory_order_relaxed);
 }
 
 extern "C" int mublis_gfx950_trsm_pipe_max_n(int max_n) {
-  const int old = g_pipe_max_n;
-  if (max_n >= 0) g_pipe_max_n = std::min(max_n, 4096);
-  return old;
+  return max_n >= 0
+             ? g_pipe_max_n_set.exchange(std::min(max_n, 4096), std::memory_order_relaxed)
+             : g_pipe_max_n_set.load(std::memory_order_relaxed);
 }

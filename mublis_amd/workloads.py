@@ -12,54 +12,83 @@ import math
 
 import numpy as np
 
-from . import BATCH_GEMM, BATCH_SYRK, BATCH_TRSM, SbatchOp
+from . import BATCH_GEMM, BATCH_SYR2K, BATCH_SYRK, BATCH_TRSM, SbatchOp
 
 
 def _logu(rng, lo, hi, size):
     return np.exp(rng.uniform(np.log(lo), np.log(hi), size)).astype(np.int64).clip(lo, hi)
 
 
+_ALIGN = 64  # elements: every carved operand starts 256-byte aligned
+
+
+def _carve(torch, shapes, gen, device, dtype=None):
+    """Views of the given (cols, ld) shapes into ONE device allocation filled
+    uniform[-1, 1] by ONE kernel.  Building thousands of operands tensor by
+    tensor issued one allocation and two or three fill kernels each (about
+    43,000 launches for the batched workload); this issues one."""
+    offs, total = [], 0
+    for c, r in shapes:
+        offs.append(total)
+        total += (c * r + _ALIGN - 1) // _ALIGN * _ALIGN
+    pool = torch.empty(max(total, 1), device=device, dtype=dtype or torch.float32)
+    pool.uniform_(-1.0, 1.0, generator=gen)
+    return pool, [pool[o:o + c * r].view(c, r) for o, (c, r) in zip(offs, shapes)]
+
+
 def grouped_problems(torch, seed=314159, n_gemm=4096, n_syrk=1024, gemm_range=(16, 512),
                      syrk_n=(32, 1024), syrk_k=(16, 512), alpha=1.0, beta=1.0,
-                     vary_flags=False, device="cuda"):
+                     vary_flags=False, device="cuda", n_syr2k=0):
     """Returns (ops, keep, specs): SbatchOp list, the tensors they point at
-    (kept alive by the caller) and per-problem dicts for checking."""
+    (kept alive by the caller) and per-problem dicts for checking.  SYR2K
+    problems (n_syr2k, drawn like the SYRK ones) come last."""
     rng = np.random.default_rng(seed)
     g = torch.Generator(device=device).manual_seed(seed)
-    ops, keep, specs = [], [], []
-
-    def urand(rows, cols):
-        # column-major rows x cols = tensor (cols, rows)
-        return torch.rand((cols, rows), device=device, generator=g) * 2 - 1
-
     ms = _logu(rng, *gemm_range, n_gemm)
     ns = _logu(rng, *gemm_range, n_gemm)
     ks = _logu(rng, *gemm_range, n_gemm)
+    gflags = [(int(rng.integers(0, 2)), int(rng.integers(0, 2))) if vary_flags else (0, 0)
+              for _ in range(n_gemm)]
+    sn = _logu(rng, *syrk_n, n_syrk + n_syr2k)
+    sk = _logu(rng, *syrk_k, n_syrk + n_syr2k)
+    sflags = [(int(rng.integers(0, 2)), int(rng.integers(0, 2))) if vary_flags else (0, 0)
+              for _ in range(n_syrk + n_syr2k)]
+    # column-major rows x cols = tensor (cols, rows)
+    shapes = []
     for i in range(n_gemm):
         m, n, k = int(ms[i]), int(ns[i]), int(ks[i])
-        ta = int(rng.integers(0, 2)) if vary_flags else 0
-        tb = int(rng.integers(0, 2)) if vary_flags else 0
-        A = urand(k, m) if ta else urand(m, k)
-        B = urand(n, k) if tb else urand(k, n)
-        C = urand(m, n)
+        ta, tb = gflags[i]
+        shapes += [(m, k) if ta else (k, m), (k, n) if tb else (n, k), (n, m)]
+    for i in range(n_syrk + n_syr2k):
+        n, k = int(sn[i]), int(sk[i])
+        tr = sflags[i][0]
+        a_shape = (n, k) if tr else (k, n)
+        shapes += [a_shape] + ([a_shape] if i >= n_syrk else []) + [(n, n)]
+    pool, views = _carve(torch, shapes, g, device)
+    it = iter(views)
+    ops, keep, specs = [], [pool], []
+    for i in range(n_gemm):
+        m, n, k = int(ms[i]), int(ns[i]), int(ks[i])
+        ta, tb = gflags[i]
+        A, B, C = next(it), next(it), next(it)
         lda, ldb = A.shape[1], B.shape[1]
         ops.append(SbatchOp(BATCH_GEMM, ta, tb, 0, m, n, k, alpha, beta, A.data_ptr(), lda,
                             B.data_ptr(), ldb, C.data_ptr(), m))
-        keep += [A, B, C]
         specs.append(dict(op="gemm", m=m, n=n, k=k, ta=ta, tb=tb, A=A, B=B, C=C, lda=lda,
                           ldb=ldb))
-    sn = _logu(rng, *syrk_n, n_syrk)
-    sk = _logu(rng, *syrk_k, n_syrk)
-    for i in range(n_syrk):
+    for i in range(n_syrk + n_syr2k):
         n, k = int(sn[i]), int(sk[i])
-        tr = int(rng.integers(0, 2)) if vary_flags else 0
-        up = int(rng.integers(0, 2)) if vary_flags else 0
-        A = urand(k, n) if tr else urand(n, k)
-        C = urand(n, n)
-        ops.append(SbatchOp(BATCH_SYRK, tr, 0, 2 if up else 1, n, n, k, alpha, beta,
-                            A.data_ptr(), A.shape[1], None, 1, C.data_ptr(), n))
-        keep += [A, C]
-        specs.append(dict(op="syrk", n=n, k=k, trans=tr, upper=up, A=A, C=C, lda=A.shape[1]))
+        tr, up = sflags[i]
+        two = i >= n_syrk
+        A = next(it)
+        B = next(it) if two else None
+        C = next(it)
+        ops.append(SbatchOp(BATCH_SYR2K if two else BATCH_SYRK, tr, 0, 2 if up else 1, n, n,
+                            k, alpha, beta, A.data_ptr(), A.shape[1],
+                            B.data_ptr() if two else None, B.shape[1] if two else 1,
+                            C.data_ptr(), n))
+        specs.append(dict(op="syr2k" if two else "syrk", n=n, k=k, trans=tr, upper=up, A=A,
+                          B=B, C=C, lda=A.shape[1], ldb=B.shape[1] if two else 1))
     return ops, keep, specs
 
 
@@ -69,22 +98,35 @@ def grouped_trsm_problems(torch, seed=271828, count=1024, size=(16, 512), alpha=
     log-uniform in `size`, side, uplo and trans_a drawn uniformly, non-unit
     diagonal; A's stored triangle has off-diagonal entries uniform[-1, 1] /
     order and diagonal uniform[1, 2] (diagonally dominant), B uniform[-1, 1].
-    Returns (ops, keep, specs) like grouped_problems."""
+    The triangles are drawn on the host and uploaded in one copy, the
+    right-hand sides carved from one device pool.  Returns (ops, keep,
+    specs) like grouped_problems."""
     rng = np.random.default_rng(seed)
     g = torch.Generator(device=device).manual_seed(seed)
-    ops, keep, specs = [], [], []
     ms = _logu(rng, *size, count)
     ns = _logu(rng, *size, count)
+    flags = [tuple(int(x) for x in rng.integers(0, 2, size=3)) for _ in range(count)]
+    orders = [int(ns[i]) if flags[i][0] else int(ms[i]) for i in range(count)]
+    offs, total = [], 0
+    for o in orders:
+        offs.append(total)
+        total += (o * o + _ALIGN - 1) // _ALIGN * _ALIGN
+    host = np.empty(max(total, 1), np.float32)
+    for o, off in zip(orders, offs):
+        a = rng.uniform(-1.0, 1.0, (o, o)) / o
+        np.fill_diagonal(a, 1.0 + rng.uniform(0.0, 1.0, o))
+        host[off:off + o * o] = a.ravel()
+    apool = torch.from_numpy(host).to(device)
+    bpool, bviews = _carve(torch, [(int(ns[i]), int(ms[i])) for i in range(count)], g, device)
+    ops, keep, specs = [], [apool, bpool], []
     for i in range(count):
         m, n = int(ms[i]), int(ns[i])
-        side, up, tr = (int(x) for x in rng.integers(0, 2, size=3))
-        order = n if side else m
-        A = (torch.rand((order, order), device=device, generator=g) * 2 - 1) / order
-        A.diagonal().copy_(1 + torch.rand(order, device=device, generator=g))
-        B = torch.rand((n, m), device=device, generator=g) * 2 - 1
+        side, up, tr = flags[i]
+        order = orders[i]
+        A = apool[offs[i]:offs[i] + order * order].view(order, order)
+        B = bviews[i]
         ops.append(SbatchOp(BATCH_TRSM, tr, 0, 2 if up else 1, m, n, 0, alpha, 0.0,
                             A.data_ptr(), order, None, 1, B.data_ptr(), m, side, 0))
-        keep += [A, B]
         specs.append(dict(op="trsm", m=m, n=n, side=side, upper=up, trans=tr, A=A, B=B,
                           order=order))
     return ops, keep, specs

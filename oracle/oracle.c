@@ -85,17 +85,25 @@ VIEW_FN(double, long double, view_d)
 /*
  * Generic driver contract (frame/l3/l3_driver.c:49-249, include/mublis.h):
  * C(i,j) := alpha * sum_p Aview(i,p) * Bview(p,j) + beta * C(i,j) for every
- * (i, j) with ji(j, i).  B is viewed through its transpose with flipped uplo
- * when packed (l3_driver.c:133-142), which for a SYMMETRIC/TRIANGULAR B with
- * stored triangle `uplo` is the same as reading Bview(p, j) with `uplo`.
- * jp/pi do not change the value: the shims only set them for TRIANGULAR
- * operands whose excluded part is already zero.
+ * (i, j) with ji(j, i).  Operand views put the diagonal at i == p (A) and
+ * p == j (B) whatever the shape: packm is called with diagoff = ic - pc and
+ * jc - pc (l3_driver.c:133-161, packm.c:41), so an m x k TRIANGULAR A with
+ * m != k is a trapezoid and a SYMMETRIC one reads the reflected element
+ * (p, i) (packm.c:50-58).  B is packed through its transpose with flipped
+ * uplo, which reads Bview(p, j) with B's own `uplo`.
+ * jp and pi (include/mublis.h: "which (j, p) and (p, i) coordinates to
+ * read") are applied per term: term p of C(i,j) is included only when
+ * pi(p, i) and jp(j, p) hold.  The reference applies them per cache block
+ * (l3_driver.c:27-47,130,148,163,171), which equals this whenever the
+ * relation excludes only structural zeros (pi = uplo of a TRIANGULAR A, jp =
+ * uplo of a TRIANGULAR B) -- the only descriptors whose reference result
+ * does not depend on its block sizes.
  */
 #define DRIVER_FN(T, W, name, view)                                             \
   int name(int m, int n, int k, T alpha, const T *a, int rs_a, int cs_a,       \
            const T *b, int rs_b, int cs_b, T beta, T *c, int rs_c, int cs_c,   \
-           int ji, int a_struc, int a_uplo, int a_diag, int b_struc,           \
-           int b_uplo, int b_diag) {                                           \
+           int jp, int ji, int pi, int a_struc, int a_uplo, int a_diag,        \
+           int b_struc, int b_uplo, int b_diag) {                              \
     if (m == 0 || n == 0) return 0;                                            \
     for (long j = 0; j < n; j++)                                               \
       for (long i = 0; i < m; i++) {                                           \
@@ -103,9 +111,12 @@ VIEW_FN(double, long double, view_d)
         T *cij = &c[i * rs_c + j * cs_c];                                      \
         W acc = 0;                                                             \
         if (alpha != 0)                                                        \
-          for (long p = 0; p < k; p++)                                         \
+          for (long p = 0; p < k; p++) {                                       \
+            if (!rel_holds(pi, (int)p, (int)i) || !rel_holds(jp, (int)j, (int)p)) \
+              continue;                                                        \
             acc += view(a, rs_a, cs_a, i, p, a_struc, a_uplo, a_diag) *        \
                    view(b, rs_b, cs_b, p, j, b_struc, b_uplo, b_diag);         \
+          }                                                                    \
         W prev = (beta == 0) ? (W)0 : (W)beta * (W)*cij;                        \
         *cij = (T)((W)alpha * acc + prev);                                     \
       }                                                                        \
@@ -146,8 +157,8 @@ DRIVER_FN(double, long double, orc_ddriver, view_d)
     if (t1 == TR) { rs1 = l1; cs1 = 1; }                                       \
     if (t2 == TR) { rs2 = l2; cs2 = 1; }                                       \
     return drv(m, n, K, alpha, x1, rs1, cs1, x2, rs2, cs2, beta, C, 1, ldc,   \
-               R_ALL, S_GENERAL, M_DENSE, D_NONUNIT, S_GENERAL, M_DENSE,       \
-               D_NONUNIT);                                                     \
+               R_ALL, R_ALL, R_ALL, S_GENERAL, M_DENSE, D_NONUNIT, S_GENERAL,  \
+               M_DENSE, D_NONUNIT);                                            \
   }
 GEMM_FN(float, orc_sgemm, orc_sdriver)
 GEMM_FN(double, orc_dgemm, orc_ddriver)
@@ -175,9 +186,11 @@ GEMM_FN(double, orc_dgemm, orc_ddriver)
     int mu = ul == UP ? M_UPPER : M_LOWER;                                     \
     if (sd == LEFT)                                                            \
       return drv(m, n, m, alpha, A, 1, lda, B, 1, ldb, beta, C, 1, ldc, R_ALL, \
-                 S_SYMMETRIC, mu, D_NONUNIT, S_GENERAL, M_DENSE, D_NONUNIT);   \
+                 R_ALL, R_ALL, S_SYMMETRIC, mu, D_NONUNIT, S_GENERAL, M_DENSE, \
+                 D_NONUNIT);                                                   \
     return drv(m, n, n, alpha, B, 1, ldb, A, 1, lda, beta, C, 1, ldc, R_ALL,   \
-               S_GENERAL, M_DENSE, D_NONUNIT, S_SYMMETRIC, mu, D_NONUNIT);     \
+               R_ALL, R_ALL, S_GENERAL, M_DENSE, D_NONUNIT, S_SYMMETRIC, mu,   \
+               D_NONUNIT);                                                     \
   }
 SYMM_FN(float, orc_ssymm, orc_sdriver)
 SYMM_FN(double, orc_dsymm, orc_ddriver)

@@ -362,3 +362,195 @@ def golden_mask(c, n):
     return None
 
 
+
+
+# ------------------------------------------------ caller-built descriptors --
+# Generic-driver calls with descriptors the caller builds
+# (include/mublis.h:41-83): trapezoidal TRIANGULAR operands (m != k), the
+# UNIT and INVERT diagonals, SYMMETRIC operands on either side (offset when
+# m != k), ji masks on rectangular C, jp / pi read domains, and strides with
+# no unit stride.  Enum values are mublis.h's: struc G/S/T = 0/1/2, uplo
+# dense/lower/upper = 0/1/2, diag nonunit/unit/invert = 0/1/2, relations
+# all/lower/upper = 0/1/2.
+G, S, T = 0, 1, 2
+DENSE, MLO, MUP = 0, 1, 2
+DN, DU, DI = 0, 1, 2
+RALL, RLO, RUP = 0, 1, 2
+
+# name, m, n, k, A (struc, uplo, diag), B (struc, uplo, diag),
+# (jp, ji, pi), alpha, beta, storage ("col", "row" or "gen" per operand A, B, C)
+DRIVER_CASES = [
+    ("triA_lo_tall", 150, 70, 110, (T, MLO, DN), (G, DENSE, DN), (RALL, RALL, RLO), 1.25, -0.5,
+     ("col", "col", "col")),
+    ("triA_up_wide_unit", 90, 60, 170, (T, MUP, DU), (G, DENSE, DN), (RALL, RALL, RUP), 0.75,
+     0.0, ("col", "row", "col")),
+    ("triA_lo_invert", 101, 53, 101, (T, MLO, DI), (G, DENSE, DN), (RALL, RALL, RALL), 1.0, 1.0,
+     ("row", "col", "col")),
+    ("triA_up_tall_invert", 130, 40, 75, (T, MUP, DI), (G, DENSE, DN), (RALL, RALL, RUP), -1.5,
+     0.25, ("col", "col", "row")),
+    ("triB_lo_tall", 64, 90, 140, (G, DENSE, DN), (T, MLO, DN), (RLO, RALL, RALL), 1.0, 0.5,
+     ("col", "col", "col")),
+    ("triB_up_wide_unit", 77, 160, 100, (G, DENSE, DN), (T, MUP, DU), (RUP, RALL, RALL), 2.0,
+     0.0, ("row", "row", "col")),
+    ("triB_lo_invert", 50, 96, 96, (G, DENSE, DN), (T, MLO, DI), (RLO, RALL, RALL), 1.0, -1.0,
+     ("col", "gen", "col")),
+    ("symA_lo_tall", 140, 50, 90, (S, MLO, DN), (G, DENSE, DN), (RALL, RALL, RALL), 1.0, 0.5,
+     ("col", "col", "col")),
+    ("symA_up_wide", 70, 45, 130, (S, MUP, DN), (G, DENSE, DN), (RALL, RALL, RALL), -0.5, 1.0,
+     ("row", "col", "col")),
+    ("symB_lo", 60, 88, 88, (G, DENSE, DN), (S, MLO, DN), (RALL, RALL, RALL), 1.0, 0.0,
+     ("col", "col", "col")),
+    ("symB_up_offset", 66, 50, 120, (G, DENSE, DN), (S, MUP, DN), (RALL, RALL, RALL), 1.5, 0.5,
+     ("col", "row", "col")),
+    ("ji_lo_rect", 130, 95, 60, (G, DENSE, DN), (G, DENSE, DN), (RALL, RLO, RALL), 1.0, 0.5,
+     ("col", "col", "col")),
+    ("ji_up_rect", 80, 140, 70, (G, DENSE, DN), (G, DENSE, DN), (RALL, RUP, RALL), 1.0, 2.0,
+     ("row", "col", "row")),
+    ("both_tri", 120, 110, 115, (T, MLO, DU), (T, MUP, DI), (RUP, RALL, RLO), 1.0, 0.5,
+     ("col", "col", "col")),
+    ("tri_ji_gen_strides", 97, 83, 91, (T, MUP, DN), (S, MLO, DN), (RALL, RLO, RUP), 0.5, -0.5,
+     ("gen", "gen", "gen")),
+    ("alpha0_tri", 40, 30, 50, (T, MLO, DU), (G, DENSE, DN), (RALL, RUP, RLO), 0.0, 0.5,
+     ("col", "col", "col")),
+    ("deep_k_blocks", 60, 36, 300, (T, MLO, DN), (T, MLO, DN), (RLO, RALL, RLO), 1.0, 0.0,
+     ("col", "col", "col")),
+]
+
+# Inconsistent read domains (jp / pi on GENERAL operands): the reference's
+# result depends on its cache-block sizes there, so only the per-term
+# definition (and the product against the oracle) is checked.
+DRIVER_MASK_ONLY = [
+    ("mask_pi_general", 90, 70, 80, (G, DENSE, DN), (G, DENSE, DN), (RALL, RALL, RLO), 1.0, 0.5,
+     ("col", "col", "col")),
+    ("mask_jp_general", 70, 90, 85, (G, DENSE, DN), (G, DENSE, DN), (RUP, RLO, RALL), 1.0, 0.0,
+     ("row", "col", "col")),
+]
+
+
+def _extent(struc, uplo, rows, cols):
+    """Storage box an operand view reads (the reflected reads of SYMMETRIC)."""
+    if struc == S and rows != cols:
+        lo, hi = min(rows, cols), max(rows, cols)
+        if uplo == MLO:
+            return hi, lo
+        if uplo == MUP:
+            return lo, hi
+    return rows, cols
+
+
+def _strided(rows, cols, kind):
+    """(flat size, rs, cs) of rows x cols storage of a given kind."""
+    if kind == "col":
+        ld = rows + 3
+        return ld * cols, 1, ld
+    if kind == "row":
+        ld = cols + 2
+        return ld * rows, ld, 1
+    rs, cs = 2, 2 * rows + 3  # no unit stride
+    return (rows - 1) * rs + (cols - 1) * cs + 1, rs, cs
+
+
+def _stored(uplo, d):
+    return d >= 0 if uplo == MLO else (d <= 0 if uplo == MUP else True)
+
+
+def _rel(rel, lhs, rhs):
+    return lhs <= rhs if rel == RLO else (lhs >= rhs if rel == RUP else True)
+
+
+def _operand(rng, rows, cols, desc, mask, kind, dt):
+    """Storage for a rows x cols operand view (flat buffer + strides), NaN
+    wherever the view must not read, and the view itself in float64."""
+    struc, uplo, diag = desc
+    er, ec = _extent(struc, uplo, rows, cols)
+    size, rs, cs = _strided(er, ec, kind)
+    buf = np.full(size, np.nan)
+    vals = rng.standard_normal((er, ec))
+    for i in range(er):
+        for p in range(ec):
+            d = i - p
+            if struc == T and (not _stored(uplo, d) or (d == 0 and diag == DU)):
+                continue
+            if struc == S and not _stored(uplo, d):
+                continue
+            v = vals[i, p]
+            if struc == T and d == 0 and diag == DI:
+                v = rng.uniform(1.5, 2.5) * (1 if v >= 0 else -1)
+            if struc == G and i < rows and p < cols and not _rel(mask, p, i):
+                continue  # excluded by the read domain: never read
+            buf[i * rs + p * cs] = v
+    buf = buf.astype(dt)
+    x = buf.astype(np.float64)
+    view = np.zeros((rows, cols))
+    for i in range(rows):
+        for p in range(cols):
+            d = i - p
+            if not _rel(mask, p, i):
+                continue
+            if struc == S:
+                view[i, p] = x[i * rs + p * cs] if _stored(uplo, d) else x[p * rs + i * cs]
+            elif struc == T:
+                if not _stored(uplo, d):
+                    continue
+                if d == 0 and diag == DU:
+                    view[i, p] = 1.0
+                elif d == 0 and diag == DI:
+                    view[i, p] = 1.0 / x[i * rs + p * cs]
+                else:
+                    view[i, p] = x[i * rs + p * cs]
+            else:
+                view[i, p] = x[i * rs + p * cs]
+    return buf, rs, cs, view
+
+
+def driver_problem(case, dt, seed=SEED):
+    """Inputs of one caller-built driver call, its expected value in float64,
+    and the mask of C elements it may write."""
+    name, m, n, k, da, db, (jp, ji, pi), alpha, beta, kinds = case
+    rng = np.random.default_rng([seed, sum(map(ord, name))])
+    a, rs_a, cs_a, av = _operand(rng, m, k, da, pi, kinds[0], dt)
+    # jp(j, p) for B(p, j) is "column <= row" for LOWER, like pi for A(i, p)
+    b, rs_b, cs_b, bv = _operand(rng, k, n, db, jp, kinds[1], dt)
+    size, rs_c, cs_c = _strided(m, n, kinds[2])
+    c = np.full(size, np.nan)
+    idx = np.add.outer(np.arange(m) * rs_c, np.arange(n) * cs_c)
+    c[idx] = rng.standard_normal((m, n))
+    c = c.astype(dt)
+    if beta == 0:
+        c[idx] = np.nan  # beta == 0 never reads C
+    write = np.array([[_rel(ji, j, i) for j in range(n)] for i in range(m)])
+    c0 = c[idx].astype(np.float64)
+    exact = alpha * (av @ bv) if alpha != 0 else np.zeros((m, n))
+    if beta != 0:
+        exact = exact + beta * c0
+    return dict(m=m, n=n, k=k, alpha=alpha, beta=beta, a=a, rs_a=rs_a, cs_a=cs_a, b=b,
+                rs_b=rs_b, cs_b=cs_b, c=c, rs_c=rs_c, cs_c=cs_c, idx=idx, write=write,
+                exact=exact, av=av, bv=bv, c0=c0,
+                desc=(jp, ji, pi) + tuple(da) + tuple(db))
+
+
+class Product(ctypes.Structure):
+    """mublis_l3_product_t (include/mublis.h:41-63): nine enum ints."""
+    _fields_ = [(f, ctypes.c_int) for f in
+                ("jp", "ji", "pi", "a_struc", "a_uplo", "a_diag", "b_struc", "b_uplo",
+                 "b_diag")]
+
+
+def driver_fn(lib, dt):
+    """The library's mublis_l3_?driver with its argtypes set."""
+    ct = ctypes.c_float if np.dtype(dt) == np.float32 else ctypes.c_double
+    f = getattr(lib, "mublis_l3_sdriver" if ct is ctypes.c_float else "mublis_l3_ddriver")
+    f.argtypes = [_I, _I, _I, ct, _VP, _I, _I, _VP, _I, _I, ct, _VP, _I, _I,
+                  ctypes.POINTER(Product)]
+    f.restype = _I
+    return f
+
+
+def driver_bound(prob, dt):
+    """Normwise bound on the computed C: 2 (k + 2) eps (|alpha| ||Av|| ||Bv||
+    + |beta| ||C0||) over the written elements."""
+    eps = np.finfo(dt).eps
+    nb = abs(prob["beta"]) * np.linalg.norm(np.where(prob["write"], prob["c0"], 0)) \
+        if prob["beta"] else 0.0
+    return 2 * (prob["k"] + 2) * eps * (abs(prob["alpha"]) * np.linalg.norm(prob["av"]) *
+                                        np.linalg.norm(prob["bv"]) + nb)

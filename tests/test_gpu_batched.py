@@ -37,10 +37,14 @@ def single_calls(lib, torch, specs, initial, alpha, beta, times=1):
                 lib.cblas_sgemm(102, 112 if s["ta"] else 111, 112 if s["tb"] else 111, s["m"],
                                 s["n"], s["k"], alpha, M.ptr(s["A"]), s["lda"], M.ptr(s["B"]),
                                 s["ldb"], beta, M.ptr(C), s["m"])
-            else:
+            elif s["op"] == "syrk":
                 lib.cblas_ssyrk(102, 121 if s["upper"] else 122, 112 if s["trans"] else 111,
                                 s["n"], s["k"], alpha, M.ptr(s["A"]), s["lda"], beta, M.ptr(C),
                                 s["n"])
+            else:
+                lib.cblas_ssyr2k(102, 121 if s["upper"] else 122, 112 if s["trans"] else 111,
+                                 s["n"], s["k"], alpha, M.ptr(s["A"]), s["lda"], M.ptr(s["B"]),
+                                 s["ldb"], beta, M.ptr(C), s.get("ldc", s["n"]))
         outs.append(C)
     torch.cuda.synchronize()
     return outs
@@ -66,6 +70,64 @@ def test_grouped_equals_single_calls(lib, alpha, beta):
     assert info["items"] > 0 and sum(info["tiles"]) == len(ops)
     bad = [i for i, (s, w) in enumerate(zip(specs, want)) if not torch.equal(s["C"], w)]
     assert not bad, f"{len(bad)} problems differ, first {specs[bad[0]]['op']}"
+
+
+def _syr2k_edge(torch, n, k, lda, ldb, trans, upper, seed):
+    """One SYR2K problem with explicit leading dimensions (column-major
+    storage as (cols, ld) tensors) for the extent-1 layout cases."""
+    g = torch.Generator(device="cuda").manual_seed(seed)
+    cols = n if trans else k
+    A = torch.rand((cols, lda), device="cuda", generator=g) * 2 - 1
+    B = torch.rand((cols, ldb), device="cuda", generator=g) * 2 - 1
+    C = torch.rand((n, n), device="cuda", generator=g) * 2 - 1
+    op = M.SbatchOp(M.BATCH_SYR2K, trans, 0, 2 if upper else 1, n, n, k, 1.0, 1.0,
+                    A.data_ptr(), lda, B.data_ptr(), ldb, C.data_ptr(), n)
+    return op, dict(op="syr2k", n=n, k=k, trans=trans, upper=upper, A=A, B=B, C=C, lda=lda,
+                    ldb=ldb)
+
+
+def test_grouped_syr2k_equals_single_calls(lib):
+    """Grouped SYR2K problems (both products as K segments of one tile loop)
+    give the bits of single cblas_ssyr2k calls, including n = 1 with lda = 1
+    and ldb = 5 (and the reverse, and transposed): the two products' operand
+    strides differ there, and both paths read them with one layout after
+    normalising the stride of the extent-1 dimension."""
+    torch = pytest.importorskip("torch")
+    ops, keep, specs = grouped_problems(torch, seed=12, n_gemm=30, n_syrk=10, n_syr2k=90,
+                                        syrk_n=(1, 400), syrk_k=(1, 300), vary_flags=True)
+    edges = [(1, 7, 1, 5, 0, 0), (1, 7, 5, 1, 0, 1), (1, 9, 9, 12, 1, 0), (1, 9, 12, 9, 1, 1),
+             (1, 1, 1, 3, 0, 0), (2, 33, 2, 6, 0, 1), (40, 1, 40, 41, 1, 0)]
+    for i, (n, k, lda, ldb, tr, up) in enumerate(edges):
+        o, sp = _syr2k_edge(torch, n, k, lda, ldb, tr, up, seed=100 + i)
+        ops.append(o)
+        specs.append(sp)
+    initial = [s["C"].clone() for s in specs]
+    want = single_calls(lib, torch, specs, initial, 1.0, 1.0)
+    for s, c0 in zip(specs, initial):
+        s["C"].copy_(c0)
+    plan = M.GroupedPlan(ops)
+    try:
+        plan.execute()
+        torch.cuda.synchronize()
+    finally:
+        plan.close()
+    bad = [i for i, (s, w) in enumerate(zip(specs, want)) if not torch.equal(s["C"], w)]
+    assert not bad, [(specs[i]["op"], specs[i]["n"], specs[i]["k"]) for i in bad[:5]]
+    # the edge problems against the oracle, so equal bits are also right bits
+    orc = L.Cblas(L.ORACLE_LIB, prefix="orc_", returns_status=True)
+    eps = np.finfo(np.float32).eps
+    for s, c0 in zip(specs[-len(edges):], initial[-len(edges):]):
+        n, k = s["n"], s["k"]
+        a = np.asfortranarray(s["A"].cpu().numpy().T)
+        b = np.asfortranarray(s["B"].cpu().numpy().T)
+        ref = np.asfortranarray(c0.cpu().numpy().T)
+        c00 = ref.copy(order="F")
+        assert orc.call(np.float32, "syr2k", 102, 121 if s["upper"] else 122,
+                        112 if s["trans"] else 111, n, k, 1.0, a, s["lda"], b, s["ldb"], 1.0,
+                        ref, n) == 0
+        got = s["C"].cpu().numpy().T
+        scale = 2 * np.linalg.norm(a) * np.linalg.norm(b) + np.linalg.norm(c00)
+        assert np.linalg.norm(got - ref) <= 2 * (k + 2) * eps * scale, (n, k)
 
 
 def test_forced_tiles_and_replay(lib):
