@@ -42,16 +42,46 @@ struct GroupItem {
   int prob, tm, tn, kind;  // kind = tile index * 4 + layout (akc | bnc << 1)
 };
 
+// The device descriptor of one grouped problem: what the per-tile routine
+// reads, with 32-bit leading dimensions.  The unit strides follow from the
+// item's layout and C is column-major, so every layout case of run_tile
+// builds its GemmArgs with those strides as constants -- the full GemmArgs
+// (64-bit strides of every operand) held in scalar registers had the
+// grouped kernels spilling SGPRs to VGPR lanes.
+struct GroupProb {
+  const float *a, *b, *a2, *b2;
+  float *c;
+  int m, n, k, lda, ldb, lda2, ldb2, ldc;
+  float alpha, beta;
+  int ji, vec;  // vec bits: 1 a, 2 b, 4 a2, 8 b2 (whole 16-byte vectors may be loaded)
+};
+
+template <bool AKC, bool BNC>
+__device__ inline GemmArgs<float> args_of(const GroupProb &g) {
+  GemmArgs<float> q{};
+  q.m = g.m; q.n = g.n; q.k = g.k; q.alpha = g.alpha; q.beta = g.beta;
+  q.a = g.a; q.rs_a = AKC ? g.lda : 1; q.cs_a = AKC ? 1 : g.lda;
+  q.b = g.b; q.rs_b = BNC ? g.ldb : 1; q.cs_b = BNC ? 1 : g.ldb;
+  q.a2 = g.a2; q.rs_a2 = AKC ? g.lda2 : 1; q.cs_a2 = AKC ? 1 : g.lda2;
+  q.b2 = g.b2; q.rs_b2 = BNC ? g.ldb2 : 1; q.cs_b2 = BNC ? 1 : g.ldb2;
+  q.c = g.c; q.rs_c = 1; q.cs_c = g.ldc; q.ji = g.ji;
+  q.vec_a = g.vec & 1; q.vec_b = (g.vec >> 1) & 1; q.vec_a2 = (g.vec >> 2) & 1;
+  q.vec_b2 = (g.vec >> 3) & 1;
+  return q;
+}
+
 constexpr int kTiles[3] = {32, 64, 128};
 
+
+// SMALL (gemm_tile's mode for small tiles) for the 64- and 32-tile items.
 template <int TILE, int BK, int NTH>
-__device__ inline void run_tile(const GemmArgs<float> &p, int tm, int tn, int layout,
-                                float *lds) {
+__device__ inline void run_tile(const GroupProb &g, int tm, int tn, int layout, float *lds) {
+  constexpr bool SM = TILE <= 64;
   switch (layout) {
-    case 0: gemm_tile<float, TILE, false, false, BK, NTH>(p, tm, tn, lds); break;
-    case 1: gemm_tile<float, TILE, true, false, BK, NTH>(p, tm, tn, lds); break;
-    case 2: gemm_tile<float, TILE, false, true, BK, NTH>(p, tm, tn, lds); break;
-    default: gemm_tile<float, TILE, true, true, BK, NTH>(p, tm, tn, lds); break;
+    case 0: gemm_tile<float, TILE, false, false, BK, NTH, SM>(args_of<false, false>(g), tm, tn, lds); break;
+    case 1: gemm_tile<float, TILE, true, false, BK, NTH, SM>(args_of<true, false>(g), tm, tn, lds); break;
+    case 2: gemm_tile<float, TILE, false, true, BK, NTH, SM>(args_of<false, true>(g), tm, tn, lds); break;
+    default: gemm_tile<float, TILE, true, true, BK, NTH, SM>(args_of<true, true>(g), tm, tn, lds); break;
   }
 }
 
@@ -64,16 +94,22 @@ __device__ inline void run_tile(const GemmArgs<float> &p, int tm, int tn, int la
 // per replay; 3 per CU of 8 waves: 1.589; 4 of 8 waves spill: 1.643.)
 template <int MAXT, int WG_PER_CU, int BK, int NTH = NT>
 __global__ __launch_bounds__(NTH, WG_PER_CU * NTH / 256) void grouped_kernel(
-    const GemmArgs<float> *__restrict__ probs, const GroupItem *__restrict__ items,
-    int nitems) {
+    const GroupProb *__restrict__ probs, const GroupItem *__restrict__ items, int nitems) {
   __shared__ float lds[LdsSize<float, MAXT, BK>::value];
-  for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
-    const GroupItem w = items[it];
-    const GemmArgs<float> p = probs[w.prob];
-    const int tile = w.kind >> 2, layout = w.kind & 3;
-    (void)tile;  // every item of a launch has the launch's tile shape
-    run_tile<MAXT, BK, NTH>(p, w.tm, w.tn, layout, lds);
-    __syncthreads();  // the next item restages lds
+  int it = blockIdx.x;
+  if (it >= nitems) return;
+  GroupItem w = items[it];
+  for (;;) {
+    // the next item's entry is read while this one runs
+    const int nx = it + gridDim.x;
+    const GroupItem wn = items[nx < nitems ? nx : it];
+    const GroupProb g = probs[w.prob];
+    // every item of a launch has the launch's tile shape (kind >> 2)
+    run_tile<MAXT, BK, NTH>(g, w.tm, w.tn, w.kind & 3, lds);
+    __syncthreads();  // the next item restages lds (gemm_tile's SMALL mode relies on it)
+    if (nx >= nitems) break;
+    it = nx;
+    w = wn;
   }
 }
 
@@ -138,14 +174,17 @@ int choose_tile(int m, int n, int k) {
   return it != g_tab.end() ? it->second : heuristic_tile(m, n, k);
 }
 
-bool aligned16f(const float *x, int64_t ld) {
-  return ((uintptr_t)x % 16 == 0) && ((ld * 4) % 16 == 0);
+// Staged slices load 16-byte vectors wherever a whole vector is in range;
+// gfx950's global loads need only element (dword) alignment for that.
+bool vec_ok_f(const float *x, int64_t ld) {
+  (void)ld;
+  return (uintptr_t)x % sizeof(float) == 0;
 }
 
 }  // namespace
 
 struct mublis_grouped_plan_s {
-  GemmArgs<float> *d_probs = nullptr;
+  GroupProb *d_probs = nullptr;
   GroupItem *d_items = nullptr;
   int nitems = 0, ncls[3] = {0, 0, 0};
   void *d_trsm = nullptr;  // TRSM descriptors, work lists, inverses, solo workspace
@@ -186,6 +225,7 @@ struct SoloTrsm {
 
 struct Work {
   std::vector<GemmArgs<float>> probs;
+  std::vector<GroupProb> gprobs;  // the device form of probs
   std::vector<GroupItem> items;  // by tile shape (128, 64, 32), each by cost
   int ncls[3] = {0, 0, 0};       // items per tile shape (index 0: 32, 1: 64, 2: 128)
   double flops = 0, bytes = 0;
@@ -315,11 +355,11 @@ int build_work(const mublis_sbatch_op_t *ops, int count, int flags, Work &w) {
                                  pair ? &p.cs_b2 : nullptr, false);
     if (la < 0 || lb < 0) return MUBLIS_EINVAL;
     const bool akc = la == 1, bnc = lb == 1;
-    p.vec_a = aligned16f(p.a, akc ? p.rs_a : p.cs_a);
-    p.vec_b = aligned16f(p.b, bnc ? p.rs_b : p.cs_b);
+    p.vec_a = vec_ok_f(p.a, akc ? p.rs_a : p.cs_a);
+    p.vec_b = vec_ok_f(p.b, bnc ? p.rs_b : p.cs_b);
     if (syr2k) {
-      p.vec_a2 = aligned16f(p.a2, akc ? p.rs_a2 : p.cs_a2);
-      p.vec_b2 = aligned16f(p.b2, bnc ? p.rs_b2 : p.cs_b2);
+      p.vec_a2 = vec_ok_f(p.a2, akc ? p.rs_a2 : p.cs_a2);
+      p.vec_b2 = vec_ok_f(p.b2, bnc ? p.rs_b2 : p.cs_b2);
     }
     if (p.k == 0) p.a2 = p.b2 = nullptr;
     const int tix = forced ? forced - 1 : choose_tile(m, n, k);
@@ -329,6 +369,17 @@ int build_work(const mublis_sbatch_op_t *ops, int count, int flags, Work &w) {
     const int tmn = (m + T - 1) / T, tnn = (n + T - 1) / T;
     const int pid = (int)w.probs.size();
     w.probs.push_back(p);
+    GroupProb g;
+    g.a = p.a; g.b = p.b; g.a2 = p.a2; g.b2 = p.b2; g.c = p.c;
+    g.m = p.m; g.n = p.n; g.k = p.k;
+    g.lda = (int)(akc ? p.rs_a : p.cs_a);
+    g.ldb = (int)(bnc ? p.rs_b : p.cs_b);
+    g.lda2 = p.a2 ? (int)(akc ? p.rs_a2 : p.cs_a2) : 0;
+    g.ldb2 = p.b2 ? (int)(bnc ? p.rs_b2 : p.cs_b2) : 0;
+    g.ldc = o.ldc;
+    g.alpha = p.alpha; g.beta = p.beta; g.ji = p.ji;
+    g.vec = (p.vec_a ? 1 : 0) | (p.vec_b ? 2 : 0) | (p.vec_a2 ? 4 : 0) | (p.vec_b2 ? 8 : 0);
+    w.gprobs.push_back(g);
     const double tri = n * (n + 1) / 2.0;
     w.flops += syrk ? (double)n * (n + 1) * k : syr2k ? 2.0 * n * (n + 1) * k
                                                       : 2.0 * m * n * k;
@@ -462,18 +513,21 @@ int persistent_grid(int nitems, int per_cu) {
 // graph branches (1.86 vs 2.69 ms for two classes) and one launch for all
 // items at 2 workgroups per CU (2.27 ms).
 constexpr int W128 = 2, BK128 = 32, W64 = 3, BK64 = 32, W32 = 6, BK32 = 32;
-int launch_items(const GemmArgs<float> *probs, const GroupItem *items, const int *ncls,
+constexpr int NT64 = 512;  // threads of a 64-tile workgroup
+int launch_items(const GroupProb *probs, const GroupItem *items, const int *ncls,
                  hipStream_t s) {
   const int n128 = ncls[2], n64 = ncls[1], n32 = ncls[0];
   if (n128 > 0)
-    hipLaunchKernelGGL((grouped_kernel<128, W128, BK128>), dim3(persistent_grid(n128, W128)),
-                       dim3(NT), 0, s, probs, items, n128);
+    hipLaunchKernelGGL((grouped_kernel<128, W128, BK128, NT>),
+                       dim3(persistent_grid(n128, W128)), dim3(NT), 0, s, probs, items, n128);
   if (n64 > 0)
-    hipLaunchKernelGGL((grouped_kernel<64, W64, BK64, 512>), dim3(persistent_grid(n64, W64)),
-                       dim3(512), 0, s, probs, items + n128, n64);
+    hipLaunchKernelGGL((grouped_kernel<64, W64, BK64, NT64>),
+                       dim3(persistent_grid(n64, W64)), dim3(NT64), 0, s, probs, items + n128,
+                       n64);
   if (n32 > 0)
-    hipLaunchKernelGGL((grouped_kernel<32, W32, BK32>), dim3(persistent_grid(n32, W32)),
-                       dim3(NT), 0, s, probs, items + n128 + n64, n32);
+    hipLaunchKernelGGL((grouped_kernel<32, W32, BK32, NT>),
+                       dim3(persistent_grid(n32, W32)), dim3(NT), 0, s, probs,
+                       items + n128 + n64, n32);
   return hipGetLastError() == hipSuccess ? MUBLIS_OK : MUBLIS_EHIP;
 }
 
@@ -492,18 +546,18 @@ int mublis_sgrouped_plan_create(const mublis_sbatch_op_t *ops, int count, int fl
   plan->flops = w.flops;
   plan->bytes = w.bytes;
   for (int i = 0; i < 3; ++i) plan->tile_hist[i] = w.hist[i];
-  const std::vector<GemmArgs<float>> &probs = w.probs;
+  const std::vector<GroupProb> &probs = w.gprobs;
   const std::vector<GroupItem> &sorted = w.items;
   plan->nitems = (int)sorted.size() + (int)w.solo.size();
   for (int c = 2; c < kGroupTrsmLists; ++c) plan->nitems += w.tcounts[c];
   for (int c = 0; c < 3; ++c) plan->ncls[c] = w.ncls[c];
   st = MUBLIS_OK;
-  if (!probs.empty() && hipMalloc(&plan->d_probs, sizeof(GemmArgs<float>) * probs.size()))
+  if (!probs.empty() && hipMalloc(&plan->d_probs, sizeof(GroupProb) * probs.size()))
     st = MUBLIS_ENOMEM;
   if (!st && !sorted.empty() && hipMalloc(&plan->d_items, sizeof(GroupItem) * sorted.size()))
     st = MUBLIS_ENOMEM;
   if (!st && !probs.empty() &&
-      hipMemcpy(plan->d_probs, probs.data(), sizeof(GemmArgs<float>) * probs.size(),
+      hipMemcpy(plan->d_probs, probs.data(), sizeof(GroupProb) * probs.size(),
                 hipMemcpyHostToDevice))
     st = MUBLIS_EHIP;
   if (!st && !sorted.empty() &&
@@ -545,7 +599,7 @@ int mublis_sgrouped_plan_create(const mublis_sbatch_op_t *ops, int count, int fl
 size_t mublis_sgrouped_ws_bytes(const mublis_sbatch_op_t *ops, int count) {
   Work w;
   if (!ops || count < 0 || build_work(ops, count, 0, w)) return 0;
-  return align256(sizeof(GemmArgs<float>) * w.probs.size()) +
+  return align256(sizeof(GroupProb) * w.gprobs.size()) +
          align256(sizeof(GroupItem) * w.items.size()) + trsm_bytes(w) + 256;
 }
 
@@ -555,7 +609,7 @@ int mublis_sgrouped_launch_ws(const mublis_sbatch_op_t *ops, int count, void *ho
   Work w;
   int st = build_work(ops, count, 0, w);
   if (st) return st;
-  const size_t pb = align256(sizeof(GemmArgs<float>) * w.probs.size());
+  const size_t pb = align256(sizeof(GroupProb) * w.gprobs.size());
   const size_t ib = align256(sizeof(GroupItem) * w.items.size());
   const size_t hb = trsm_host_bytes(w);
   if (pb + ib + trsm_bytes(w) > ws_bytes) return MUBLIS_EINVAL;
@@ -563,12 +617,12 @@ int mublis_sgrouped_launch_ws(const mublis_sbatch_op_t *ops, int count, void *ho
   // host_stage mirrors dev_ws: GEMM descriptors and items, then the TRSM
   // descriptors and items (the inverses and solo workspace stay on device)
   char *h = (char *)host_stage;
-  memcpy(h, w.probs.data(), sizeof(GemmArgs<float>) * w.probs.size());
+  memcpy(h, w.gprobs.data(), sizeof(GroupProb) * w.gprobs.size());
   memcpy(h + pb, w.items.data(), sizeof(GroupItem) * w.items.size());
   trsm_stage(w, h + pb + ib);
   hipStream_t s = (hipStream_t)(stream ? stream : mublis_rt_stream());
   if (hipMemcpyAsync(dev_ws, h, pb + ib + hb, hipMemcpyHostToDevice, s)) return MUBLIS_EHIP;
-  st = launch_items((const GemmArgs<float> *)dev_ws, (const GroupItem *)((char *)dev_ws + pb),
+  st = launch_items((const GroupProb *)dev_ws, (const GroupItem *)((char *)dev_ws + pb),
                     w.ncls, s);
   if (!st) st = launch_trsm(w, (char *)dev_ws + pb + ib, s);
   return st;

@@ -2,9 +2,10 @@
 //
 // Every FP32 operand element is split exactly as x = hi + mid + lo with hi,
 // mid and lo BF16 (8 significant bits each, round to nearest even; the
-// remainders x - hi and x - hi - mid are exact in FP32; Inf and NaN are
-// carried by hi alone, and values past the largest finite BF16 round hi
-// toward zero: bf16_split.h).  Then
+// remainders x - hi and x - hi - mid are exact in FP32; values past the
+// largest finite BF16 round hi toward zero: bf16_split.h).  Operands holding
+// an Inf or NaN take the native FP32 MFMA GEMM instead (the split kernels
+// flag them on the device).  Then
 //   A*B ~= Ah*Bh + (Ah*Bm + Am*Bh) + (Ah*Bl + Al*Bh + Am*Bm)
 // keeps every partial product down to 2^-16 relative; the three dropped
 // terms are below FP32 rounding.  Products of two BF16 values are exact in
@@ -47,11 +48,13 @@ using gfx950::bf16_split3;
 // Split an m x k FP32 view (strides rs, cs) into three K-contiguous BF16
 // planes of mp x kp.  A 64 x 64 tile goes through LDS so that reads run
 // along the source's unit stride and writes along k.
+// Any Inf or NaN in the view sets *nonfinite (plain stores of 1; the caller
+// clears it first).
 __global__ __launch_bounds__(256) void split3_kernel(int m, int k, const float *__restrict__ x,
                                                      int64_t rs, int64_t cs, int mp, int kp,
                                                      int ldp,
                                                      u16 *__restrict__ hi, u16 *__restrict__ mid,
-                                                     u16 *__restrict__ lo) {
+                                                     u16 *__restrict__ lo, int *nonfinite) {
   __shared__ float t[64][65];
   const int r0 = blockIdx.y * 64, k0 = blockIdx.x * 64;
   const int tid = threadIdx.x;
@@ -70,14 +73,18 @@ __global__ __launch_bounds__(256) void split3_kernel(int m, int k, const float *
   const int r = tid >> 2, c0 = (tid & 3) * 16;
   if (r0 + r >= mp) return;
   u16x8 vh[2], vm[2], vl[2];
+  bool bad = false;
 #pragma unroll
   for (int e = 0; e < 16; ++e) {
     u16 h, md, l;
-    bf16_split3(t[r][c0 + e], h, md, l);
+    const float v = t[r][c0 + e];
+    bad |= (__float_as_uint(v) & 0x7F800000u) == 0x7F800000u;
+    bf16_split3(v, h, md, l);
     vh[e / 8][e % 8] = h;
     vm[e / 8][e % 8] = md;
     vl[e / 8][e % 8] = l;
   }
+  if (bad) *nonfinite = 1;
   const int64_t off = (int64_t)(r0 + r) * ldp + k0 + c0;
 #pragma unroll
   for (int h = 0; h < 2; ++h) {
@@ -277,11 +284,11 @@ __global__ __launch_bounds__(64 * NW, 1) void gemm_bf16x3_kernel(XArgs p) {
 }
 
 int split3(int rows, int k, const float *x, int64_t rs, int64_t cs, int rp, int kp, int ldp,
-           u16 *planes, hipStream_t s) {
+           u16 *planes, int *nonfinite, hipStream_t s) {
   const size_t plane = (size_t)rp * ldp;
   dim3 grid((kp + 63) / 64, (rp + 63) / 64);
   hipLaunchKernelGGL(split3_kernel, grid, dim3(256), 0, s, rows, k, x, rs, cs, rp, kp, ldp,
-                     planes, planes + plane, planes + 2 * plane);
+                     planes, planes + plane, planes + 2 * plane, nonfinite);
   return hipGetLastError() == hipSuccess ? MUBLIS_OK : MUBLIS_EHIP;
 }
 
@@ -305,12 +312,33 @@ extern "C" int mublis_gfx950_sgemm_bf16x3_dev(int m, int n, int k, float alpha,
   // evict each other (at K = 16384 the L2 hit rate halved and HBM traffic
   // grew twentyfold).
   const int ldp = kp + 64;
-  u16 *pa = (u16 *)mublis_rt_alloc(sizeof(u16) * 3 * (size_t)mp * ldp);
+  const size_t a_bytes = sizeof(u16) * 3 * (size_t)mp * ldp;
+  u16 *pa = (u16 *)mublis_rt_alloc(a_bytes + 256);  // + the non-finite flag
   u16 *pb = (u16 *)mublis_rt_alloc(sizeof(u16) * 3 * (size_t)np * ldp);
+  int *flag = pa ? (int *)((char *)pa + a_bytes) : nullptr;
   int st = (pa && pb) ? MUBLIS_OK : MUBLIS_ENOMEM;
-  if (st == MUBLIS_OK) st = split3(m, k, a, rs_a, cs_a, mp, kp, ldp, pa, s);
+  if (st == MUBLIS_OK && hipMemsetAsync(flag, 0, sizeof(int), s) != hipSuccess)
+    st = MUBLIS_EHIP;
+  if (st == MUBLIS_OK) st = split3(m, k, a, rs_a, cs_a, mp, kp, ldp, pa, flag, s);
   // B is split as its transpose: row j of the plane holds B(:, j).
-  if (st == MUBLIS_OK) st = split3(n, k, b, cs_b, rs_b, np, kp, ldp, pb, s);
+  if (st == MUBLIS_OK) st = split3(n, k, b, cs_b, rs_b, np, kp, ldp, pb, flag, s);
+  // An Inf or NaN operand element runs the native FP32 MFMA GEMM instead: its
+  // products with the other operand's zero remainder parts would be Inf * 0 =
+  // NaN where FP32 gives Inf.  (This call synchronises before it returns
+  // anyway, to release the planes, so reading the flag costs one round trip.)
+  int nonfinite = 0;
+  if (st == MUBLIS_OK &&
+      (hipMemcpyAsync(&nonfinite, flag, sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess ||
+       hipStreamSynchronize(s) != hipSuccess))
+    st = MUBLIS_EHIP;
+  if (st == MUBLIS_OK && nonfinite) {
+    mublis_rt_free(pa);
+    mublis_rt_free(pb);
+    st = mublis_gfx950_sgemm_dev(m, n, k, alpha, a, rs_a, cs_a, b, rs_b, cs_b, nullptr, 0, 0,
+                                 nullptr, 0, 0, beta, c, rs_c, cs_c, 0, s);
+    if (hipStreamSynchronize(s) != hipSuccess && st == MUBLIS_OK) st = MUBLIS_EHIP;
+    return st;
+  }
   if (st == MUBLIS_OK) {
     XArgs p;
     p.m = m; p.n = n; p.kp = kp; p.ldp = ldp; p.mp = mp; p.np = np;

@@ -187,6 +187,21 @@ __host__ __device__ inline void map_tile(int w, int ntiles, int tiles_m, int til
   tn = in / gsize;
 }
 
+// Operands and C are in global memory.  A pointer that reaches a kernel
+// inside a descriptor loaded from memory (the grouped kernel's problem
+// list) is a generic pointer to the compiler, which then emits flat loads
+// and stores: those count against the LDS counter as well, so every wait
+// for an LDS access also waits for every load in flight.  Accesses go
+// through these casts instead, which make them global loads and stores.
+template <typename T>
+__device__ inline const __attribute__((address_space(1))) T *gmem(const T *p) {
+  return (const __attribute__((address_space(1))) T *)p;
+}
+template <typename T>
+__device__ inline __attribute__((address_space(1))) T *gmem(T *p) {
+  return (__attribute__((address_space(1))) T *)p;
+}
+
 // One operand slice (A: BM x BK, B: BK x BN) in registers.  `c` runs along the
 // operand's unit-stride dimension, `o` along the other; V elements per vector.
 template <typename T, int LC, int LO, int S, int NTH>
@@ -197,6 +212,12 @@ struct Slice {
   static_assert(LC % V == 0, "a vector stays within one row");
   T v[NV][V];
 
+  // EV (edge vectors): in a slice that is not whole, a vector whose row is
+  // in range and whose elements all are is still loaded whole, a vector
+  // whose row is out of range is zero, and only the vector that straddles
+  // the unit-stride edge goes element by element.  Without it the whole
+  // slice goes element by element.
+  template <bool EV = false>
   __device__ inline void load(const T *__restrict__ x, int64_t ld, int c0, int o0,
                               int c_lim, int o_lim, bool vec_ok, int tid) {
     typedef typename Mfma<T>::vec_t vec_t;
@@ -208,14 +229,30 @@ struct Slice {
       const int o = lin / (LC / V);
       const T *p = x + (int64_t)(o0 + o) * ld + (c0 + c);
       if (full) {
-        vec_t t = *reinterpret_cast<const vec_t *>(p);
+        // 16 bytes from an address aligned to the element only: gfx950's
+        // global loads need no more than dword alignment
+        vec_t t;
+        __builtin_memcpy(&t, gmem(p), sizeof t);
 #pragma unroll
         for (int e = 0; e < V; ++e) v[i][e] = t[e];
+      } else if constexpr (EV) {
+        if (o0 + o >= o_lim) {
+#pragma unroll
+          for (int e = 0; e < V; ++e) v[i][e] = T(0);
+        } else if (vec_ok && c0 + c + V <= c_lim) {
+          vec_t t;
+          __builtin_memcpy(&t, gmem(p), sizeof t);
+#pragma unroll
+          for (int e = 0; e < V; ++e) v[i][e] = t[e];
+        } else {
+#pragma unroll
+          for (int e = 0; e < V; ++e) v[i][e] = c0 + c + e < c_lim ? gmem(p)[e] : T(0);
+        }
       } else {
 #pragma unroll
         for (int e = 0; e < V; ++e) {
           T t = T(0);
-          if ((c0 + c + e < c_lim) && (o0 + o < o_lim)) t = p[e];
+          if ((c0 + c + e < c_lim) && (o0 + o < o_lim)) t = gmem(p)[e];
           v[i][e] = t;
         }
       }
@@ -256,16 +293,16 @@ struct Operands {
   typedef Slice<T, B_NC ? BN : BK, B_NC ? BK : BN, S, NTH> SB;
 };
 
-template <typename T, int TILE, bool A_KC, bool B_NC, int BK, int NTH>
+template <typename T, int TILE, bool A_KC, bool B_NC, int BK, int NTH, bool EV = false>
 __device__ inline void load_slices(typename Operands<T, TILE, A_KC, B_NC, BK, NTH>::SA &ra,
                                    typename Operands<T, TILE, A_KC, B_NC, BK, NTH>::SB &rb,
                                    const T *a, int64_t lda, const T *b,
                                    int64_t ldb, int m0, int n0, int k0, int m,
                                    int n, int k, bool va, bool vb, int tid) {
-  if (A_KC) ra.load(a, lda, k0, m0, k, m, va, tid);
-  else ra.load(a, lda, m0, k0, m, k, va, tid);
-  if (B_NC) rb.load(b, ldb, n0, k0, n, k, vb, tid);
-  else rb.load(b, ldb, k0, n0, k, n, vb, tid);
+  if (A_KC) ra.template load<EV>(a, lda, k0, m0, k, m, va, tid);
+  else ra.template load<EV>(a, lda, m0, k0, m, k, va, tid);
+  if (B_NC) rb.template load<EV>(b, ldb, n0, k0, n, k, vb, tid);
+  else rb.template load<EV>(b, ldb, k0, n0, k, n, vb, tid);
 }
 
 // LDS elements one workgroup needs for a TILE x TILE tile of type T staged
@@ -281,7 +318,20 @@ struct LdsSize {
 // BK, the K depth of one staged slice, sets only how K is cut into slices:
 // every depth runs the same MFMA steps in the same k order (a slice past k
 // adds exact zeros), so the result does not depend on it.
-template <typename T, int TILE, bool A_KC, bool B_NC, int BK = Mfma<T>::BK, int NTH = NT>
+//
+// SMALL is the grouped kernel's mode for small tiles, whose short K loops
+// leave the fixed costs of a tile a large share of its time; it changes only
+// how data moves, never the arithmetic of an element:
+//  * a column-major C (rs_c == 1) is written through LDS: the MFMA output
+//    layout gives each lane 4 rows of one column, so a wave's store touches
+//    16 columns, 16 bytes in each; staged through LDS each wave reads and
+//    writes runs of consecutive rows of one column instead;
+//  * slices of edge tiles are loaded by whole vectors where they can
+//    (Slice::load's EV);
+//  * the caller has synchronised before the tile, so the first segment
+//    stages its first slice without a barrier of its own.
+template <typename T, int TILE, bool A_KC, bool B_NC, int BK = Mfma<T>::BK, int NTH = NT,
+          bool SMALL = false>
 __device__ inline void gemm_tile(const GemmArgs<T> &p, int tm, int tn, T *lds) {
   typedef Mfma<T> MF;
   typedef typename MF::acc_t acc_t;
@@ -326,9 +376,11 @@ __device__ inline void gemm_tile(const GemmArgs<T> &p, int tm, int tn, T *lds) {
     const int ktiles = kend > kbeg ? (kend + BK - 1) / BK - kt0 : 0;
     if (ktiles <= 0) continue;
 
-    load_slices<T, TILE, A_KC, B_NC, BK, NTH>(ra, rb, a, lda, b, ldb, m0, n0, kt0 * BK, p.m, p.n,
+    load_slices<T, TILE, A_KC, B_NC, BK, NTH, SMALL>(ra, rb, a, lda, b, ldb, m0, n0, kt0 * BK, p.m, p.n,
                                      p.k, va, vb, tid);
-    __syncthreads();  // previous segment's readers are done with buffer 0
+    // previous segment's readers are done with buffer 0 (SMALL: the caller
+    // synchronised before the first)
+    if (!SMALL || seg > 0) __syncthreads();
     ra.template store<A_KC>(As[0], tid);
     rb.template store<!B_NC>(Bs[0], tid);
     __syncthreads();
@@ -338,7 +390,7 @@ __device__ inline void gemm_tile(const GemmArgs<T> &p, int tm, int tn, T *lds) {
       const int cur = kt & 1;
       const bool more = kt + 1 < ktiles;
       if (more)
-        load_slices<T, TILE, A_KC, B_NC, BK, NTH>(ra, rb, a, lda, b, ldb, m0, n0,
+        load_slices<T, TILE, A_KC, B_NC, BK, NTH, SMALL>(ra, rb, a, lda, b, ldb, m0, n0,
                                          (kt0 + kt + 1) * BK, p.m, p.n, p.k, va, vb, tid);
       const T *as = As[cur];
       const T *bs = Bs[cur];
@@ -382,6 +434,7 @@ __device__ inline void gemm_tile(const GemmArgs<T> &p, int tm, int tn, T *lds) {
     }
   }
 
+
   // Epilogue.  MFMA 16x16 output layout: lane l, register r holds column
   // l%16 and row MF::row(l/16, r) of its 16 x 16 tile.  C is read and written
   // through the same pointer, so the compiler must assume a store may alias
@@ -389,6 +442,55 @@ __device__ inline void gemm_tile(const GemmArgs<T> &p, int tm, int tn, T *lds) {
   // every element a load -> wait -> store round trip.  The C values of a row
   // of tiles are therefore all loaded before any of them is written (one
   // memory latency per row of tiles instead of one per element).
+  if constexpr (SMALL) {
+    if (p.rs_c == 1) {
+      // The K loop ended with a barrier (and without one the caller's
+      // barrier stands): LDS is free.  Column c of the tile is LDS row c
+      // (pitch BM + 4: a lane's 4 rows stay one aligned 16-byte store).
+      // Thread tid handles elements e = tid + q * NTH (row e % BM, column
+      // e / BM), so consecutive lanes take consecutive rows of one column.
+      constexpr int PT = BM + 4, NQ = BM * BN / NTH;
+      static_assert(BN * PT <= LdsSize<T, TILE, BK>::value, "staged tile fits the LDS");
+      static_assert(NQ * NTH == BM * BN, "whole element rounds");
+      T *ct = lds;
+#pragma unroll
+      for (int i = 0; i < TIM; ++i)
+#pragma unroll
+        for (int j = 0; j < TIN; ++j)
+#pragma unroll
+          for (int r = 0; r < 4; ++r)
+            ct[(wn + j * 16 + lr) * PT + wm + i * 16 + MF::row(lk, r)] = acc[i][j][r];
+      __syncthreads();
+      const T alpha = p.alpha, beta = p.beta;
+      auto ok = [&](int row, int col) {
+        return row < p.m && col < p.n && !(p.ji == 1 && col > row) && !(p.ji == 2 && col < row);
+      };
+      constexpr int CH = NQ < 8 ? NQ : 8;  // C values in flight per thread
+#pragma unroll
+      for (int q0 = 0; q0 < NQ; q0 += CH) {
+        T cv[CH];
+        if (beta != T(0)) {
+#pragma unroll
+          for (int q = 0; q < CH; ++q) {
+            const int e = tid + (q0 + q) * NTH, row = m0 + e % BM, col = n0 + e / BM;
+            cv[q] = ok(row, col) ? gmem(p.c)[(int64_t)row + (int64_t)col * p.cs_c] : T(0);
+          }
+        }
+#pragma unroll
+        for (int q = 0; q < CH; ++q) {
+          const int e = tid + (q0 + q) * NTH, rl = e % BM, cl = e / BM;
+          const int row = m0 + rl, col = n0 + cl;
+          if (!ok(row, col)) continue;
+          auto *cp = gmem(p.c) + (int64_t)row + (int64_t)col * p.cs_c;
+          const T prod = alpha * ct[cl * PT + rl];
+          if (beta == T(0)) *cp = prod;
+          else if (beta == T(1)) *cp = cv[q] + prod;
+          else *cp = prod + beta * cv[q];
+        }
+      }
+      return;
+    }
+  }
   const T alpha = p.alpha, beta = p.beta;
   auto c_ok = [&](int row, int col) {
     return row < p.m && col < p.n && !(p.ji == 1 && col > row) && !(p.ji == 2 && col < row);
@@ -403,7 +505,8 @@ __device__ inline void gemm_tile(const GemmArgs<T> &p, int tm, int tn, T *lds) {
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
           const int row = m0 + wm + i * 16 + MF::row(lk, r);
-          cv[j][r] = c_ok(row, col) ? p.c[(int64_t)row * p.rs_c + (int64_t)col * p.cs_c] : T(0);
+          cv[j][r] =
+              c_ok(row, col) ? gmem(p.c)[(int64_t)row * p.rs_c + (int64_t)col * p.cs_c] : T(0);
         }
       }
     }
@@ -414,7 +517,7 @@ __device__ inline void gemm_tile(const GemmArgs<T> &p, int tm, int tn, T *lds) {
       for (int r = 0; r < 4; ++r) {
         const int row = m0 + wm + i * 16 + MF::row(lk, r);
         if (!c_ok(row, col)) continue;
-        T *cp = p.c + (int64_t)row * p.rs_c + (int64_t)col * p.cs_c;
+        auto *cp = gmem(p.c) + (int64_t)row * p.rs_c + (int64_t)col * p.cs_c;
         const T prod = alpha * acc[i][j][r];
         if (beta == T(0)) *cp = prod;
         else if (beta == T(1)) *cp = cv[j][r] + prod;
