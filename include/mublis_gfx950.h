@@ -163,6 +163,20 @@ int mublis_gfx950_strsm_ws_dev(int lower, int unit, int m, int n, float alpha,
                                int64_t rs_b, int64_t cs_b, void *ws, size_t ws_bytes,
                                void *stream);
 
+/* Allocation-free DTRSM: the solve of mublis_gfx950_dtrsm_ws_dev given
+ * mublis_gfx950_trsm_ws_bytes(m) bytes (the same schedule and bits), with
+ * that workspace taken from a static device array of the library instead
+ * (8.5 MB per GPU, held until the process ends).  No counterpart in the
+ * reference: for callers that cannot allocate device memory -- code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc.
+ * Limits: m <= 16384, and A and B each need a unit stride (else
+ * MUBLIS_EINVAL with mublis_rt_last_error() set); stream NULL is the null
+ * stream.  Calls never allocate; calls that overlap on one GPU share the
+ * array and race, so callers serialise them per device. */
+int mublis_gfx950_dtrsm_noalloc_dev(int lower, int unit, int m, int n, double alpha,
+                                    const double *a, int64_t rs_a, int64_t cs_a,
+                                    double *b, int64_t rs_b, int64_t cs_b, void *stream);
+
 /* Strided copy of an m x n matrix (dst may have other strides). */
 int mublis_gfx950_dcopy_dev(int m, int n, const double *src, int64_t rs_s,
                             int64_t cs_s, double *dst, int64_t rs_d,

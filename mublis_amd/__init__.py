@@ -134,6 +134,9 @@ def _declare(lib):
         f = getattr(lib, name)
         f.argtypes = [_I, _I, _I, _I, ct, _VP, _I64, _I64, _VP, _I64, _I64, _VP, _SZ, _VP]
         f.restype = _I
+    f = lib.mublis_gfx950_dtrsm_noalloc_dev
+    f.argtypes = [_I, _I, _I, _I, _D, _VP, _I64, _I64, _VP, _I64, _I64, _VP]
+    f.restype = _I
     # serving loop
     for name, args in (("mublis_server_create", [_VP, _VP]),
                        ("mublis_server_destroy", [_VP]),

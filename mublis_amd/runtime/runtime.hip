@@ -116,6 +116,16 @@ const char *mublis_rt_arch(void) { return g_arch; }
 
 const char *mublis_rt_last_error(void) { return t_last_error; }
 
+}  // extern "C"
+
+// The calling thread's mublis_rt_last_error() text, for argument errors
+// found inside the HIP layer.
+void gfx950_set_error(const char *msg) {
+  snprintf(t_last_error, sizeof t_last_error, "%s", msg);
+}
+
+extern "C" {
+
 int mublis_rt_is_device_ptr(const void *p) {
   if (!p) return 0;
   hipPointerAttribute_t attr;

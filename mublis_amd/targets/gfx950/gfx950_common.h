@@ -9,6 +9,9 @@
 
 #include "mublis_gfx950.h"
 
+// Set the calling thread's mublis_rt_last_error() text.
+void gfx950_set_error(const char *msg);
+
 // dst(i, j) := src(i, j) for an m x n matrix; any strides.
 template <typename T>
 int gfx950_copy(int m, int n, const T *src, int64_t rs_s, int64_t cs_s, T *dst,

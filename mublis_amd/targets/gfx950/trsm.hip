@@ -25,6 +25,7 @@
 
 #include <algorithm>
 #include <atomic>
+#include <mutex>
 
 #include "gfx950_common.h"
 
@@ -1287,6 +1288,75 @@ extern "C" size_t mublis_gfx950_dtrsm_sched_ws_bytes(int m, int n) {
 }
 extern "C" size_t mublis_gfx950_strsm_sched_ws_bytes(int m, int n) {
   return m <= 0 || n <= 0 ? 0 : sched_ws_bytes<float>(m, n);
+}
+
+// ------------------------------------------------- allocation-free DTRSM --
+// mublis_gfx950_dtrsm_noalloc_dev takes the workspace of the workspace
+// entry's schedule (the 64-row diagonal-block inverses and the pipeline's
+// flags; no GEMM leaves) from a file-scope device array instead of an
+// allocation: kNoAllocMaxM / 64 blocks of BLK doubles (8.5 MB for orders up
+// to 16384), held by every GPU the library runs on until the process ends.
+// Each solve rewrites what it reads (the inverses, and the flags, which the
+// inversion kernel clears), so nothing carries over from the last call; two
+// calls that overlap on one GPU share the array and race.
+constexpr int kNoAllocMaxM = 16384;
+constexpr int64_t kScratchDoubles = (int64_t)(kNoAllocMaxM / NB) * BLK;
+static __device__ __align__(256) double scratch[kScratchDoubles];
+
+namespace {
+// The array's device address, one per device: in host code its name is a
+// host-side stand-in, not a device address.
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double *g_scratch_dev[kMaxDevices] = {};
+
+// Resolve (once per device, with that device current) the address of this
+// device's copy of `scratch`.
+double *scratch_address() {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) return nullptr;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_dev[dev]) {
+    void *ptr = nullptr;
+    if (hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)) != hipSuccess || !ptr) {
+      (void)hipGetLastError();
+      return nullptr;
+    }
+    g_scratch_dev[dev] = (double *)ptr;
+  }
+  return g_scratch_dev[dev];
+}
+}  // namespace
+
+extern "C" int mublis_gfx950_dtrsm_noalloc_dev(int lower, int unit, int m, int n,
+                                               double alpha, const double *a, int64_t rs_a,
+                                               int64_t cs_a, double *b, int64_t rs_b,
+                                               int64_t cs_b, void *stream) {
+  if (m < 0 || n < 0) return MUBLIS_EINVAL;
+  if (m > kNoAllocMaxM) {
+    gfx950_set_error("mublis_gfx950_dtrsm_noalloc_dev: order m above 16384, the size of the "
+                     "static workspace");
+    return MUBLIS_EINVAL;
+  }
+  // an operand without a unit stride would need a packed copy (an allocation)
+  if ((rs_a != 1 && cs_a != 1) || (rs_b != 1 && cs_b != 1)) {
+    gfx950_set_error("mublis_gfx950_dtrsm_noalloc_dev: A and B each need a unit stride");
+    return MUBLIS_EINVAL;
+  }
+  if (m == 0 || n == 0) return MUBLIS_OK;
+  int st = mublis_rt_init();
+  if (st) return st;
+  // NULL: the null stream (creating the thread's own stream would allocate)
+  hipStream_t s = (hipStream_t)stream;
+  if (alpha == 0.0) return gfx950_fill<double>(m, n, 0.0, b, rs_b, cs_b, s);
+  double *ws = scratch_address();
+  if (!ws) {
+    gfx950_set_error("mublis_gfx950_dtrsm_noalloc_dev: no device address for the static "
+                     "workspace");
+    return MUBLIS_EHIP;
+  }
+  return solve_ws<double>(lower != 0, unit ? 1 : 0, m, n, alpha, a, rs_a, cs_a, b, rs_b, cs_b,
+                          ws, nullptr, nullptr, nullptr, s);
 }
 
 extern "C" int mublis_gfx950_trsm_pipe_spin_limit(int spins) {
