@@ -291,6 +291,10 @@ typedef struct {
 typedef struct mublis_grouped_plan_s *mublis_grouped_plan_t;
 /* flags for plan creation: force one tile shape (0: 32, 1: 64, 2: 128) */
 #define MUBLIS_GROUPED_FORCE_TILE(t) (((t) + 1) << 4)
+/* flags for plan creation: force launch schedule variant v (0 or 1) of every
+ * tile class (workgroups per CU, threads, K-slice depth; results are the
+ * same under every schedule) */
+#define MUBLIS_GROUPED_FORCE_SCHED(v) (((v) + 1) << 8)
 
 /* Expand the problems into a cost-sorted tile work list (tile shapes from
  * the autotuner table, else a size heuristic), upload it and capture the
@@ -316,11 +320,15 @@ int mublis_sgrouped_launch_ws(const mublis_sbatch_op_t *ops, int count, void *ho
                               void *dev_ws, size_t ws_bytes, void *stream);
 /* One-shot: plan, execute, synchronise, destroy. */
 int mublis_sgemm_grouped(const mublis_sbatch_op_t *ops, int count);
-/* Autotuner: time each tile shape per (log2 m, log2 n, log2 k) bucket over
- * [lo, hi] and write/load the table at `path`.  The environment variable
- * MUBLIS_TUNE_TABLE names the table loaded at first use;
- * mublis_grouped_load_table switches tables (NULL: heuristic only) and
- * returns the number of buckets read. */
+/* Autotuner: choose the launch schedule of each tile class, then the tile
+ * shape of every GEMM bucket (floor log2 of m, n, k in [lo, hi], 4 <= lo <=
+ * hi <= 12) and every SYRK bucket (log2 n, log2 k), each candidate timed on
+ * a batch that fills the GPU (at least per_bucket problems), best of reps
+ * replays; near-ties take the size heuristic's choice.  Writes the table at
+ * `path` and loads it.  The environment variable MUBLIS_TUNE_TABLE names
+ * the table loaded at first use; mublis_grouped_load_table switches tables
+ * (NULL: heuristic only, schedule variant 0) and returns the number of
+ * buckets read. */
 int mublis_grouped_autotune(const char *path, int lo, int hi, int per_bucket, int reps);
 int mublis_grouped_load_table(const char *path);
 

@@ -72,6 +72,20 @@ __device__ inline GemmArgs<float> args_of(const GroupProb &g) {
 
 constexpr int kTiles[3] = {32, 64, 128};
 
+// Launch schedules of each tile class: workgroups per CU, threads per
+// workgroup, K-slice depth.  No schedule changes the MFMA order of an
+// element (gemm_tile: neither the slice depth nor the wave layout does), so
+// results do not depend on it; the autotuner picks one per class.
+struct Sched {
+  int wg_per_cu, nth, bk;
+};
+constexpr int kSchedVariants = 2;
+constexpr Sched kScheds[3][kSchedVariants] = {
+    {{6, 256, 32}, {12, 128, 16}},  // 32-tiles: 4 waves of 16 x 16 / 2 of 16 x 32
+    {{3, 512, 32}, {4, 256, 32}},   // 64-tiles: 8 waves of 32 x 16 / 4 of 32 x 32
+    {{2, 256, 32}, {2, 512, 32}},   // 128-tiles: 4 waves of 64 x 64 / 8 of 64 x 32
+};
+
 
 // SMALL (gemm_tile's mode for small tiles) for the 64- and 32-tile items.
 template <int TILE, int BK, int NTH>
@@ -114,11 +128,15 @@ __global__ __launch_bounds__(NTH, WG_PER_CU * NTH / 256) void grouped_kernel(
 }
 
 // ------------------------------------------------------------ tune table --
-// Bucket = (floor log2 m, floor log2 n, floor log2 k), each clamped to
-// [4, 13]; value = index into kTiles.
+// Tile shape per size bucket, separately for GEMM problems (bucket (floor
+// log2 m, floor log2 n, floor log2 k)) and SYRK / SYR2K problems (bucket
+// (floor log2 n, floor log2 k)), each log clamped to [4, 13]; value = index
+// into kTiles.  And a launch schedule per tile class (index into kScheds).
 std::mutex g_tab_mu;
 bool g_tab_loaded = false;
 std::map<std::tuple<int, int, int>, int> g_tab;
+std::map<std::pair<int, int>, int> g_stab;
+int g_sched[3] = {0, 0, 0};
 
 int lg(int v) {
   int l = 0;
@@ -134,16 +152,27 @@ int heuristic_tile(int m, int n, int k) {
   return 0;
 }
 
+// Table lines: "gemm lm ln lk tile", "syrk ln lk tile", "sched class
+// variant" (a trailing "~" marks a near-tie; '#' starts a comment).  The
+// first table format, "lm ln lk tile", reads as gemm lines.
 void load_table_locked(const char *path) {
   g_tab.clear();
+  g_stab.clear();
+  for (int c = 0; c < 3; ++c) g_sched[c] = 0;
   FILE *f = fopen(path, "r");
   if (!f) return;
   char line[256];
   while (fgets(line, sizeof line, f)) {
     int a, b, c, t;
     if (line[0] == '#') continue;
-    if (sscanf(line, "%d %d %d %d", &a, &b, &c, &t) == 4 && t >= 0 && t < 3)
-      g_tab[std::make_tuple(a, b, c)] = t;
+    if (sscanf(line, "gemm %d %d %d %d", &a, &b, &c, &t) == 4 ||
+        sscanf(line, "%d %d %d %d", &a, &b, &c, &t) == 4) {
+      if (t >= 0 && t < 3) g_tab[std::make_tuple(a, b, c)] = t;
+    } else if (sscanf(line, "syrk %d %d %d", &a, &b, &t) == 3) {
+      if (t >= 0 && t < 3) g_stab[std::make_pair(a, b)] = t;
+    } else if (sscanf(line, "sched %d %d", &c, &t) == 2) {
+      if (c >= 0 && c < 3 && t >= 0 && t < kSchedVariants) g_sched[c] = t;
+    }
   }
   fclose(f);
 }
@@ -163,15 +192,29 @@ std::string default_table_path() {
   return "";
 }
 
-int choose_tile(int m, int n, int k) {
-  std::lock_guard<std::mutex> lk(g_tab_mu);
+void ensure_table_locked() {
   if (!g_tab_loaded) {
     const std::string p = default_table_path();
     if (!p.empty()) load_table_locked(p.c_str());
     g_tab_loaded = true;
   }
+}
+
+int choose_tile(int m, int n, int k, bool sym) {
+  std::lock_guard<std::mutex> lk(g_tab_mu);
+  ensure_table_locked();
+  if (sym) {
+    auto it = g_stab.find(std::make_pair(lg(n), lg(k)));
+    return it != g_stab.end() ? it->second : heuristic_tile(n, n, k);
+  }
   auto it = g_tab.find(std::make_tuple(lg(m), lg(n), lg(k)));
   return it != g_tab.end() ? it->second : heuristic_tile(m, n, k);
+}
+
+void current_scheds(int out[3]) {
+  std::lock_guard<std::mutex> lk(g_tab_mu);
+  ensure_table_locked();
+  for (int c = 0; c < 3; ++c) out[c] = g_sched[c];
 }
 
 // Staged slices load 16-byte vectors wherever a whole vector is in range;
@@ -201,10 +244,15 @@ extern "C" {
 
 int mublis_grouped_load_table(const char *path) {
   std::lock_guard<std::mutex> lk(g_tab_mu);
-  if (path) load_table_locked(path);
-  else g_tab.clear();
+  if (path) {
+    load_table_locked(path);
+  } else {
+    g_tab.clear();
+    g_stab.clear();
+    for (int c = 0; c < 3; ++c) g_sched[c] = 0;
+  }
   g_tab_loaded = true;
-  return (int)g_tab.size();
+  return (int)(g_tab.size() + g_stab.size());
 }
 
 }  // extern "C"
@@ -224,6 +272,7 @@ struct SoloTrsm {
 };
 
 struct Work {
+  int sched[3] = {0, 0, 0};  // launch schedule of each tile class
   std::vector<GemmArgs<float>> probs;
   std::vector<GroupProb> gprobs;  // the device form of probs
   std::vector<GroupItem> items;  // by tile shape (128, 64, 32), each by cost
@@ -297,6 +346,10 @@ int build_work(const mublis_sbatch_op_t *ops, int count, int flags, Work &w) {
   std::vector<std::vector<TrsmGroupItem>> tlists(kGroupTrsmLists);
   w.probs.reserve(count);
   const int forced = (flags >> 4) & 3;  // MUBLIS_GROUPED_FORCE_TILE(t) stores t + 1
+  const int forced_s = (flags >> 8) & 3;  // MUBLIS_GROUPED_FORCE_SCHED(v) stores v + 1
+  current_scheds(w.sched);
+  if (forced_s)
+    for (int c = 0; c < 3; ++c) w.sched[c] = std::min(forced_s - 1, kSchedVariants - 1);
   for (int i = 0; i < count; ++i) {
     const mublis_sbatch_op_t &o = ops[i];
     if (o.op == MUBLIS_BATCH_TRSM) {
@@ -362,7 +415,7 @@ int build_work(const mublis_sbatch_op_t *ops, int count, int flags, Work &w) {
       p.vec_b2 = vec_ok_f(p.b2, bnc ? p.rs_b2 : p.cs_b2);
     }
     if (p.k == 0) p.a2 = p.b2 = nullptr;
-    const int tix = forced ? forced - 1 : choose_tile(m, n, k);
+    const int tix = forced ? forced - 1 : choose_tile(m, n, k, sym);
     const int T = kTiles[tix];
     w.hist[tix]++;
     const int layout = (akc ? 1 : 0) | (bnc ? 2 : 0);
@@ -507,29 +560,169 @@ int persistent_grid(int nitems, int per_cu) {
   return (g + 7) / 8 * 8;
 }
 
+template <int TILE, int W, int BK, int NTH>
+void launch_class(const GroupProb *probs, const GroupItem *items, int n, hipStream_t s) {
+  hipLaunchKernelGGL((grouped_kernel<TILE, W, BK, NTH>), dim3(persistent_grid(n, W)),
+                     dim3(NTH), 0, s, probs, items, n);
+}
+
 // The launches of a work list, back to back on one stream (any may be
-// empty): the 128-tile items, the 64-tile items, the 32-tile items.
-// Measured on the batched workload, back-to-back launches beat parallel
-// graph branches (1.86 vs 2.69 ms for two classes) and one launch for all
-// items at 2 workgroups per CU (2.27 ms).
-constexpr int W128 = 2, BK128 = 32, W64 = 3, BK64 = 32, W32 = 6, BK32 = 32;
-constexpr int NT64 = 512;  // threads of a 64-tile workgroup
+// empty): the 128-tile items, the 64-tile items, the 32-tile items, each
+// class with its schedule (kScheds).  Measured on the batched workload,
+// back-to-back launches beat parallel graph branches (1.86 vs 2.69 ms for
+// two classes) and one launch for all items at 2 workgroups per CU (2.27 ms).
 int launch_items(const GroupProb *probs, const GroupItem *items, const int *ncls,
-                 hipStream_t s) {
+                 const int *sched, hipStream_t s) {
   const int n128 = ncls[2], n64 = ncls[1], n32 = ncls[0];
-  if (n128 > 0)
-    hipLaunchKernelGGL((grouped_kernel<128, W128, BK128, NT>),
-                       dim3(persistent_grid(n128, W128)), dim3(NT), 0, s, probs, items, n128);
-  if (n64 > 0)
-    hipLaunchKernelGGL((grouped_kernel<64, W64, BK64, NT64>),
-                       dim3(persistent_grid(n64, W64)), dim3(NT64), 0, s, probs, items + n128,
-                       n64);
-  if (n32 > 0)
-    hipLaunchKernelGGL((grouped_kernel<32, W32, BK32, NT>),
-                       dim3(persistent_grid(n32, W32)), dim3(NT), 0, s, probs,
-                       items + n128 + n64, n32);
+#define MUBLIS_CLASS(T, C, V, P, N)                                                   \
+  launch_class<T, kScheds[C][V].wg_per_cu, kScheds[C][V].bk, kScheds[C][V].nth>(probs, P, N, s)
+  if (n128 > 0) {
+    if (sched[2] == 1) MUBLIS_CLASS(128, 2, 1, items, n128);
+    else MUBLIS_CLASS(128, 2, 0, items, n128);
+  }
+  if (n64 > 0) {
+    if (sched[1] == 1) MUBLIS_CLASS(64, 1, 1, items + n128, n64);
+    else MUBLIS_CLASS(64, 1, 0, items + n128, n64);
+  }
+  if (n32 > 0) {
+    if (sched[0] == 1) MUBLIS_CLASS(32, 0, 1, items + n128 + n64, n32);
+    else MUBLIS_CLASS(32, 0, 0, items + n128 + n64, n32);
+  }
+#undef MUBLIS_CLASS
   return hipGetLastError() == hipSuccess ? MUBLIS_OK : MUBLIS_EHIP;
 }
+
+// ------------------------------------------------------------ autotuner --
+// Every candidate (tile shape of a bucket, schedule of a class) is timed on
+// a batch that fills the GPU -- at least kTuneTiles tiles at the 32-tile
+// shape -- after a warm-up replay, as the best of `reps` replays: throughput
+// inside a large batch, which is what the grouped workloads run, not the
+// latency of a few problems.  A choice within kTieMargin of the heuristic's
+// (for schedules, variant 0) takes the heuristic's, so repeated tuning
+// settles on the same table wherever the timings cannot tell the choices
+// apart.
+constexpr int kTuneTiles = 6144;
+constexpr double kTieMargin = 0.03;
+
+struct Tuner {
+  float *buf = nullptr;
+  size_t cap = 0;  // floats
+  hipStream_t s = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int reps = 1;
+  uint64_t seed = 314159;
+
+  ~Tuner() {
+    if (buf) (void)hipFree(buf);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+  }
+  int init(int r) {
+    reps = r;
+    s = (hipStream_t)mublis_rt_stream();
+    if (!s || hipEventCreate(&e0) || hipEventCreate(&e1)) return MUBLIS_EHIP;
+    return MUBLIS_OK;
+  }
+  int uniform(int lo, int hi) {  // in [lo, hi)
+    seed = seed * 6364136223846793005ULL + 1442695040888963407ULL;
+    return lo + (int)((seed >> 33) % (uint64_t)(hi - lo));
+  }
+  // device memory for `floats` operand elements (zero-filled; values do not
+  // change the timing of the MFMA path)
+  int reserve(size_t floats) {
+    if (floats <= cap) return MUBLIS_OK;
+    if (buf) (void)hipFree(buf);
+    buf = nullptr;
+    cap = 0;
+    if (hipMalloc(&buf, floats * sizeof(float)) != hipSuccess) return MUBLIS_ENOMEM;
+    if (hipMemset(buf, 0, floats * sizeof(float)) != hipSuccess) return MUBLIS_EHIP;
+    cap = floats;
+    return MUBLIS_OK;
+  }
+  // point the ops (sizes set) at disjoint column-major operands in buf
+  int place(std::vector<mublis_sbatch_op_t> &ops) {
+    size_t total = 0;
+    for (const auto &o : ops)
+      total += (size_t)o.m * o.k + (o.op == MUBLIS_BATCH_GEMM ? (size_t)o.k * o.n : 0) +
+               (size_t)o.m * o.n + 64;
+    int st = reserve(total);
+    if (st) return st;
+    size_t off = 0;
+    for (auto &o : ops) {
+      o.a = buf + off;
+      off += (size_t)o.m * o.k + 16;
+      if (o.op == MUBLIS_BATCH_GEMM) {
+        o.b = buf + off;
+        off += (size_t)o.k * o.n + 16;
+      }
+      o.c = buf + off;
+      off += (size_t)o.m * o.n + 32;
+    }
+    return MUBLIS_OK;
+  }
+  static mublis_sbatch_op_t op_of(bool sym, int m, int n, int k) {
+    mublis_sbatch_op_t o;
+    memset(&o, 0, sizeof o);
+    o.op = sym ? MUBLIS_BATCH_SYRK : MUBLIS_BATCH_GEMM;
+    o.uplo = MUBLIS_LOWER;
+    o.m = sym ? n : m;
+    o.n = n;
+    o.k = k;
+    o.alpha = 1.0f;
+    o.beta = 1.0f;
+    o.lda = o.m;
+    o.ldb = o.k;
+    o.ldc = o.m;
+    return o;
+  }
+  static int tiles32(const mublis_sbatch_op_t &o) {
+    const int tm = (o.m + 31) / 32, tn = (o.n + 31) / 32;
+    return o.op == MUBLIS_BATCH_GEMM ? tm * tn : tm * (tm + 1) / 2;
+  }
+  // problems of one bucket: each size uniform in [2^l, 2^(l+1))
+  int batch_bucket(std::vector<mublis_sbatch_op_t> &ops, bool sym, int lm, int ln, int lk,
+                   int min_count) {
+    int tiles = 0;
+    while ((int)ops.size() < min_count || tiles < kTuneTiles) {
+      const int n = uniform(1 << ln, 2 << ln), k = uniform(1 << lk, 2 << lk);
+      const int m = sym ? n : uniform(1 << lm, 2 << lm);
+      ops.push_back(op_of(sym, m, n, k));
+      tiles += tiles32(ops.back());
+    }
+    return place(ops);
+  }
+  // GEMM problems with m, n in [lo, hi) and k in [16, 512)
+  int batch_range(std::vector<mublis_sbatch_op_t> &ops, int lo, int hi, int cls,
+                  int min_count) {
+    int tiles = 0;
+    const int need = kTuneTiles << (2 * cls);  // as many tiles of the class's own shape
+    while ((int)ops.size() < min_count || tiles < need) {
+      ops.push_back(op_of(false, uniform(lo, hi), uniform(lo, hi), uniform(16, 512)));
+      tiles += tiles32(ops.back());
+    }
+    return place(ops);
+  }
+  // best of `reps` replays of a plan of ops created with flags
+  int time(const std::vector<mublis_sbatch_op_t> &ops, int flags, double *best) {
+    mublis_grouped_plan_t plan = nullptr;
+    int st = mublis_sgrouped_plan_create(ops.data(), (int)ops.size(), flags, &plan);
+    if (st) return st;
+    st = mublis_grouped_plan_execute(plan, s);  // warm-up
+    *best = 1e30;
+    for (int r = 0; r < reps && !st; ++r) {
+      float ms = 0;
+      if (hipEventRecord(e0, s) != hipSuccess) st = MUBLIS_EHIP;
+      if (!st) st = mublis_grouped_plan_execute(plan, s);
+      if (!st && (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                  hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+        st = MUBLIS_EHIP;
+      if (!st) *best = std::min(*best, (double)ms);
+    }
+    if (hipStreamSynchronize(s) != hipSuccess && !st) st = MUBLIS_EHIP;
+    mublis_grouped_plan_destroy(plan);
+    return st;
+  }
+};
 
 }  // namespace
 
@@ -580,7 +773,7 @@ int mublis_sgrouped_plan_create(const mublis_sbatch_op_t *ops, int count, int fl
     st = MUBLIS_EHIP;
   if (!st && hipStreamBeginCapture(plan->cap, hipStreamCaptureModeThreadLocal)) st = MUBLIS_EHIP;
   if (!st) {
-    int l = launch_items(plan->d_probs, plan->d_items, plan->ncls, plan->cap);
+    int l = launch_items(plan->d_probs, plan->d_items, plan->ncls, w.sched, plan->cap);
     if (!l) l = launch_trsm(w, (char *)plan->d_trsm, plan->cap, plan->side, plan->fork,
                             plan->join);
     hipError_t e2 = hipStreamEndCapture(plan->cap, &plan->graph);
@@ -623,7 +816,7 @@ int mublis_sgrouped_launch_ws(const mublis_sbatch_op_t *ops, int count, void *ho
   hipStream_t s = (hipStream_t)(stream ? stream : mublis_rt_stream());
   if (hipMemcpyAsync(dev_ws, h, pb + ib + hb, hipMemcpyHostToDevice, s)) return MUBLIS_EHIP;
   st = launch_items((const GroupProb *)dev_ws, (const GroupItem *)((char *)dev_ws + pb),
-                    w.ncls, s);
+                    w.ncls, w.sched, s);
   if (!st) st = launch_trsm(w, (char *)dev_ws + pb + ib, s);
   return st;
 }
@@ -683,78 +876,64 @@ int mublis_sgemm_grouped(const mublis_sbatch_op_t *ops, int count) {
 int mublis_grouped_autotune(const char *path, int lo, int hi, int per_bucket, int reps) {
   int st = mublis_rt_init();
   if (st) return st;
-  if (!path || lo < 4 || hi > 13 || lo > hi || per_bucket < 1 || reps < 1) return MUBLIS_EINVAL;
-  const int maxd = 2 << hi;  // exclusive bound of the largest bucket
-  const size_t per = (size_t)maxd * maxd;
-  float *buf = nullptr;
-  const size_t total = per * 3 * per_bucket;
-  if (hipMalloc(&buf, total * sizeof(float))) return MUBLIS_ENOMEM;
-  if (hipMemset(buf, 0, total * sizeof(float))) {
-    (void)hipFree(buf);
-    return MUBLIS_EHIP;
-  }
-  hipStream_t s = (hipStream_t)mublis_rt_stream();
-  hipEvent_t e0, e1;
-  if (hipEventCreate(&e0) || hipEventCreate(&e1)) {
-    (void)hipFree(buf);
-    return MUBLIS_EHIP;
-  }
+  if (!path || lo < 4 || hi > 12 || lo > hi || per_bucket < 1 || reps < 1) return MUBLIS_EINVAL;
+  Tuner t;
+  if ((st = t.init(reps))) return st;
   FILE *f = fopen(path, "w");
-  if (!f) {
-    (void)hipFree(buf);
-    return MUBLIS_EINVAL;
+  if (!f) return MUBLIS_EINVAL;
+  fprintf(f, "# gfx950 grouped-kernel table (tiles 32 64 128)\n");
+  fprintf(f, "# sched class variant | gemm lm ln lk tile | syrk ln lk tile; buckets are "
+             "floor(log2) of the sizes\n");
+  fprintf(f, "# each candidate: a batch of >= %d tiles (>= %d problems), best of %d replays; "
+             "a choice within %.0f%% of the\n# heuristic's (schedule variant 0) takes the "
+             "heuristic's and is marked ~\n",
+          kTuneTiles, per_bucket, reps, 100 * kTieMargin);
+  // 1. the schedule of each tile class, on a batch of problems of that
+  //    class's typical sizes with the tile shape forced
+  static const int kSchedRange[3][2] = {{16, 128}, {64, 512}, {256, 1024}};
+  int sched[3] = {0, 0, 0};
+  for (int c = 0; c < 3 && !st; ++c) {
+    std::vector<mublis_sbatch_op_t> ops;
+    st = t.batch_range(ops, kSchedRange[c][0], kSchedRange[c][1], c, per_bucket);
+    double ms[kSchedVariants];
+    for (int v = 0; v < kSchedVariants && !st; ++v)
+      st = t.time(ops, MUBLIS_GROUPED_FORCE_TILE(c) | MUBLIS_GROUPED_FORCE_SCHED(v), &ms[v]);
+    if (st) break;
+    int best = 0;
+    for (int v = 1; v < kSchedVariants; ++v)
+      if (ms[v] < ms[best]) best = v;
+    const bool tie = best != 0 && ms[0] <= ms[best] * (1 + kTieMargin);
+    sched[c] = tie ? 0 : best;
+    fprintf(f, "sched %d %d%s\n", c, sched[c], tie ? " ~" : "");
   }
-  fprintf(f, "# gfx950 grouped-kernel tile table: lm ln lk tile_index (tiles 32 64 128)\n");
-  fprintf(f, "# bucket = floor(log2) of m, n, k; %d problems per bucket, %d reps\n",
-          per_bucket, reps);
-  uint64_t seed = 314159;
-  auto rnd = [&](int l) {
-    seed = seed * 6364136223846793005ULL + 1442695040888963407ULL;
-    const int span = 1 << l;
-    return span + (int)((seed >> 33) % (uint64_t)span);
-  };
-  for (int lm = lo; lm <= hi && !st; ++lm)
-    for (int ln = lo; ln <= hi && !st; ++ln)
-      for (int lk = lo; lk <= hi && !st; ++lk) {
-        std::vector<mublis_sbatch_op_t> ops(per_bucket);
-        for (int i = 0; i < per_bucket; ++i) {
-          mublis_sbatch_op_t &o = ops[i];
-          memset(&o, 0, sizeof o);
-          o.op = MUBLIS_BATCH_GEMM;
-          o.m = std::min(rnd(lm), maxd - 1);
-          o.n = std::min(rnd(ln), maxd - 1);
-          o.k = std::min(rnd(lk), maxd - 1);
-          o.alpha = 1.0f; o.beta = 1.0f;
-          float *base = buf + per * 3 * i;
-          o.a = base; o.lda = o.m;
-          o.b = base + per; o.ldb = o.k;
-          o.c = base + 2 * per; o.ldc = o.m;
-        }
-        float best = 1e30f;
-        int best_t = heuristic_tile(1 << lm, 1 << ln, 1 << lk);
-        for (int t = 0; t < 3 && !st; ++t) {
-          mublis_grouped_plan_t plan = nullptr;
-          st = mublis_sgrouped_plan_create(ops.data(), per_bucket, (t + 1) << 4, &plan);
+  // 2. the tile shape of each GEMM bucket, then of each SYRK bucket, with
+  //    those schedules
+  {
+    std::lock_guard<std::mutex> lk(g_tab_mu);
+    for (int c = 0; c < 3; ++c) g_sched[c] = sched[c];
+    g_tab_loaded = true;
+  }
+  for (int sym = 0; sym < 2 && !st; ++sym)
+    for (int lm = sym ? 0 : lo; lm <= (sym ? 0 : hi) && !st; ++lm)
+      for (int ln = lo; ln <= hi && !st; ++ln)
+        for (int lk = lo; lk <= hi && !st; ++lk) {
+          std::vector<mublis_sbatch_op_t> ops;
+          st = t.batch_bucket(ops, sym != 0, lm, ln, lk, per_bucket);
+          double ms[3];
+          for (int tile = 0; tile < 3 && !st; ++tile)
+            st = t.time(ops, MUBLIS_GROUPED_FORCE_TILE(tile), &ms[tile]);
           if (st) break;
-          st = mublis_grouped_plan_execute(plan, s);  // warm-up
-          if (!st && hipEventRecord(e0, s)) st = MUBLIS_EHIP;
-          for (int r = 0; r < reps && !st; ++r) st = mublis_grouped_plan_execute(plan, s);
-          float ms = 0;
-          if (!st && (hipEventRecord(e1, s) || hipEventSynchronize(e1) ||
-                      hipEventElapsedTime(&ms, e0, e1)))
-            st = MUBLIS_EHIP;
-          mublis_grouped_plan_destroy(plan);
-          if (!st && ms < best) {
-            best = ms;
-            best_t = t;
-          }
+          const int h = sym ? heuristic_tile(1 << ln, 1 << ln, 1 << lk)
+                            : heuristic_tile(1 << lm, 1 << ln, 1 << lk);
+          int best = 0;
+          for (int tile = 1; tile < 3; ++tile)
+            if (ms[tile] < ms[best]) best = tile;
+          const bool tie = best != h && ms[h] <= ms[best] * (1 + kTieMargin);
+          const int pick = tie ? h : best;
+          if (sym) fprintf(f, "syrk %d %d %d%s\n", ln, lk, pick, tie ? " ~" : "");
+          else fprintf(f, "gemm %d %d %d %d%s\n", lm, ln, lk, pick, tie ? " ~" : "");
         }
-        if (!st) fprintf(f, "%d %d %d %d\n", lm, ln, lk, best_t);
-      }
   fclose(f);
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
-  (void)hipFree(buf);
   if (!st) mublis_grouped_load_table(path);
   return st;
 }

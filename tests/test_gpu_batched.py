@@ -131,25 +131,28 @@ def test_grouped_syr2k_equals_single_calls(lib):
 
 
 def test_forced_tiles_and_replay(lib):
+    """Every tile shape under every launch schedule gives the bits of the
+    single calls, and a second replay accumulates again (beta = 1)."""
     torch = pytest.importorskip("torch")
     ops, keep, specs = grouped_problems(torch, seed=8, n_gemm=120, n_syrk=40,
                                         syrk_n=(32, 300), vary_flags=True)
     initial = [s["C"].clone() for s in specs]
     want = single_calls(lib, torch, specs, initial, 1.0, 1.0, times=2)
     for t in range(3):
-        for s, c0 in zip(specs, initial):
-            s["C"].copy_(c0)
-        plan = M.GroupedPlan(ops, flags=(t + 1) << 4)
-        try:
-            assert plan.info()["tiles"][t] == len(ops)
-            plan.execute()
-            torch.cuda.synchronize()
-            plan.execute()  # beta = 1: the second replay accumulates again
-            torch.cuda.synchronize()
-        finally:
-            plan.close()
-        bad = [i for i, (s, w) in enumerate(zip(specs, want)) if not torch.equal(s["C"], w)]
-        assert not bad, f"tile {t}: {len(bad)} problems differ"
+        for v in range(2):
+            for s, c0 in zip(specs, initial):
+                s["C"].copy_(c0)
+            plan = M.GroupedPlan(ops, flags=((t + 1) << 4) | ((v + 1) << 8))
+            try:
+                assert plan.info()["tiles"][t] == len(ops)
+                plan.execute()
+                torch.cuda.synchronize()
+                plan.execute()  # beta = 1: the second replay accumulates again
+                torch.cuda.synchronize()
+            finally:
+                plan.close()
+            bad = [i for i, (s, w) in enumerate(zip(specs, want)) if not torch.equal(s["C"], w)]
+            assert not bad, f"tile {t} schedule {v}: {len(bad)} problems differ"
 
 
 def test_grouped_sample_vs_oracle(lib):
@@ -180,26 +183,62 @@ def test_grouped_sample_vs_oracle(lib):
         assert np.linalg.norm(got - want) <= 2 * s["k"] * eps * scale
 
 
+def _table(path):
+    """{key: (choice, near_tie)} of a tuning table."""
+    out = {}
+    for ln in open(path):
+        if ln.startswith("#") or not ln.strip():
+            continue
+        f = ln.split()
+        tie = f[-1] == "~"
+        if tie:
+            f = f[:-1]
+        out[tuple(f[:-1])] = (int(f[-1]), tie)
+    return out
+
+
 def test_autotuner_writes_and_uses_table(lib, tmp_path):
+    """The table holds a schedule per tile class, a tile per GEMM bucket and a
+    tile per SYRK bucket; plans follow it (GEMM and SYRK problems alike)."""
     torch = pytest.importorskip("torch")
     path = str(tmp_path / "tune.txt").encode()
-    assert lib.mublis_grouped_autotune(path, 4, 5, 4, 1) == 0
-    lines = [ln for ln in open(path.decode()) if not ln.startswith("#")]
-    assert len(lines) == 8
-    assert lib.mublis_grouped_load_table(path) == 8
-    ops, keep, specs = grouped_problems(torch, seed=10, n_gemm=50, n_syrk=0,
-                                        gemm_range=(16, 63))
+    assert lib.mublis_grouped_autotune(path, 4, 5, 4, 2) == 0
+    tab = _table(path.decode())
+    gemm = {k[1:]: v[0] for k, v in tab.items() if k[0] == "gemm"}
+    syrk = {k[1:]: v[0] for k, v in tab.items() if k[0] == "syrk"}
+    sched = {k[1:]: v[0] for k, v in tab.items() if k[0] == "sched"}
+    assert len(gemm) == 8 and len(syrk) == 4 and len(sched) == 3
+    assert lib.mublis_grouped_load_table(path) == 12
+    ops, keep, specs = grouped_problems(torch, seed=10, n_gemm=50, n_syrk=20,
+                                        gemm_range=(16, 63), syrk_n=(16, 63),
+                                        syrk_k=(16, 63))
     plan = M.GroupedPlan(ops)
     try:
         hist = plan.info()["tiles"]
     finally:
         plan.close()
-    table = {tuple(map(int, ln.split()[:3])): int(ln.split()[3]) for ln in lines}
-    lg = lambda v: min(max(int(np.floor(np.log2(v))), 4), 13)  # noqa: E731
+    lg = lambda v: str(min(max(int(np.floor(np.log2(v))), 4), 13))  # noqa: E731
     expect = [0, 0, 0]
     for s in specs:
-        expect[table[(lg(s["m"]), lg(s["n"]), lg(s["k"]))]] += 1
+        if s["op"] == "gemm":
+            expect[gemm[(lg(s["m"]), lg(s["n"]), lg(s["k"]))]] += 1
+        else:
+            expect[syrk[(lg(s["n"]), lg(s["k"]))]] += 1
     assert hist == expect
+    lib.mublis_grouped_load_table(None)
+
+
+def test_autotuner_is_repeatable(lib, tmp_path):
+    """Two tuning runs over the same buckets choose the same schedule and tile
+    for every entry that neither run marks as a near-tie (within the
+    tuner's 3 % margin of the heuristic's choice, which a near-tie takes)."""
+    paths = [str(tmp_path / f"t{i}.txt").encode() for i in range(2)]
+    for p in paths:
+        assert lib.mublis_grouped_autotune(p, 5, 6, 4, 3) == 0
+    a, b = (_table(p.decode()) for p in paths)
+    assert a.keys() == b.keys()
+    differ = [k for k in a if a[k][0] != b[k][0] and not (a[k][1] or b[k][1])]
+    assert not differ, [(k, a[k], b[k]) for k in differ]
     lib.mublis_grouped_load_table(None)
 
 
