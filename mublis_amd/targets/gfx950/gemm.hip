@@ -46,14 +46,15 @@ using namespace gfx950;
 // 1536 52.0 -> 56.3; DSYRK 3072 47.0 -> 52.7; SSYRK 3072 83.4 -> 93.5.  But
 // launches with a triangular operand (long-first walk, the TRSM leaves) ran
 // 13 % slower, and grids that the deep kernel already holds in one round
-// (FP64 64-tiles: fewer than four per CU; FP32: four per CU or two for
-// 128-tiles; the few long SYRK diagonal tiles, the smaller updates of a
-// TRSM recursion) gain no residency and lose 1-2 %: those keep the deep
-// slice.  The slice depth never changes the k order of the MFMAs.
+// (64-tiles: at most four per CU; FP32 128-tiles: two per CU; the few long
+// SYRK diagonal tiles, the smaller updates of a TRSM recursion) gain no
+// residency and lose 1-5 % (FP64 1024 x 4096 x 1024, exactly four per CU:
+// 161.7 us shallow, 153.5 us deep): those keep the deep slice.  The slice
+// depth never changes the k order of the MFMAs.
 template <typename T, int TILE>
 bool use_shallow(const GemmArgs<T> &p, int64_t wgs) {
   if (p.lpt) return false;
-  if (TILE == 64) return p.tri != 3 && wgs >= (sizeof(T) == 8 ? 1024 : 1025);
+  if (TILE == 64) return p.tri != 3 && wgs > 1024;
   return sizeof(T) == 4 && wgs > 512;
 }
 //
@@ -174,7 +175,8 @@ int gemm_dev(int m, int n, int k, T alpha, const T *a, int64_t rs_a, int64_t cs_
              const T *b, int64_t rs_b, int64_t cs_b, const T *a2, int64_t rs_a2,
              int64_t cs_a2, const T *b2, int64_t rs_b2, int64_t cs_b2, T beta,
              T *c, int64_t rs_c, int64_t cs_c, int ji, void *stream, int tri_a = 0,
-             int tri_b = 0) {
+             int tri_b = 0, T *c2 = nullptr, int64_t rs_c2 = 0, int64_t cs_c2 = 0,
+             int c2_lo = 0, int c2_hi = 0) {
   if (m < 0 || n < 0 || k < 0) return MUBLIS_EINVAL;
   if (m == 0 || n == 0) return MUBLIS_OK;
   hipStream_t s = (hipStream_t)(stream ? stream : mublis_rt_stream());
@@ -232,6 +234,9 @@ int gemm_dev(int m, int n, int k, T alpha, const T *a, int64_t rs_a, int64_t cs_
     p.a2 = same2 ? a2 : nullptr; p.rs_a2 = rs_a2; p.cs_a2 = cs_a2;
     p.b2 = same2 ? b2 : nullptr; p.rs_b2 = rs_b2; p.cs_b2 = cs_b2;
     p.c = c; p.rs_c = rs_c; p.cs_c = cs_c; p.ji = ji;
+    p.c2 = c2; p.rs_c2 = rs_c2; p.cs_c2 = cs_c2;
+    p.c2_lo = c2 ? c2_lo : 0;
+    p.c2_hi = c2 ? c2_hi : 0;
     p.tri = (ji != 0 && m == n) ? 1 : 0;
     p.tri_a = a2 ? 0 : tri_a;
     p.tri_b = a2 ? 0 : tri_b;
@@ -330,6 +335,28 @@ template int gfx950_gemm<float>(int, int, int, float, const float *, int64_t, in
 template int gfx950_gemm<double>(int, int, int, double, const double *, int64_t,
                                  int64_t, const double *, int64_t, int64_t, double,
                                  double *, int64_t, int64_t, hipStream_t);
+
+template <typename T>
+int gfx950_gemm_split_c(int m, int n, int k, T alpha, const T *a, int64_t rs_a, int64_t cs_a,
+                        const T *b, int64_t rs_b, int64_t cs_b, T beta, T *c, int64_t rs_c,
+                        int64_t cs_c, T *c2, int64_t rs_c2, int64_t cs_c2, int lo, int hi,
+                        hipStream_t s) {
+  // whole 128-row tiles (and so whole 64-row ones) on either side of both
+  // bounds; the beta-only path (alpha == 0 or k == 0) has no split
+  if (!c2 || k <= 0 || alpha == T(0) || lo < 0 || hi <= lo || lo % 128 != 0 ||
+      (hi % 128 != 0 && hi < m))
+    return MUBLIS_EINVAL;
+  return gemm_dev<T>(m, n, k, alpha, a, rs_a, cs_a, b, rs_b, cs_b, nullptr, 0, 0, nullptr, 0,
+                     0, beta, c, rs_c, cs_c, 0, (void *)s, 0, 0, c2, rs_c2, cs_c2, lo, hi);
+}
+template int gfx950_gemm_split_c<float>(int, int, int, float, const float *, int64_t,
+                                        int64_t, const float *, int64_t, int64_t, float,
+                                        float *, int64_t, int64_t, float *, int64_t, int64_t,
+                                        int, int, hipStream_t);
+template int gfx950_gemm_split_c<double>(int, int, int, double, const double *, int64_t,
+                                         int64_t, const double *, int64_t, int64_t, double,
+                                         double *, int64_t, int64_t, double *, int64_t,
+                                         int64_t, int, int, hipStream_t);
 
 // Batched form for the TRSM inversion: `count` problems of one shape whose
 // operands are offset per problem as GemmArgs::bat / bs* describe (strides

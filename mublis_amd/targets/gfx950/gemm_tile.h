@@ -62,6 +62,13 @@ struct GemmArgs {
   const T *a2; int64_t rs_a2, cs_a2;
   const T *b2; int64_t rs_b2, cs_b2;
   T *c; int64_t rs_c, cs_c;
+  // Rows c2_lo <= i < c2_hi of the result are written to c2 (row i - c2_lo)
+  // instead of c; beta still reads them from c.  Both bounds are multiples
+  // of the tile height, so a tile writes to one of the two.  Used by the
+  // TRSM recursion: the update that last writes a diagonal block's rows of
+  // B writes them straight into the buffer its explicit-inverse leaf reads.
+  T *c2; int64_t rs_c2, cs_c2;
+  int c2_lo, c2_hi;
   int ji;          // 0 all, 1 lower (j <= i), 2 upper (j >= i)
   int tiles_m, tiles_n;
   // Grid over the ji triangle (SYRK/SYR2K): 0 every tile; 1 the tiles
@@ -495,6 +502,13 @@ __device__ inline void gemm_tile(const GemmArgs<T> &p, int tm, int tn, T *lds) {
   auto c_ok = [&](int row, int col) {
     return row < p.m && col < p.n && !(p.ji == 1 && col > row) && !(p.ji == 2 && col < row);
   };
+  T *cw = p.c;
+  int64_t rs_w = p.rs_c, cs_w = p.cs_c;
+  if (m0 >= p.c2_lo && m0 < p.c2_hi) {
+    cw = p.c2 - (int64_t)p.c2_lo * p.rs_c2;
+    rs_w = p.rs_c2;
+    cs_w = p.cs_c2;
+  }
 #pragma unroll
   for (int i = 0; i < TIM; ++i) {
     T cv[TIN][4];
@@ -517,7 +531,7 @@ __device__ inline void gemm_tile(const GemmArgs<T> &p, int tm, int tn, T *lds) {
       for (int r = 0; r < 4; ++r) {
         const int row = m0 + wm + i * 16 + MF::row(lk, r);
         if (!c_ok(row, col)) continue;
-        auto *cp = gmem(p.c) + (int64_t)row * p.rs_c + (int64_t)col * p.cs_c;
+        auto *cp = gmem(cw) + (int64_t)row * rs_w + (int64_t)col * cs_w;
         const T prod = alpha * acc[i][j][r];
         if (beta == T(0)) *cp = prod;
         else if (beta == T(1)) *cp = cv[j][r] + prod;

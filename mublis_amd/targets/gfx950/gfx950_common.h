@@ -32,6 +32,14 @@ template <typename T>
 int gfx950_gemm(int m, int n, int k, T alpha, const T *a, int64_t rs_a, int64_t cs_a,
                 const T *b, int64_t rs_b, int64_t cs_b, T beta, T *c, int64_t rs_c,
                 int64_t cs_c, hipStream_t s);
+// Same, except that rows lo <= i < hi of the result go to c2 (its row
+// i - lo) instead of c; beta still reads them from c.  lo a multiple of 128,
+// hi too unless hi >= m; k > 0 and alpha != 0.
+template <typename T>
+int gfx950_gemm_split_c(int m, int n, int k, T alpha, const T *a, int64_t rs_a, int64_t cs_a,
+                        const T *b, int64_t rs_b, int64_t cs_b, T beta, T *c, int64_t rs_c,
+                        int64_t cs_c, T *c2, int64_t rs_c2, int64_t cs_c2, int lo, int hi,
+                        hipStream_t s);
 // `count` problems of one shape in one launch; operand X of problem z starts
 // at x + (z % bat) * bsx[0] + (z / bat) * bsx[1].  A and B need a unit
 // stride.  tri_a / tri_b as for gfx950_gemm_tri_a.

@@ -953,13 +953,18 @@ int64_t inv_flag_count(int m) {
 }
 
 // How the recursion's diagonal sub-problems end: GEMM leaves when `w` (a
-// kInvLeaf x n copy buffer) is set, else pipelined or column-panel leaves
-// when `flags` is set, else NB-row inverse products.
+// kInvLeaf x n buffer for the leaf's right-hand side) is set, else
+// pipelined or column-panel leaves when `flags` is set, else NB-row inverse
+// products.
+//  * w_ready: the update before the next GEMM leaf has written that leaf's
+//    rows of B into w already (gfx950_gemm_split_c), so the leaf copies
+//    nothing;
 template <typename T>
 struct Leaves {
   int *flags;
   int epoch;
   T *w;
+  bool w_ready = false;
 };
 
 // Recursive schedule.  `ad` holds the NB-block inverses of this sub-problem
@@ -970,7 +975,9 @@ int trsm_rec(bool lower, const double *ad, const T *iv, int m, int n, T alpha, c
              int64_t rs_a, int64_t cs_a, T *b, int64_t rs_b, int64_t cs_b, Leaves<T> &lv,
              hipStream_t s) {
   if (lv.w && m <= kInvLeaf<T>) {
-    int st = gfx950_copy<T>(m, n, b, rs_b, cs_b, lv.w, 1, m, s);
+    int st = MUBLIS_OK;
+    if (!lv.w_ready) st = gfx950_copy<T>(m, n, b, rs_b, cs_b, lv.w, 1, m, s);
+    lv.w_ready = false;
     if (st) return st;
     return gfx950_gemm_tri_a<T>(m, n, m, alpha, iv, 1, kInvLeaf<T>, lv.w, 1, m, T(0), b, rs_b,
                                 cs_b, lower ? 1 : 2, s);
@@ -1002,6 +1009,20 @@ int trsm_rec(bool lower, const double *ad, const T *iv, int m, int n, T alpha, c
   const double *ad1 = ad, *ad2 = ad + (int64_t)(m1 / NB) * NB * NB;
   constexpr int64_t KL = kInvLeaf<T>;
   const T *iv1 = iv, *iv2 = iv ? iv + (int64_t)(m1 / KL) * KL * KL : iv;
+  // With GEMM leaves the update writes the rows of the next leaf (the first
+  // kInvLeaf-block of B2 for lower, the last of B1 for upper: the splits sit
+  // at multiples of kInvLeaf) straight into the leaf's buffer.
+  auto update = [&](int mu, int ku, const T *au, const T *xu, T *bu, int lo, int hi) {
+    if (lv.w) {
+      const int st = gfx950_gemm_split_c<T>(mu, n, ku, T(-1), au, rs_a, cs_a, xu, rs_b, cs_b,
+                                            alpha, bu, rs_b, cs_b, lv.w, 1, hi - lo, lo, hi,
+                                            s);
+      lv.w_ready = st == MUBLIS_OK;
+      return st;
+    }
+    return gfx950_gemm<T>(mu, n, ku, T(-1), au, rs_a, cs_a, xu, rs_b, cs_b, alpha, bu, rs_b,
+                          cs_b, s);
+  };
   int st;
   if (lower) {
     if ((st = trsm_rec<T>(lower, ad1, iv1, m1, n, alpha, a11, rs_a, cs_a, b1, rs_b, cs_b, lv,
@@ -1009,16 +1030,14 @@ int trsm_rec(bool lower, const double *ad, const T *iv, int m, int n, T alpha, c
       return st;
     // B2 := alpha*B2 - A21 * X1 (the reference's GEMM with alpha -1 and
     // beta = alpha_eff, frame/l3/trsm.c:165-197, 241-249)
-    if ((st = gfx950_gemm<T>(m2, n, m1, T(-1), a21, rs_a, cs_a, b1, rs_b, cs_b, alpha,
-                             b2, rs_b, cs_b, s)))
+    if ((st = update(m2, m1, a21, b1, b2, 0, std::min(m2, (int)KL))))
       return st;
     return trsm_rec<T>(lower, ad2, iv2, m2, n, T(1), a22, rs_a, cs_a, b2, rs_b, cs_b, lv, s);
   }
   if ((st = trsm_rec<T>(lower, ad2, iv2, m2, n, alpha, a22, rs_a, cs_a, b2, rs_b, cs_b, lv,
                         s)))
     return st;
-  if ((st = gfx950_gemm<T>(m1, n, m2, T(-1), a12, rs_a, cs_a, b2, rs_b, cs_b, alpha,
-                           b1, rs_b, cs_b, s)))
+  if ((st = update(m1, m2, a12, b2, b1, m1 - (int)KL, m1)))
     return st;
   return trsm_rec<T>(lower, ad1, iv1, m1, n, T(1), a11, rs_a, cs_a, b1, rs_b, cs_b, lv, s);
 }
