@@ -13,7 +13,10 @@ GEMM kernel): achieved = 2*M*N*K per launch / average launch time measured
 with HIP events on the library's own stream; peak = MI355X dense FP64 MFMA
 (78.6 TFLOP/s).  `cpu_baseline` times the reference's own linux_avx2 build
 (oracle/_ref, compiled from the reference sources) on one host core over a
-bounded slice of the same DGEMM.
+bounded slice of the same DGEMM (`value`), and beside it on slices of the
+DTRSM and SGEMM workloads and, with nproc concurrent single-threaded workers,
+on the batched problems and a sample of the serving requests; it runs before
+the process touches the GPU.
 """
 import argparse
 import ctypes
@@ -83,33 +86,211 @@ def timed_calls(torch, lib, fn, steps, warmup):
     return wall, e0.elapsed_time(e1) / steps
 
 
-def cpu_baseline(n_k, m_slice):
-    """Reference linux_avx2 build, one thread, DGEMM op(A)=A^T on an
-    m_slice x m_slice x n_k slice of the workload."""
+def _ref_lib():
+    """The reference's own linux_avx2 build (oracle/_ref, compiled from the
+    reference sources by oracle/Makefile), or None when it is not built."""
     ref = os.path.join(ROOT, "oracle", "_ref", "libref_avx2.so")
-    kind = "reference"
     if not os.path.exists(ref):
         return None
     lib = ctypes.CDLL(ref)
-    f = lib.cblas_dgemm
-    f.argtypes = [ctypes.c_int] * 6 + [ctypes.c_double, ctypes.c_void_p, ctypes.c_int,
-                                       ctypes.c_void_p, ctypes.c_int, ctypes.c_double,
-                                       ctypes.c_void_p, ctypes.c_int]
+    I, P = ctypes.c_int, ctypes.c_void_p
+    for p, ct in (("s", ctypes.c_float), ("d", ctypes.c_double)):
+        getattr(lib, f"cblas_{p}gemm").argtypes = [I] * 6 + [ct, P, I, P, I, ct, P, I]
+        getattr(lib, f"cblas_{p}syrk").argtypes = [I] * 5 + [ct, P, I, ct, P, I]
+        getattr(lib, f"cblas_{p}syr2k").argtypes = [I] * 5 + [ct, P, I, P, I, ct, P, I]
+        getattr(lib, f"cblas_{p}trsm").argtypes = [I] * 7 + [ct, P, I, P, I]
+    return lib
+
+
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def _cpu_workers():
+    """Concurrent CPU workers: the cores this process may run on, at most 16
+    (a one-GPU box's share of its host)."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(n, 16))
+
+
+def _ref_call(lib, r, rng):
+    """Operands (U[-1, 1], column-major) and a closure running request or
+    problem `r` through the reference build; returns (closure, flop)."""
+    op = r["op"]
+    f32 = np.float32
+    if op in ("sgemm", "gemm"):
+        m, n, k = r["m"], r["n"], r["k"]
+        a = rng.uniform(-1, 1, (k, m)).astype(f32)  # column-major m x k, ld m
+        b = rng.uniform(-1, 1, (n, k)).astype(f32)
+        c = rng.uniform(-1, 1, (n, m)).astype(f32)
+        beta = r.get("beta", 0.5)
+        return (lambda: lib.cblas_sgemm(COL, NOTR, NOTR, m, n, k, 1.0, a.ctypes.data, m,
+                                        b.ctypes.data, k, beta, c.ctypes.data, m),
+                2.0 * m * n * k), (a, b, c)
+    if op in ("ssyrk", "ssyr2k", "syrk"):
+        n, k = r["n"], r["k"]
+        uplo = UP if r.get("upper") else LO
+        a = rng.uniform(-1, 1, (k, n)).astype(f32)
+        c = rng.uniform(-1, 1, (n, n)).astype(f32)
+        beta = r.get("beta", 0.5)
+        if op == "ssyr2k":
+            b = rng.uniform(-1, 1, (k, n)).astype(f32)
+            return (lambda: lib.cblas_ssyr2k(COL, uplo, NOTR, n, k, 1.0, a.ctypes.data, n,
+                                             b.ctypes.data, n, beta, c.ctypes.data, n),
+                    2.0 * n * (n + 1) * k), (a, b, c)
+        return (lambda: lib.cblas_ssyrk(COL, uplo, NOTR, n, k, 1.0, a.ctypes.data, n, beta,
+                                        c.ctypes.data, n),
+                1.0 * n * (n + 1) * k), (a, c)
+    m, n = r["m"], r["n"]  # dtrsm: Left, Lower, NoTrans, NonUnit; diagonal m
+    a = rng.uniform(-1, 1, (m, m))
+    np.fill_diagonal(a, float(m))
+    b = rng.uniform(-1, 1, (n, m))
+    return (lambda: lib.cblas_dtrsm(COL, LEFT, LO, NOTR, NONUNIT, m, n, 1.0, a.ctypes.data, m,
+                                    b.ctypes.data, m),
+            float(m) * m * n), (a, b)
+
+
+_BARRIER = None
+
+
+def _cpu_worker(items, reps, seed, out):
+    """One concurrent CPU worker: builds its problems, waits for the others,
+    then runs them `reps` times; puts (seconds spent in the calls, flop)."""
+    lib = _ref_lib()
+    rng = np.random.default_rng(seed)
+    busy, flop = 0.0, 0.0
+    try:
+        _BARRIER.wait(timeout=300)
+        for r in items:
+            (fn, fl), keep = _ref_call(lib, r, rng)
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                fn()
+            busy += time.perf_counter() - t0
+            flop += fl * reps
+            del keep
+    finally:
+        out.put((busy, flop))
+
+
+def _concurrent(items, reps, workers):
+    """Aggregate throughput of `workers` processes, each running its share of
+    `items` (dealt largest first onto the least loaded worker) `reps` times:
+    total flop / the longest worker's time in the calls.  The workers are
+    forked before this process touches the GPU."""
+    import multiprocessing as mp
+    global _BARRIER
+    ctx = mp.get_context("fork")
+    shares, load = [[] for _ in range(workers)], [0.0] * workers
+    for r in sorted(items, key=lambda r: -r["flop"]):
+        w = load.index(min(load))
+        shares[w].append(r)
+        load[w] += r["flop"]
+    _BARRIER = ctx.Barrier(workers)
+    out = ctx.Queue()
+    procs = [ctx.Process(target=_cpu_worker, args=(shares[w], reps, 1000 + w, out))
+             for w in range(workers)]
+    for p in procs:
+        p.start()
+    res = [out.get(timeout=600) for _ in procs]
+    for p in procs:
+        p.join(timeout=60)
+    busy = max(b for b, _ in res)
+    flop = sum(f for _, f in res)
+    return flop / busy / 1e12, busy
+
+
+def cpu_baseline(n_k, m_slice):
+    """The reference's own linux_avx2 build (-O3 -mavx2 -mfma -ffast-math) on
+    the GPU box's host.  `value`: one thread, DGEMM op(A)=A^T on an m_slice x
+    m_slice x n_k slice of the workload.  Beside it, one thread on slices of
+    the DTRSM and SGEMM workloads, and nproc concurrent single-threaded
+    workers (the reference has no threading inside a call) on the batched
+    configuration's problems and on a sample of the serving requests."""
+    lib = _ref_lib()
+    if lib is None:
+        return None
     rng = np.random.default_rng(314159)
     m = m_slice
     a = np.asfortranarray(rng.standard_normal((n_k, m)))
     b = np.asfortranarray(rng.standard_normal((n_k, m)))
     c = np.zeros((m, m), order="F")
     t0 = time.perf_counter()
-    f(COL, TR, NOTR, m, m, n_k, 1.0, a.ctypes.data, n_k, b.ctypes.data, n_k, 0.0,
-      c.ctypes.data, m)
+    lib.cblas_dgemm(COL, TR, NOTR, m, m, n_k, 1.0, a.ctypes.data, n_k, b.ctypes.data, n_k,
+                    0.0, c.ctypes.data, m)
     dt = time.perf_counter() - t0
     flops = 2.0 * m * m * n_k
+    del a, b, c
+    single = {}
+    # DTRSM L/L/N/NonUnit m = n_k on 256 of the 4096 right-hand sides
+    # (off-diagonal N(0,1)/sqrt(m), diagonal 1 + |N(0,1)|, as on the GPU)
+    nrhs = 256
+    at = np.tril(rng.standard_normal((n_k, n_k)) / n_k ** 0.5, -1)
+    at[np.diag_indices(n_k)] = 1.0 + np.abs(rng.standard_normal(n_k))
+    at = np.asfortranarray(at)
+    bt = np.asfortranarray(rng.standard_normal((n_k, nrhs)))
+    t0 = time.perf_counter()
+    lib.cblas_dtrsm(COL, LEFT, LO, NOTR, NONUNIT, n_k, nrhs, 1.0, at.ctypes.data, n_k,
+                    bt.ctypes.data, n_k)
+    t = time.perf_counter() - t0
+    single[f"dtrsm_{n_k}x{nrhs}"] = {"tflops": round(n_k * n_k * nrhs / t / 1e12, 5),
+                                     "seconds": round(t, 2)}
+    del at, bt
+    # SGEMM op(A)=A^T on an m_slice x m_slice x n_k slice
+    a = np.asfortranarray(rng.standard_normal((n_k, m)).astype(np.float32))
+    b = np.asfortranarray(rng.standard_normal((n_k, m)).astype(np.float32))
+    c = np.zeros((m, m), dtype=np.float32, order="F")
+    t0 = time.perf_counter()
+    lib.cblas_sgemm(COL, TR, NOTR, m, m, n_k, 1.0, a.ctypes.data, n_k, b.ctypes.data, n_k,
+                    0.0, c.ctypes.data, m)
+    t = time.perf_counter() - t0
+    single[f"sgemm_{m}x{m}x{n_k}"] = {"tflops": round(2.0 * m * m * n_k / t / 1e12, 5),
+                                      "seconds": round(t, 2)}
+    del a, b, c
+    concurrent = {}
+    workers = _cpu_workers()
+    from mublis_amd.workloads import grouped_shapes, serving_plan
+    from mublis_amd import serving as S
+    ms, ns, ks, _, sn, sk, _ = grouped_shapes()
+    items = [dict(op="gemm", m=int(x), n=int(y), k=int(z), beta=1.0, flop=2.0 * x * y * z)
+             for x, y, z in zip(ms, ns, ks)]
+    items += [dict(op="syrk", n=int(x), k=int(z), beta=1.0, flop=1.0 * x * (x + 1) * z)
+              for x, z in zip(sn, sk)]
+    reps = 8
+    tf, busy = _concurrent(items, reps, workers)
+    concurrent["batched_4096sgemm_1024ssyrk"] = {
+        "tflops": round(tf, 4), "workers": workers, "reps": reps,
+        "seconds": round(busy, 2)}
+    names = {S.SGEMM: "sgemm", S.SSYRK: "ssyrk", S.SSYR2K: "ssyr2k", S.DTRSM: "dtrsm"}
+    sample, total = [], 0.0
+    for r in serving_plan():
+        if total >= 1.2e12:
+            break
+        q = dict(op=names[r["op"]], flop=r["flop"], upper=r.get("uplo") == S.UPPER,
+                 m=r.get("m"), n=r["n"], k=r.get("k"))
+        sample.append(q)
+        total += r["flop"]
+    tf, busy = _concurrent(sample, 1, workers)
+    concurrent["serving_sample"] = {
+        "tflops": round(tf, 4), "requests_per_s": round(len(sample) / busy, 1),
+        "requests": len(sample), "of": 2000, "workers": workers, "seconds": round(busy, 2)}
     return {"value": round(flops / dt / 1e12, 5), "unit": "TFLOP/s", "cores": 1,
-            "kind": kind,
+            "kind": "reference",
             "sample": f"DGEMM op(A)=A^T M=N={m} K={n_k} N(0,1), reference linux_avx2 "
-                      f"build (-O3 -mavx2 -mfma -ffast-math), one thread, "
-                      f"{dt:.2f} s"}
+                      f"build (-O3 -mavx2 -mfma -ffast-math), one thread, {dt:.2f} s",
+            "cpu_model": _cpu_model(), "cores_available": workers,
+            "single_thread": single,
+            "concurrent": concurrent}
 
 
 def profile_mode(args, torch, M, lib, dev, g):
@@ -216,12 +397,28 @@ def trsm_residual(torch, At, Xt, Bt, seed=1):
 
 
 def run_extras(args, torch, M, lib, dev, g, A, B, C, extra):
-    """DTRSM, DSYRK and SGEMM at 8192; the precision workloads (SGEMM 16384
+    """The DGEMM with host pointers; DTRSM, DSYRK and SGEMM at 8192; the
+    precision workloads (SGEMM 16384
     native vs BF16x3 split, STRSM 8192 x 1024 plain vs FP64-refined)."""
     n = args.n
     errs = extra["errors_vs_fp64"]
     steps = max(3, args.steps // 2)
     flops = 2.0 * n ** 3
+    # The same DGEMM with host-pointer operands (numpy arrays): end to end,
+    # PCIe transfers included (pinned staging; C streamed by column panels)
+    Ah, Bh = A.cpu().numpy(), B.cpu().numpy()
+    Ch = np.zeros((n, n))
+
+    def dgemm_host():
+        lib.cblas_dgemm(COL, TR, NOTR, n, n, n, 1.0, Ah.ctypes.data, n, Bh.ctypes.data, n,
+                        0.0, Ch.ctypes.data, n)
+    w_h, _ = timed_calls(torch, lib, dgemm_host, 3, 1)
+    extra["dgemm_8192_host_pointers_ms"] = round(w_h, 2)
+    extra["dgemm_8192_host_pointers_tflops"] = round(flops / (w_h * 1e-3) / 1e12, 3)
+    cols = np.random.default_rng(7).integers(0, n, 16)
+    extra["dgemm_8192_host_pointers_bits_match"] = bool(
+        np.array_equal(Ch[cols], C[torch.from_numpy(cols).to(dev)].cpu().numpy()))
+    del Ah, Bh, Ch
     # DTRSM L/L/N/NonUnit m=8192 nrhs=4096 (off-diag N(0,1)/sqrt(m), diag 1+|N|)
     m, nrhs = n, n // 2
     At = torch.randn((m, m), dtype=torch.float64, device=dev, generator=g) / m ** 0.5
@@ -237,7 +434,9 @@ def run_extras(args, torch, M, lib, dev, g, A, B, C, extra):
 
     def copy_only():
         Bt.copy_(B0)
-    w_t, _ = timed_calls(torch, lib, dtrsm, steps, 1)
+    # three untimed solves first: the host-pointer DGEMM before leaves the
+    # GPU idle between its transfers, and the first solves after it run slow
+    w_t, _ = timed_calls(torch, lib, dtrsm, steps, 3)
     w_c, _ = timed_calls(torch, lib, copy_only, steps, 1)
     t_trsm = max(w_t - w_c, 1e-6)
     extra["dtrsm_8192x4096_tflops"] = round(m * m * nrhs / (t_trsm * 1e-3) / 1e12, 3)
@@ -528,6 +727,9 @@ def run_precision(args, torch, M, lib, dev, g, extra):
     errs["strsm_8192x1024_refined_fwd_vs_dtrsm"] = ((Xr.double() - Xd).norm() /
                                                     Xd.norm()).item()
     errs["strsm_8192x1024_refined_residual"] = trsm_residual(torch, At, Xr, B0)
+    # the floor for an FP32 answer: the FP64 solution rounded to FP32, same
+    # FP32 triangle and right-hand side
+    errs["dtrsm_8192x1024_fp32_rounded_residual"] = trsm_residual(torch, At, Xd.float(), B0)
     errs["dtrsm_8192x1024_residual"] = trsm_residual(torch, Ad, Xd, B0.double())
 
 
@@ -549,6 +751,11 @@ def main():
         hwq = 4
     if hwq < 8:
         os.environ["GPU_MAX_HW_QUEUES"] = "8"
+    # The CPU baseline runs first: its concurrent workers are forked from a
+    # process that has not touched the GPU yet.
+    cpu = None
+    if not (args.no_cpu or args.only or args.traffic_probe):
+        cpu = cpu_baseline(args.n, args.cpu_slice)
     import torch
     import mublis_amd as M
 
@@ -610,7 +817,7 @@ def main():
         "roofline": {"bound": "mfma", "achieved": round(achieved, 3), "peak": PEAK["f64"],
                      "unit": "TFLOP/s", "frac": round(achieved / PEAK["f64"], 4),
                      "traffic": traffic},
-        "cpu_baseline": None if args.no_cpu else cpu_baseline(n, args.cpu_slice),
+        "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
 

@@ -48,6 +48,19 @@ void *mublis_rt_alloc(size_t bytes);
 void mublis_rt_free(void *p);
 int mublis_rt_h2d(void *dst, const void *src, size_t bytes, void *stream);
 int mublis_rt_d2h(void *dst, const void *src, size_t bytes, void *stream);
+/* A rows x cols host block (rows contiguous, ld elements between column
+ * starts, elements of esz bytes) to or from a dense device block, through
+ * the library's pool of pinned host blocks: host columns are gathered into
+ * a pinned block by a team of copy threads while the previous block's DMA
+ * runs.  h2d returns once every piece is queued on stream (the host memory
+ * may be reused then); d2h returns once the data is in host memory.  The
+ * host-pointer staging of the reference's pack-buffer pool
+ * (frame/base/pool.c:120-318) for a device across PCIe; calls serialise on
+ * the pool. */
+int mublis_rt_h2d_2d(void *dev, const void *host, int64_t rows, int64_t cols,
+                     int64_t ld, size_t esz, void *stream);
+int mublis_rt_d2h_2d(void *host, const void *dev, int64_t rows, int64_t cols,
+                     int64_t ld, size_t esz, void *stream);
 int mublis_rt_sync(void *stream);
 /* Last HIP error text seen by this thread (for diagnostics). */
 const char *mublis_rt_last_error(void);
@@ -77,6 +90,22 @@ int mublis_gfx950_sgemm_dev(int m, int n, int k, float alpha,
                             const float *b2, int64_t rs_b2, int64_t cs_b2,
                             float beta, float *c, int64_t rs_c, int64_t cs_c,
                             int ji, void *stream);
+/* C := alpha * Aview * Bview + beta * C with C in host memory (rs_c == 1,
+ * cs_c >= m) and A, B in host or device memory (a host operand needs a unit
+ * stride): host A, and host B whose columns are not contiguous, are sent
+ * whole; C is then computed by column panels, each panel's B and C columns
+ * sent, multiplied and brought back while the next panel is sent, so the
+ * PCIe transfers overlap the GEMM kernels.  Returns once C is in host
+ * memory.  k > 0 and m, n > 0.  The host-pointer form of cblas_?gemm
+ * (frame/compat/cblas/cblas.c:188-200 -> frame/l3/gemm.c:14-24). */
+int mublis_gfx950_dgemm_host(int m, int n, int k, double alpha, const double *a,
+                             int64_t rs_a, int64_t cs_a, const double *b,
+                             int64_t rs_b, int64_t cs_b, double beta, double *c,
+                             int64_t rs_c, int64_t cs_c, void *stream);
+int mublis_gfx950_sgemm_host(int m, int n, int k, float alpha, const float *a,
+                             int64_t rs_a, int64_t cs_a, const float *b,
+                             int64_t rs_b, int64_t cs_b, float beta, float *c,
+                             int64_t rs_c, int64_t cs_c, void *stream);
 
 /*
  * Materialise the view of an m x n operand into dense column-major `dst`

@@ -25,6 +25,7 @@ typedef struct {
   void *dev;        /* device pointer used by kernels */
   size_t bytes;     /* footprint in bytes */
   int owned;        /* dev is a staging buffer owned by the call */
+  size_t esz;       /* element size of an owned buffer */
 } stage_t;
 
 /* Stage an m x n operand with strides (rs, cs); copy == 0 skips the H2D. */

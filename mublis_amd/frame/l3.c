@@ -20,7 +20,9 @@
  * Staging: a pointer that is not device memory is copied to a device
  * buffer covering its whole strided footprint (so gaps between columns and
  * the unreferenced triangle travel unchanged), and outputs are copied back
- * the same way.  Every call synchronises its stream before returning.
+ * the same way; large footprints travel through the pinned pool.  Large
+ * GEMMs with C in host memory stream C by column panels instead
+ * (host_gemm_ok).  Every call synchronises its stream before returning.
  */
 #include <stdint.h>
 #include <stdlib.h>
@@ -85,7 +87,25 @@ static int operand_ok(const mublis_l3_operand_t *o) {
   return (unsigned)o->struc <= 2u && (unsigned)o->uplo <= 2u && (unsigned)o->diag <= 2u;
 }
 
-#define DRIVER_IMPL(T, name, gemm_dev, pack_dev)                                \
+/* Large GEMMs with C in host memory take the panel pipeline
+ * (mublis_gfx950_?gemm_host), whose transfers overlap the kernels: C
+ * column-major, host operands with a unit stride, columns short enough for
+ * 128-column panels of the pinned pool, at least 16 MiB of operands. */
+static int host_gemm_ok(int m, int n, int k, const void *a, int rs_a, int cs_a,
+                        const void *b, int rs_b, int cs_b, const void *c, int rs_c,
+                        int cs_c, size_t esz) {
+  const double bytes = ((double)m * k + (double)k * n + (double)m * n) * (double)esz;
+  const int tall = m > k ? m : k;
+  if (rs_c != 1 || cs_c < m || bytes < 16.0 * (1 << 20) ||
+      (double)tall * (double)esz * 128.0 > 64.0 * (1 << 20))
+    return 0;
+  if (mublis_rt_is_device_ptr(c)) return 0;
+  if (rs_a != 1 && cs_a != 1 && m > 1 && k > 1 && !mublis_rt_is_device_ptr(a)) return 0;
+  if (rs_b != 1 && cs_b != 1 && k > 1 && n > 1 && !mublis_rt_is_device_ptr(b)) return 0;
+  return 1;
+}
+
+#define DRIVER_IMPL(T, name, gemm_dev, pack_dev, gemm_host)                     \
   int name(int m, int n, int k, T alpha, const T *a, int rs_a, int cs_a,       \
            const T *b, int rs_b, int cs_b, T beta, T *c, int rs_c, int cs_c,   \
            const mublis_l3_product_t *product) {                               \
@@ -105,6 +125,13 @@ static int operand_ok(const mublis_l3_operand_t *o) {
     const int mask_a = (int)dom->pi, mask_b = (int)dom->jp;                    \
     const int pack_a = oa->struc != MUBLIS_STRUC_GENERAL || mask_a;            \
     const int pack_b = ob->struc != MUBLIS_STRUC_GENERAL || mask_b;            \
+    if (use && !pack_a && !pack_b && dom->ji == 0 && c &&                      \
+        host_gemm_ok(m, n, k, a, rs_a, cs_a, b, rs_b, cs_b, c, rs_c, cs_c,     \
+                     sizeof(T))) {                                             \
+      st = gemm_host(m, n, k, alpha, a, rs_a, cs_a, b, rs_b, cs_b, beta, c,    \
+                     rs_c, cs_c, s);                                           \
+      return finish(st, s, NULL, NULL, 0);                                     \
+    }                                                                          \
     int ra_rows, ra_cols, rb_rows, rb_cols;                                    \
     view_extent(oa, m, k, &ra_rows, &ra_cols);                                 \
     view_extent(ob, k, n, &rb_rows, &rb_cols);                                 \
@@ -151,9 +178,9 @@ static int operand_ok(const mublis_l3_operand_t *o) {
   }
 
 DRIVER_IMPL(float, mublis_l3_sdriver, mublis_gfx950_sgemm_dev,
-            mublis_gfx950_spack_dev)
+            mublis_gfx950_spack_dev, mublis_gfx950_sgemm_host)
 DRIVER_IMPL(double, mublis_l3_ddriver, mublis_gfx950_dgemm_dev,
-            mublis_gfx950_dpack_dev)
+            mublis_gfx950_dpack_dev, mublis_gfx950_dgemm_host)
 
 static const mublis_l3_product_t k_general = {
     {MUBLIS_L3_ALL, MUBLIS_L3_ALL, MUBLIS_L3_ALL},

@@ -10,6 +10,10 @@
 #include "frame_internal.h"
 #include "mublis_gfx950.h"
 
+/* Footprints from this size travel through the pinned pool
+ * (mublis_rt_h2d_2d / _d2h_2d); smaller ones as one pageable copy. */
+#define PINNED_MIN ((size_t)4 << 20)
+
 /* Footprint, in elements, of an m x n view with strides (rs, cs). */
 static int footprint(int m, int n, int64_t rs, int64_t cs, int64_t *elems) {
   if (m <= 0 || n <= 0) {
@@ -37,11 +41,19 @@ int mublis_stage_in(const void *p, int m, int n, int64_t rs, int64_t cs,
   st->dev = mublis_rt_alloc(st->bytes);
   if (!st->dev) return MUBLIS_ENOMEM;
   st->owned = 1;
-  return copy ? mublis_rt_h2d(st->dev, p, st->bytes, stream) : MUBLIS_OK;
+  st->esz = esz;
+  if (!copy) return MUBLIS_OK;
+  if (st->bytes >= PINNED_MIN)
+    return mublis_rt_h2d_2d(st->dev, p, elems, 1, elems, esz, stream);
+  return mublis_rt_h2d(st->dev, p, st->bytes, stream);
 }
 
 int mublis_stage_out(stage_t *st, void *stream) {
   if (!st->owned) return MUBLIS_OK;
+  if (st->bytes >= PINNED_MIN) {
+    const int64_t elems = (int64_t)(st->bytes / st->esz);
+    return mublis_rt_d2h_2d((void *)st->host, st->dev, elems, 1, elems, st->esz, stream);
+  }
   return mublis_rt_d2h((void *)st->host, st->dev, st->bytes, stream);
 }
 

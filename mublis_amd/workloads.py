@@ -1,5 +1,8 @@
 """Synthetic workloads of the benchmark configurations, built on the device.
 
+The sizes are drawn on the host first (grouped_shapes, serving_plan), so a
+CPU run can draw the same problems without a device.
+
 `grouped_problems` draws the batched configuration: GEMM problems with M,
 N, K log-uniform in a range and SYRK problems with n and k log-uniform,
 entries uniform[-1, 1], column-major device tensors, plus the
@@ -36,14 +39,12 @@ def _carve(torch, shapes, gen, device, dtype=None):
     return pool, [pool[o:o + c * r].view(c, r) for o, (c, r) in zip(offs, shapes)]
 
 
-def grouped_problems(torch, seed=314159, n_gemm=4096, n_syrk=1024, gemm_range=(16, 512),
-                     syrk_n=(32, 1024), syrk_k=(16, 512), alpha=1.0, beta=1.0,
-                     vary_flags=False, device="cuda", n_syr2k=0):
-    """Returns (ops, keep, specs): SbatchOp list, the tensors they point at
-    (kept alive by the caller) and per-problem dicts for checking.  SYR2K
-    problems (n_syr2k, drawn like the SYRK ones) come last."""
+def grouped_shapes(seed=314159, n_gemm=4096, n_syrk=1024, gemm_range=(16, 512),
+                   syrk_n=(32, 1024), syrk_k=(16, 512), vary_flags=False, n_syr2k=0):
+    """The batched configuration's problem sizes and flags, drawn on the host:
+    (ms, ns, ks, gflags, sn, sk, sflags) -- GEMM M, N, K and (trans_a,
+    trans_b), then SYRK (and SYR2K, last) n, k and (trans, upper)."""
     rng = np.random.default_rng(seed)
-    g = torch.Generator(device=device).manual_seed(seed)
     ms = _logu(rng, *gemm_range, n_gemm)
     ns = _logu(rng, *gemm_range, n_gemm)
     ks = _logu(rng, *gemm_range, n_gemm)
@@ -53,6 +54,18 @@ def grouped_problems(torch, seed=314159, n_gemm=4096, n_syrk=1024, gemm_range=(1
     sk = _logu(rng, *syrk_k, n_syrk + n_syr2k)
     sflags = [(int(rng.integers(0, 2)), int(rng.integers(0, 2))) if vary_flags else (0, 0)
               for _ in range(n_syrk + n_syr2k)]
+    return ms, ns, ks, gflags, sn, sk, sflags
+
+
+def grouped_problems(torch, seed=314159, n_gemm=4096, n_syrk=1024, gemm_range=(16, 512),
+                     syrk_n=(32, 1024), syrk_k=(16, 512), alpha=1.0, beta=1.0,
+                     vary_flags=False, device="cuda", n_syr2k=0):
+    """Returns (ops, keep, specs): SbatchOp list, the tensors they point at
+    (kept alive by the caller) and per-problem dicts for checking.  SYR2K
+    problems (n_syr2k, drawn like the SYRK ones) come last."""
+    g = torch.Generator(device=device).manual_seed(seed)
+    ms, ns, ks, gflags, sn, sk, sflags = grouped_shapes(seed, n_gemm, n_syrk, gemm_range,
+                                                        syrk_n, syrk_k, vary_flags, n_syr2k)
     # column-major rows x cols = tensor (cols, rows)
     shapes = []
     for i in range(n_gemm):
@@ -136,66 +149,92 @@ def _log_uniform(rng, lo, hi):
     return int(round(math.exp(rng.uniform(math.log(lo), math.log(hi)))))
 
 
-def serving_requests(torch, lib, server, seed=2024, count=2000, alpha=1.0, beta=0.5):
-    """The mixed-L3 serving workload on device-resident matrices.
+_SERVE_POOL = 1 << 25  # elements of each random pool; covers the largest operand (4096^2)
+
+
+def serving_plan(seed=2024, count=2000):
+    """The mixed-L3 serving workload's request shapes, drawn on the host.
 
     60% SGEMM (M, N, K log-uniform [64, 4096]), 20% SSYRK/SSYR2K (half each;
     n log-uniform [128, 4096], k log-uniform [64, 2048], triangle chosen at
     random), 20% DTRSM (Left, Lower, NoTrans, NonUnit; m log-uniform
-    [256, 4096], nrhs log-uniform [1, 512], A diagonally dominant: diagonal
-    m, off-diagonal U[-1, 1]).  Entries U[-1, 1], filled on the device from a
-    shared random pool.  Returns (requests, matrices): each request is a dict
-    of mublis_submit keyword arguments plus "op" and "flop".
-    """
+    [256, 4096], nrhs log-uniform [1, 512]).  Each entry is a dict with "op"
+    (a mublis_amd.serving op code), its sizes, "uplo" where it applies and
+    "mats": the operands in draw order, (role, dtype, rows, cols, offset)
+    with `offset` the operand's start in its random pool.  Pure Python: the
+    CPU baseline draws the same requests without a device."""
     from . import serving as S
     import random
     rng = random.Random(seed)
-    gen = torch.Generator(device="cuda").manual_seed(seed)
-    n_pool = 1 << 25  # covers the largest operand (4096 x 4096)
-    pools = {S.F32: torch.rand(n_pool, device="cuda", generator=gen) * 2 - 1,
-             S.F64: torch.rand(n_pool, device="cuda", generator=gen,
-                               dtype=torch.float64) * 2 - 1}
-    copy = {S.F32: lib.mublis_gfx950_scopy_dev, S.F64: lib.mublis_gfx950_dcopy_dev}
-    mats = []
+    plan = []
 
-    def matrix(dt, rows, cols):
-        h = server.matrix(dt, rows, cols)
-        pool = pools[dt]
-        off = rng.randrange(0, n_pool - rows * cols + 1)
-        addr = pool.data_ptr() + off * pool.element_size()
-        st = copy[dt](rows, cols, addr, 1, rows, h.ptr, 1, h.ld, None)
-        assert st == 0, st
-        mats.append(h)
-        return h
+    def mat(r, role, dt, rows, cols):
+        r["mats"].append((role, dt, rows, cols, rng.randrange(0, _SERVE_POOL - rows * cols + 1)))
 
-    reqs, keep = [], []
     for _ in range(count):
         u = rng.random()
         if u < 0.6:
             m, n, k = (_log_uniform(rng, 64, 4096) for _ in range(3))
-            r = dict(op=S.SGEMM, m=m, n=n, k=k, alpha=alpha, beta=beta,
-                     a=matrix(S.F32, m, k), b=matrix(S.F32, k, n), c=matrix(S.F32, m, n))
+            r = dict(op=S.SGEMM, m=m, n=n, k=k, mats=[])
+            mat(r, "a", S.F32, m, k)
+            mat(r, "b", S.F32, k, n)
+            mat(r, "c", S.F32, m, n)
             r["flop"] = 2.0 * m * n * k
         elif u < 0.8:
             n, k = _log_uniform(rng, 128, 4096), _log_uniform(rng, 64, 2048)
             two = rng.random() < 0.5
             uplo = S.LOWER if rng.random() < 0.5 else S.UPPER
-            r = dict(op=S.SSYR2K if two else S.SSYRK, n=n, k=k, alpha=alpha, beta=beta,
-                     uplo=uplo, a=matrix(S.F32, n, k), c=matrix(S.F32, n, n))
+            r = dict(op=S.SSYR2K if two else S.SSYRK, n=n, k=k, uplo=uplo, mats=[])
+            mat(r, "a", S.F32, n, k)
+            mat(r, "c", S.F32, n, n)
             if two:
-                r["b"] = matrix(S.F32, n, k)
+                mat(r, "b", S.F32, n, k)
             r["flop"] = (2.0 if two else 1.0) * n * (n + 1) * k
         else:
             m, nrhs = _log_uniform(rng, 256, 4096), _log_uniform(rng, 1, 512)
-            a = matrix(S.F64, m, m)
-            diag = torch.full((m,), float(m), dtype=torch.float64, device="cuda")
-            st = lib.mublis_gfx950_dcopy_dev(1, m, diag.data_ptr(), 1, 1, a.ptr, 1, a.ld + 1,
-                                             None)
-            assert st == 0, st
-            keep.append(diag)  # alive until the copies have run
-            r = dict(op=S.DTRSM, m=m, n=nrhs, alpha=alpha, uplo=S.LOWER, a=a,
-                     b=matrix(S.F64, m, nrhs))
+            r = dict(op=S.DTRSM, m=m, n=nrhs, uplo=S.LOWER, mats=[])
+            mat(r, "a", S.F64, m, m)
+            mat(r, "b", S.F64, m, nrhs)
             r["flop"] = float(m) * m * nrhs
+        plan.append(r)
+    return plan
+
+
+def serving_requests(torch, lib, server, seed=2024, count=2000, alpha=1.0, beta=0.5):
+    """The mixed-L3 serving workload (serving_plan) on device-resident
+    matrices: entries U[-1, 1], copied from a shared random pool on the
+    device at the plan's offsets; DTRSM triangles diagonally dominant
+    (diagonal m, off-diagonal U[-1, 1]).  Returns (requests, matrices): each
+    request is a dict of mublis_submit keyword arguments plus "op" and
+    "flop"."""
+    from . import serving as S
+    gen = torch.Generator(device="cuda").manual_seed(seed)
+    pools = {S.F32: torch.rand(_SERVE_POOL, device="cuda", generator=gen) * 2 - 1,
+             S.F64: torch.rand(_SERVE_POOL, device="cuda", generator=gen,
+                               dtype=torch.float64) * 2 - 1}
+    copy = {S.F32: lib.mublis_gfx950_scopy_dev, S.F64: lib.mublis_gfx950_dcopy_dev}
+    mats, reqs, keep = [], [], []
+    for p in serving_plan(seed, count):
+        r = {k: v for k, v in p.items() if k != "mats"}
+        if p["op"] != S.DTRSM:
+            r["alpha"], r["beta"] = alpha, beta
+        else:
+            r["alpha"] = alpha
+        for role, dt, rows, cols, off in p["mats"]:
+            h = server.matrix(dt, rows, cols)
+            pool = pools[dt]
+            addr = pool.data_ptr() + off * pool.element_size()
+            st = copy[dt](rows, cols, addr, 1, rows, h.ptr, 1, h.ld, None)
+            assert st == 0, st
+            mats.append(h)
+            r[role] = h
+            if p["op"] == S.DTRSM and role == "a":
+                m = rows
+                diag = torch.full((m,), float(m), dtype=torch.float64, device="cuda")
+                st = lib.mublis_gfx950_dcopy_dev(1, m, diag.data_ptr(), 1, 1, h.ptr, 1,
+                                                 h.ld + 1, None)
+                assert st == 0, st
+                keep.append(diag)  # alive until the copies have run
         reqs.append(r)
     lib.mublis_rt_sync(None)
     torch.cuda.synchronize()
