@@ -93,7 +93,8 @@ __global__ __launch_bounds__((KernelShape<T, TILE, SHALLOW>::NTH),
   }
   int tm, tn;
   if (p.lpt) {
-    if (!map_tile_lpt(blockIdx.x, p.tiles_m, p.tiles_n, p.lpt, p.lpt_g, tm, tn)) return;
+    if (!map_tile_lpt(blockIdx.x, p.tiles_m, p.tiles_n, p.lpt, p.lpt_g, tm, tn, p.lpt_r1))
+      return;
   } else {
     map_tile(blockIdx.x, p.ntiles, p.tiles_m, p.tiles_n, p.tri, p.ji, tm, tn);
     if (tm >= p.tiles_m || tn >= p.tiles_n) return;  // mode 3 past an odd last tile
@@ -153,6 +154,8 @@ hipError_t launch(GemmArgs<T> p, bool akc, bool bnc, hipStream_t s, int nbatch =
     p.lpt_g = lpt_unit(lines, per);
     const int64_t units = (int64_t)lines * ((per + p.lpt_g - 1) / p.lpt_g);
     wgs = (int)(8 * ((units + 7) / 8) * p.lpt_g);
+    // (long-first launches always take the deep slice)
+    p.lpt_r1 = nbatch == 1 ? 32 * KernelShape<T, TILE, false>::WG_PER_CU : 0;
   }
   const dim3 grid(wgs, nbatch);
   if constexpr (TILE == 64 || sizeof(T) == 4) {
@@ -244,6 +247,7 @@ int gemm_dev(int m, int n, int k, T alpha, const T *a, int64_t rs_a, int64_t cs_
     // the tiles long-first, balanced over the XCDs.
     p.lpt = 0;
     p.lpt_g = 1;
+    p.lpt_r1 = 0;
     if (!p.tri) {
       if (p.tri_a) p.lpt = p.tri_a == 1 ? 1 : 2;
       else if (p.tri_b) p.lpt = p.tri_b == 2 ? 3 : 4;

@@ -87,6 +87,9 @@ struct GemmArgs {
   // tile rows are the lines, longest at the high / low end; 3 / 4 the same
   // for tile columns.  lpt_g: tiles per dealt unit (a whole line, or 1).
   int lpt, lpt_g;
+  // workgroups per XCD in one round, for the long-first walk's first-round
+  // snake (map_tile_lpt's r1); 0 for batched launches
+  int lpt_r1;
   // Batched launch (gridDim.y problems, the one-problem kernel only):
   // problem z starts at a + (z % bat) * bsa[0] + (z / bat) * bsa[1], and
   // likewise for b and c.
@@ -137,12 +140,29 @@ __host__ inline int common_layout(int rows, int cols, int64_t &r1, int64_t &c1, 
 // which lets the short tiles fill the slots the long ones leave (longest
 // processing time first).  The grid is 8 * ceil(units / 8) * g workgroups;
 // returns false for the padding workgroups.
+//
+// r1 (0: off) is the number of workgroups an XCD holds at once, 32 CUs
+// times the workgroups per CU.  Those first r1 of each XCD start together,
+// dealt over its CUs breadth-first (local index l to CU l % 32), so in
+// plain long-first order CU 0 would take the longest tile of every row of
+// 32 and CU 31 the shortest.  Every other row of 32 is therefore reversed
+// (a snake), pairing long tiles with short on every CU.  Measured on
+// MI355X, the 1024 x 4096 x 1024 triangular product of the TRSM leaves (one
+// round of 64-tiles): 97 -> 89.5 us; the depth-first pairing (a CU's
+// slots taking consecutive indices) ran 133 us, which is how the dispatch
+// order was told apart.
 __host__ __device__ inline bool map_tile_lpt(int w, int tiles_m, int tiles_n, int lpt, int g,
-                                             int &tm, int &tn) {
+                                             int &tm, int &tn, int r1 = 0) {
   const bool rows = lpt <= 2;
   const int lines = rows ? tiles_m : tiles_n, per = rows ? tiles_n : tiles_m;
   const int upl = (per + g - 1) / g;  // units per line
-  const int xcd = w % 8, local = w / 8;
+  const int xcd = w % 8, local0 = w / 8;
+  int local = local0;
+  if (local0 < r1) {
+    const int nx = (lines * upl + 7) / 8 * g, nr = r1 < nx ? r1 : nx;  // first round
+    const int q = local0 / 32, j = local0 - 32 * q, len = nr - 32 * q < 32 ? nr - 32 * q : 32;
+    if (q & 1) local = 32 * q + (len - 1 - j);
+  }
   const int k = local / g, oi = local - k * g;
   const int d = 8 * k + ((k & 1) ? 7 - xcd : xcd);  // unit rank in decreasing K
   const int dl = d / upl, o = (d - dl * upl) * g + oi;

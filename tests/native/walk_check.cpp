@@ -6,7 +6,11 @@
 //  * map_tile_lpt, lines along rows or columns, long end high or low: a
 //    bijection onto the tiles (padding workgroups excepted), every XCD
 //    (workgroup id mod 8) receives the same total K up to one dealt unit,
-//    and each XCD meets its tiles in non-increasing K.
+//    and each XCD meets its tiles in non-increasing K -- after its first
+//    round when the first-round snake is on (r1 > 0): then no later tile is
+//    longer than a first-round one, and within the first round, dealt over
+//    the XCD's 32 CUs breadth-first (its local index l to CU l % 32), every
+//    CU gets about the same K, where the plain order fails that bound.
 #include <cstdio>
 #include <cstdlib>
 #include <set>
@@ -85,7 +89,7 @@ static void split_triangle(int n, int ji) {
     }
 }
 
-static void lpt(int tm_, int tn_, int mode) {
+static void lpt(int tm_, int tn_, int mode, int r1 = 0) {
   const bool rows = mode <= 2;
   const int lines = rows ? tm_ : tn_, per = rows ? tn_ : tm_;
   const int g = lpt_unit(lines, per);
@@ -98,18 +102,81 @@ static void lpt(int tm_, int tn_, int mode) {
   };
   std::set<std::pair<int, int>> seen;
   long sum[8] = {0};
-  int last[8];
-  for (int x = 0; x < 8; ++x) last[x] = 1 << 30;
+  int last[8], first_min[8];
+  // first-round K per (XCD, CU), this walk and the plain one (r1 = 0)
+  std::vector<long> cu(8 * 32, 0), cu_plain(8 * 32, 0);
+  for (int x = 0; x < 8; ++x) last[x] = first_min[x] = 1 << 30;
   for (int w = 0; w < wgs; ++w) {
     int a, b;
-    if (!map_tile_lpt(w, tm_, tn_, mode, g, a, b)) continue;
+    const int x = w % 8, l = w / 8;
+    int pa, pb;
+    if (l < r1 && map_tile_lpt(w, tm_, tn_, mode, g, pa, pb))
+      cu_plain[x * 32 + l % 32] += kof(pa, pb);
+    if (!map_tile_lpt(w, tm_, tn_, mode, g, a, b, r1)) continue;
     CHECK(a >= 0 && a < tm_ && b >= 0 && b < tn_, "lpt %dx%d m%d w %d -> %d,%d", tm_, tn_,
           mode, w, a, b);
     CHECK(seen.insert({a, b}).second, "lpt %dx%d m%d duplicate", tm_, tn_, mode);
-    const int x = w % 8, k = kof(a, b);
-    CHECK(k <= last[x], "lpt %dx%d m%d XCD %d K rises %d -> %d", tm_, tn_, mode, x, last[x], k);
-    last[x] = k;
+    const int k = kof(a, b);
+    if (l < r1) {
+      cu[x * 32 + l % 32] += k;
+      first_min[x] = k < first_min[x] ? k : first_min[x];
+    } else {
+      CHECK(k <= last[x] && k <= first_min[x], "lpt %dx%d m%d XCD %d K rises %d -> %d", tm_,
+            tn_, mode, x, last[x] < first_min[x] ? last[x] : first_min[x], k);
+      last[x] = k;
+    }
     sum[x] += k;
+  }
+  if (r1 > 0) {
+    // Checked where an XCD's first round is at least two whole rows of 32.
+    // Row q of the plain order hands CU c the (32q + c)-th longest tile; the
+    // snake pairs row q's c-th with row q+1's (31 - c)-th, which cancels K's
+    // decrease along the rows up to rounding (2 per pair); an unpaired last
+    // row adds its own K range.
+    const int nx = wgs / 8, nr = r1 < nx ? r1 : nx;
+    if (nr >= 64 && nr % 32 == 0) {
+      const int nrow = nr / 32;
+      for (int x = 0; x < 8; ++x) {
+        long range_last = 0;
+        {
+          int pa, pb, kf = 0, kl = 0;
+          const int q = nrow - 1;
+          if (map_tile_lpt(x + 8 * (32 * q), tm_, tn_, mode, g, pa, pb)) kf = kof(pa, pb);
+          if (map_tile_lpt(x + 8 * (32 * q + 31), tm_, tn_, mode, g, pa, pb)) kl = kof(pa, pb);
+          range_last = kf - kl;
+        }
+        // the largest K step between neighbouring first-round workgroups
+        long step = 1;
+        for (int l = 0; l + 1 < nr; ++l) {
+          int pa, pb, qa, qb;
+          if (!map_tile_lpt(x + 8 * l, tm_, tn_, mode, g, pa, pb) ||
+              !map_tile_lpt(x + 8 * (l + 1), tm_, tn_, mode, g, qa, qb))
+            continue;
+          const long d = (long)kof(pa, pb) - kof(qa, qb);
+          step = d > step ? d : step;
+        }
+        const long bound = (2L * (nrow / 2) + 2) * step + ((nrow & 1) ? range_last : 0);
+        long lo = 1L << 40, hi = -1, plo = 1L << 40, phi = -1;
+        for (int c = 0; c < 32; ++c) {
+          const long v = cu[x * 32 + c], pv = cu_plain[x * 32 + c];
+          lo = v < lo ? v : lo;
+          hi = v > hi ? v : hi;
+          plo = pv < plo ? pv : plo;
+          phi = pv > phi ? pv : phi;
+        }
+        CHECK(hi - lo <= bound, "lpt %dx%d m%d r1 %d XCD %d first-round CU K spread %ld > %ld",
+              tm_, tn_, mode, r1, x, hi - lo, bound);
+        if (tm_ == 16 && tn_ == 64 && mode <= 2 && r1 == 128) {
+          // the TRSM leaf grid: the plain order fails the bound by far
+          CHECK(phi - plo > bound && 2 * (hi - lo) < phi - plo,
+                "lpt 16x64 XCD %d spread %ld, plain %ld, bound %ld", x, hi - lo, phi - plo,
+                bound);
+          if (x == 0 && mode == 1)
+            std::printf("16x64 first round: CU K spread %ld (plain order %ld, bound %ld)\n",
+                        hi - lo, phi - plo, bound);
+        }
+      }
+    }
   }
   CHECK((int)seen.size() == tm_ * tn_, "lpt %dx%d m%d covers %zu", tm_, tn_, mode, seen.size());
   long lo = sum[0], hi = sum[0];
@@ -138,7 +205,8 @@ int main() {
     for (int ji : {1, 2}) split_triangle(n, ji);
   for (int a : {1, 2, 5, 8, 9, 16, 31, 32, 33, 64, 128, 129})
     for (int b : {1, 3, 8, 16, 17, 64})
-      for (int m = 1; m <= 4; ++m) lpt(a, b, m);
+      for (int m = 1; m <= 4; ++m)
+        for (int r1 : {0, 64, 128}) lpt(a, b, m, r1);  // 16 x 64, r1 128: the TRSM leaves
   if (failures) {
     std::printf("%d failures\n", failures);
     return 1;
