@@ -93,7 +93,11 @@ __global__ __launch_bounds__((KernelShape<T, TILE, SHALLOW>::NTH),
   }
   int tm, tn;
   if (p.lpt) {
-    if (!map_tile_lpt(blockIdx.x, p.tiles_m, p.tiles_n, p.lpt, p.lpt_g, tm, tn, p.lpt_r1))
+    // batched: problems of at most 32 workgroups per XCD share rows of 32
+    const int per_xcd = (int)gridDim.x / 8;
+    const bool rev = gridDim.y > 1 && per_xcd <= 32 && 32 % per_xcd == 0 &&
+                     ((int)blockIdx.y * per_xcd / 32) % 2 == 1;
+    if (!map_tile_lpt(blockIdx.x, p.tiles_m, p.tiles_n, p.lpt, p.lpt_g, tm, tn, p.lpt_r1, rev))
       return;
   } else {
     map_tile(blockIdx.x, p.ntiles, p.tiles_m, p.tiles_n, p.tri, p.ji, tm, tn);

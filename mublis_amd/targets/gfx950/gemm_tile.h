@@ -151,15 +151,24 @@ __host__ inline int common_layout(int rows, int cols, int64_t &r1, int64_t &c1, 
 // round of 64-tiles): 97 -> 89.5 us; the depth-first pairing (a CU's
 // slots taking consecutive indices) ran 133 us, which is how the dispatch
 // order was told apart.
+//
+// rev reverses an XCD's order instead: a batched launch of small problems
+// (the TRSM inversion's doubling GEMMs) lays several problems' workgroups
+// side by side in one row of 32, so each CU takes the same-rank tile of two
+// problems; reversing the problems of every other row pairs long with short
+// there (the doubling's 8 x (512 x 512 x 512) level: 45.5 -> 42 us).
 __host__ __device__ inline bool map_tile_lpt(int w, int tiles_m, int tiles_n, int lpt, int g,
-                                             int &tm, int &tn, int r1 = 0) {
+                                             int &tm, int &tn, int r1 = 0, bool rev = false) {
   const bool rows = lpt <= 2;
   const int lines = rows ? tiles_m : tiles_n, per = rows ? tiles_n : tiles_m;
   const int upl = (per + g - 1) / g;  // units per line
   const int xcd = w % 8, local0 = w / 8;
+  const int nx = (lines * upl + 7) / 8 * g;  // this XCD's workgroups
   int local = local0;
-  if (local0 < r1) {
-    const int nx = (lines * upl + 7) / 8 * g, nr = r1 < nx ? r1 : nx;  // first round
+  if (rev) {
+    local = nx - 1 - local0;
+  } else if (local0 < r1) {
+    const int nr = r1 < nx ? r1 : nx;  // first round
     const int q = local0 / 32, j = local0 - 32 * q, len = nr - 32 * q < 32 ? nr - 32 * q : 32;
     if (q & 1) local = 32 * q + (len - 1 - j);
   }

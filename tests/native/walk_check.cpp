@@ -10,7 +10,9 @@
 //    round when the first-round snake is on (r1 > 0): then no later tile is
 //    longer than a first-round one, and within the first round, dealt over
 //    the XCD's 32 CUs breadth-first (its local index l to CU l % 32), every
-//    CU gets about the same K, where the plain order fails that bound.
+//    CU gets about the same K, where the plain order fails that bound;
+//  * the batched reversal (rev): bijections per problem, and the first
+//    round balanced over the CUs where the plain order pairs equal ranks.
 #include <cstdio>
 #include <cstdlib>
 #include <set>
@@ -194,6 +196,54 @@ static void lpt(int tm_, int tn_, int mode, int r1 = 0) {
         bound);
 }
 
+// A batched launch of nprob problems, each a tm_ x tn_ long-first grid of
+// P <= 32 workgroups per XCD: with the problems of odd rows of 32 reversed
+// (rev), every problem's walk is still a bijection, and over the first round
+// the CUs' K (breadth-first: problem z's XCD-local index l at CU (z P + l) %
+// 32) is balanced where the plain order pairs equal ranks.
+static void batched_rev(int tm_, int tn_, int mode, int nprob) {
+  const bool rows = mode <= 2;
+  const int lines = rows ? tm_ : tn_, per = rows ? tn_ : tm_;
+  const int g = lpt_unit(lines, per);
+  const long units = (long)lines * ((per + g - 1) / g);
+  const int wgs = (int)(8 * ((units + 7) / 8) * g), P = wgs / 8;
+  if (P > 32 || 32 % P) return;
+  auto kof = [&](int a, int b) {
+    const int line = rows ? a : b;
+    return (mode == 1 || mode == 3) ? line + 1 : lines - line;
+  };
+  std::vector<long> cu(8 * 32, 0), cu_plain(8 * 32, 0);
+  for (int z = 0; z < nprob; ++z) {
+    const bool rev = (z * P / 32) % 2 == 1;
+    std::set<std::pair<int, int>> seen;
+    for (int w = 0; w < wgs; ++w) {
+      int a, b, pa, pb;
+      const int x = w % 8, c = (z * P + w / 8) % 32;
+      if (map_tile_lpt(w, tm_, tn_, mode, g, pa, pb)) cu_plain[x * 32 + c] += kof(pa, pb);
+      if (!map_tile_lpt(w, tm_, tn_, mode, g, a, b, 0, rev)) continue;
+      CHECK(seen.insert({a, b}).second, "batched %dx%d m%d z %d duplicate", tm_, tn_, mode, z);
+      cu[x * 32 + c] += kof(a, b);
+    }
+    CHECK((int)seen.size() == tm_ * tn_, "batched %dx%d m%d z %d covers %zu", tm_, tn_, mode, z,
+          seen.size());
+  }
+  for (int x = 0; x < 8; ++x) {
+    long lo = 1L << 40, hi = -1, plo = 1L << 40, phi = -1;
+    for (int c = 0; c < 32; ++c) {
+      lo = cu[x * 32 + c] < lo ? cu[x * 32 + c] : lo;
+      hi = cu[x * 32 + c] > hi ? cu[x * 32 + c] : hi;
+      plo = cu_plain[x * 32 + c] < plo ? cu_plain[x * 32 + c] : plo;
+      phi = cu_plain[x * 32 + c] > phi ? cu_plain[x * 32 + c] : phi;
+    }
+    if (tm_ == 8 && tn_ == 8 && mode == 1 && nprob == 8) {
+      // the doubling's 512-level: CU K 16 vs 2 in plain order
+      CHECK(hi - lo <= 2 && phi - plo >= 8, "batched 8x8 XCD %d spread %ld, plain %ld", x,
+            hi - lo, phi - plo);
+      if (x == 0) std::printf("8 x (8x8) batched: CU K spread %ld (plain %ld)\n", hi - lo, phi - plo);
+    }
+  }
+}
+
 int main() {
   for (int a = 1; a <= 40; ++a)
     for (int b : {1, 2, 3, 7, 8, 9, 16, 33}) full_grid(a, b);
@@ -207,6 +257,9 @@ int main() {
     for (int b : {1, 3, 8, 16, 17, 64})
       for (int m = 1; m <= 4; ++m)
         for (int r1 : {0, 64, 128}) lpt(a, b, m, r1);  // 16 x 64, r1 128: the TRSM leaves
+  for (int t : {1, 2, 4, 8, 16})
+    for (int m = 1; m <= 4; ++m)
+      for (int np : {2, 8, 16, 64}) batched_rev(t, t, m, np);  // 8 x 8, 8 problems: doubling
   if (failures) {
     std::printf("%d failures\n", failures);
     return 1;
