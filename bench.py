@@ -740,17 +740,20 @@ def main():
               "multi-GPU runs are not part of it", file=sys.stderr)
         sys.exit(2)
     # The serving loop keeps 4 streams busy at once; HIP spreads a process's
-    # streams over GPU_MAX_HW_QUEUES hardware queues (4 by default), so with
-    # the library's per-thread stream and torch's already present two of
-    # them would share a queue and run in order (measured: p50 0.79 -> 0.18
-    # ms with 8).  Raised to 8 unless set higher; must happen before HIP
+    # streams over GPU_MAX_HW_QUEUES hardware queues (4 by default), so when
+    # streams created earlier hold queues (the library's per-thread stream,
+    # torch's, the host-staging upload and download streams of the
+    # host-pointer DGEMM) two server streams share a queue and run in order
+    # (measured: 4 queues p50 0.79 ms; 8 queues with the staging streams
+    # present p50 0.24 / p99 8.7 ms; 16 queues 0.13 / 0.96 ms, as the server
+    # alone).  Raised to 16 unless set higher; must happen before HIP
     # initialises.
     try:
         hwq = int(os.environ.get("GPU_MAX_HW_QUEUES", "4"))
     except ValueError:
         hwq = 4
-    if hwq < 8:
-        os.environ["GPU_MAX_HW_QUEUES"] = "8"
+    if hwq < 16:
+        os.environ["GPU_MAX_HW_QUEUES"] = "16"
     # The CPU baseline runs first: its concurrent workers are forked from a
     # process that has not touched the GPU yet.
     cpu = None
