@@ -7,6 +7,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 namespace gfx950 {
 
 
@@ -348,6 +350,36 @@ struct LdsSize {
   static constexpr int value = 4 * BK * Tile<TILE>::S;
 };
 
+// The MFMA steps of one staged slice (the K loop of gemm_tile below, as a
+// function for the loop that loads two slices ahead).
+template <typename T, int TILE, bool A_KC, bool B_NC, int BK, int NTH, int TIM, int TIN>
+__device__ inline void mfma_slice(typename Mfma<T>::acc_t (&acc)[TIM][TIN], const T *as,
+                                  const T *bs, int wm, int wn, int lr, int lk) {
+  typedef Mfma<T> MF;
+  constexpr int S = Tile<TILE>::S;
+  __builtin_amdgcn_s_setprio(1);
+#pragma unroll
+  for (int kk = 0; kk < BK / 4; ++kk) {
+    T af[TIM], bf[TIN];
+    const int kr = kk * 4 + lk, row = kr * S;
+    const int swa = A_KC ? MF::swz(kr) : 0, swb = B_NC ? 0 : MF::swz(kr);
+    const int hia = A_KC ? MF::swz(4 * kk) & ~15 : 0;
+    const int hib = B_NC ? 0 : MF::swz(4 * kk) & ~15;
+    const int rowa = row + (lr ^ (swa & 15)), rowb = row + (lr ^ (swb & 15));
+    constexpr bool SPLIT_B = A_KC || B_NC;
+#pragma unroll
+    for (int i = 0; i < TIM; ++i) af[i] = as[rowa + ((wm + i * 16) ^ hia)];
+#pragma unroll
+    for (int j = 0; j < TIN; ++j)
+      bf[j] = SPLIT_B ? bs[rowb + ((wn + j * 16) ^ hib)] : bs[row + ((wn + j * 16 + lr) ^ swb)];
+#pragma unroll
+    for (int i = 0; i < TIM; ++i)
+#pragma unroll
+      for (int j = 0; j < TIN; ++j) acc[i][j] = MF::step(af[i], bf[j], acc[i][j]);
+  }
+  __builtin_amdgcn_s_setprio(0);
+}
+
 // One TILE x TILE output tile (tm, tn) of the product described by p,
 // computed by the calling workgroup of NTH threads with `lds` (LdsSize
 // elements) as staging memory.  The caller synchronises before reusing lds.
@@ -412,6 +444,50 @@ __device__ inline void gemm_tile(const GemmArgs<T> &p, int tm, int tn, T *lds) {
     const int ktiles = kend > kbeg ? (kend + BK - 1) / BK - kt0 : 0;
     if (ktiles <= 0) continue;
 
+    // FP64 128-tiles with op(A) = A^T and column-major B (both operands
+    // K-contiguous) load their slices two ahead, into two register sets: a
+    // slice's loads then have two MFMA blocks to land instead of one.
+    // Measured on MI355X, DGEMM 8192 op(A) = A^T: 67.2-67.5 -> 68.4-68.6
+    // TFLOP/s, same bits (it spills 68 bytes per lane outside the MFMA
+    // block); in the other layouts the second set spilled inside the loop
+    // (no transpose: 65.5 -> 57.5), and B alone two ahead, or one set
+    // requested before the barrier instead of after it, ran 4-5 % slower.
+    constexpr bool AHEAD2 = sizeof(T) == 8 && TILE == 128 && NTH == 512 && A_KC && !B_NC && !SMALL;
+    if constexpr (AHEAD2) {
+      typename Ops::SA ra2[2];
+      typename Ops::SB rb2[2];
+      load_slices<T, TILE, A_KC, B_NC, BK, NTH, SMALL>(ra2[0], rb2[0], a, lda, b, ldb, m0, n0,
+                                                       kt0 * BK, p.m, p.n, p.k, va, vb, tid);
+      if (ktiles > 1)
+        load_slices<T, TILE, A_KC, B_NC, BK, NTH, SMALL>(ra2[1], rb2[1], a, lda, b, ldb, m0, n0,
+                                                         (kt0 + 1) * BK, p.m, p.n, p.k, va, vb, tid);
+      __syncthreads();  // previous segment's readers are done with buffer 0
+      ra2[0].template store<A_KC>(As[0], tid);
+      rb2[0].template store<!B_NC>(Bs[0], tid);
+      __syncthreads();
+      // iteration kt (set q = kt & 1): slice kt + 2 into set q (its slice kt
+      // is staged already), MFMAs on buffer q, slice kt + 1 from set q ^ 1
+      // into buffer q ^ 1
+      auto step = [&](auto Q, int kt) {
+        constexpr int q = decltype(Q)::value;
+        if (kt + 2 < ktiles)
+          load_slices<T, TILE, A_KC, B_NC, BK, NTH, SMALL>(ra2[q], rb2[q], a, lda, b, ldb, m0, n0,
+                                                           (kt0 + kt + 2) * BK, p.m, p.n, p.k, va,
+                                                           vb, tid);
+        mfma_slice<T, TILE, A_KC, B_NC, BK, NTH>(acc, As[q], Bs[q], wm, wn, lr, lk);
+        if (kt + 1 < ktiles) {
+          ra2[q ^ 1].template store<A_KC>(As[q ^ 1], tid);
+          rb2[q ^ 1].template store<!B_NC>(Bs[q ^ 1], tid);
+        }
+        __syncthreads();
+      };
+#pragma unroll 1
+      for (int kt = 0; kt < ktiles; kt += 2) {
+        step(std::integral_constant<int, 0>{}, kt);
+        if (kt + 1 < ktiles) step(std::integral_constant<int, 1>{}, kt + 1);
+      }
+      continue;
+    }
     load_slices<T, TILE, A_KC, B_NC, BK, NTH, SMALL>(ra, rb, a, lda, b, ldb, m0, n0, kt0 * BK, p.m, p.n,
                                      p.k, va, vb, tid);
     // previous segment's readers are done with buffer 0 (SMALL: the caller
