@@ -59,91 +59,171 @@ constexpr int BLK = NB * NB + NB;
 // computed in FP64 by one prep launch that also clears the flags.
 constexpr int PB = NB;  // block rows of the pipeline
 
-// Inverse of diagonal block p of A into inv + p * PB * PB (row-major), by
-// one wavefront; LT and rd are its LDS (LT[k][r] = A(base + r, base + k) in
-// the triangle, rd the reciprocal diagonal).
+// Inverse of diagonal block p of A into inv + p * PB * PB (row-major), by a
+// workgroup of kInvThreads (four waves), by blocked doubling in the lower
+// frame (an upper block is index-reversed, which makes it lower triangular):
+//  1. wave w inverts 16 x 16 diagonal sub-block w by 16-step substitution,
+//     one column per lane (the four 16-lane groups compute the same column);
+//  2. waves 0 and 1 complete the two 32 x 32 diagonal inverses,
+//     X21 = -inv(D1) * L21 * inv(D0), each two chains of four FP64 MFMAs;
+//  3. the four waves complete the 64 x 64 inverse's lower-left 32 x 32
+//     quadrant the same way, one 16 x 16 tile each.
+// The same recurrence the recursive solve uses for its large inverses
+// (inv_double below); a single 64-step substitution per column is a chain of
+// dependent LDS reads that keeps one wave busy for ~30 us.  W is the LDS:
+// the block's strict lower triangle in frame indices (W[r][c] = L(r, c)),
+// overwritten in place by the inverse; its upper-right quadrant parks the
+// phase-3 products that another wave needs.  rd is the reciprocal diagonal.
+constexpr int kInvThreads = 256;
+
 template <typename T, bool LOWER>
 __device__ __forceinline__ void inv_prep_block(int p, int m, const T *__restrict__ a, int64_t rs_a,
-                                      int64_t cs_a, int diag_mode, double *__restrict__ inv,
-                                      double (*LT)[PB + 1], double *rd) {
-  const int lane = threadIdx.x;
+                                               int64_t cs_a, int diag_mode, double *__restrict__ inv,
+                                               double (*W)[PB + 1], double *rd) {
+  typedef double d4 __attribute__((ext_vector_type(4)));
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int base = p * PB;
   const int nbr = min(PB, m - base);
+  // block index -> frame index
+  auto fr = [](int i) { return LOWER ? i : PB - 1 - i; };
   // lanes run along A's unit stride
   const bool cm = rs_a == 1;
-  // 16 loads in flight per batch, then the LDS stores.  A whole block is
-  // loaded without bounds checks (a float load under a per-element check
-  // gets its widening, and a wait, folded into the branch) and the triangle
-  // is selected at the store.
-#pragma unroll
-  for (int u0 = 0; u0 < PB; u0 += 16) {
+  {
+    // 16 elements per thread, all loads in flight before the LDS stores.  A
+    // whole block is loaded without bounds checks and the triangle is
+    // selected at the store; rows and columns past m stay 0 (identity rows).
     T v[16];
     if (nbr == PB) {
 #pragma unroll
       for (int d = 0; d < 16; ++d) {
-        const int u = u0 + d, r = cm ? lane : u, c = cm ? u : lane;
+        const int q = wave + 4 * d, r = cm ? lane : q, c = cm ? q : lane;
         v[d] = a[(int64_t)(base + r) * rs_a + (int64_t)(base + c) * cs_a];
       }
     } else {
 #pragma unroll
       for (int d = 0; d < 16; ++d) {
-        const int u = u0 + d, r = cm ? lane : u, c = cm ? u : lane;
+        const int q = wave + 4 * d, r = cm ? lane : q, c = cm ? q : lane;
         v[d] = (r < nbr && c < nbr) ? a[(int64_t)(base + r) * rs_a + (int64_t)(base + c) * cs_a]
                                     : T(0);
       }
     }
 #pragma unroll
     for (int d = 0; d < 16; ++d) {
-      const int u = u0 + d, r = cm ? lane : u, c = cm ? u : lane;
-      LT[c][r] = (LOWER ? r > c : r < c) ? (double)v[d] : 0.0;
+      const int q = wave + 4 * d, rf = fr(cm ? lane : q), cf = fr(cm ? q : lane);
+      W[rf][cf] = rf > cf ? (double)v[d] : 0.0;
     }
   }
-  {
+  if (tid < PB) {
     double v = 1.0;  // rows past m: identity
-    if (lane < nbr && diag_mode != 1) {
-      const int64_t g = base + lane;
+    if (tid < nbr && diag_mode != 1) {
+      const int64_t g = base + tid;
       const double d = (double)a[g * rs_a + g * cs_a];
       v = diag_mode == 2 ? d : 1.0 / d;
     }
-    rd[lane] = v;
+    rd[fr(tid)] = v;
   }
   __syncthreads();
-  // Lane c computes column c of the inverse, right-looking: once x_k is
-  // final it is pushed into every later row, so each step is up to 63
-  // independent FMAs on registers (the column lives in s[]).  Entries above
-  // (below) the diagonal of a lower (upper) inverse stay exactly 0; every
-  // lane reads the same LT[k][r] (an LDS broadcast).
-  const int c = lane;
-  double s[PB];
+  // 1. Right-looking substitution on sub-block w: once x_t is final it is
+  // pushed into every later row; the triangle is read as LDS broadcasts.
+  {
+    const int o = 16 * wave, cc = lane & 15;
+    double s[16];
 #pragma unroll
-  for (int r = 0; r < PB; ++r) s[r] = r == c ? 1.0 : 0.0;
+    for (int u = 0; u < 16; ++u) s[u] = u == cc ? 1.0 : 0.0;
 #pragma unroll
-  for (int t = 0; t < PB; ++t) {
-    const int k = LOWER ? t : PB - 1 - t;
-    const double xk = s[k] * rd[k];
-    s[k] = xk;
+    for (int t = 0; t < 16; ++t) {
+      const double xt = s[t] * rd[o + t];
+      s[t] = xt;
 #pragma unroll
-    for (int u = t + 1; u < PB; ++u) {
-      const int r = LOWER ? u : PB - 1 - u;
-      s[r] = fma(-LT[k][r], xk, s[r]);
+      for (int u = t + 1; u < 16; ++u) s[u] = fma(-W[o + u][o + t], xt, s[u]);
+    }
+    // the wave's own reads of its sub-block precede these stores
+    if (lane < 16) {
+#pragma unroll
+      for (int u = 0; u < 16; ++u) W[o + u][o + cc] = s[u];
     }
   }
-  double *o = inv + (int64_t)p * PB * PB;  // row-major
+  __syncthreads();
+  // FP64 MFMA 16x16x4: lane l supplies A(l & 15, k0 + (l >> 4)) and
+  // B(k0 + (l >> 4), l & 15); accumulator register r holds D((l >> 4) + 4 r,
+  // l & 15), which is also the B operand of a following product's k slice r.
+  const int lr = lane & 15, lk = lane >> 4;
+  const d4 z = {0.0, 0.0, 0.0, 0.0};
+  // 2. The two 32 x 32 diagonal blocks.
+  if (wave < 2) {
+    const int o = 32 * wave;
+    d4 t = z;
 #pragma unroll
-  for (int r = 0; r < PB; ++r) o[r * PB + c] = s[r];
+    for (int kk = 0; kk < 4; ++kk)
+      t = __builtin_amdgcn_mfma_f64_16x16x4f64(W[o + 16 + lr][o + 4 * kk + lk],
+                                               W[o + 4 * kk + lk][o + lr], t, 0, 0, 0);
+    d4 x = z;
+#pragma unroll
+    for (int kk = 0; kk < 4; ++kk)
+      x = __builtin_amdgcn_mfma_f64_16x16x4f64(W[o + 16 + lr][o + 16 + 4 * kk + lk], -t[kk], x,
+                                               0, 0, 0);
+    // overwrites L21, which only this wave read
+#pragma unroll
+    for (int r = 0; r < 4; ++r) W[o + 16 + lk + 4 * r][o + lr] = x[r];
+  }
+  __syncthreads();
+  // 3. Lower-left quadrant: T = C * inv(A), X21 = -inv(B) * T, where A and
+  // B are the 32 x 32 diagonal blocks and C the quadrant of L.  Wave w owns
+  // tile (i, n) = (w >> 1, w & 1); inv(A) tile (k, n) is 0 for k < n.
+  {
+    const int i = wave >> 1, n = wave & 1;
+    d4 t = z;
+    for (int kt = n; kt < 2; ++kt) {
+#pragma unroll
+      for (int kk = 0; kk < 4; ++kk)
+        t = __builtin_amdgcn_mfma_f64_16x16x4f64(W[32 + 16 * i + lr][16 * kt + 4 * kk + lk],
+                                                 W[16 * kt + 4 * kk + lk][16 * n + lr], t, 0, 0,
+                                                 0);
+    }
+    // wave (1, n) also needs T(0, n)
+    if (i == 0) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) W[lk + 4 * r][32 + 16 * n + lr] = t[r];
+    }
+    __syncthreads();  // also: every read of C precedes the stores below
+    d4 x = z;
+    if (i == 1) {
+#pragma unroll
+      for (int kk = 0; kk < 4; ++kk)
+        x = __builtin_amdgcn_mfma_f64_16x16x4f64(W[48 + lr][32 + 4 * kk + lk],
+                                                 -W[4 * kk + lk][32 + 16 * n + lr], x, 0, 0, 0);
+    }
+#pragma unroll
+    for (int kk = 0; kk < 4; ++kk)
+      x = __builtin_amdgcn_mfma_f64_16x16x4f64(W[32 + 16 * i + lr][32 + 16 * i + 4 * kk + lk],
+                                               -t[kk], x, 0, 0, 0);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) W[32 + 16 * i + lk + 4 * r][16 * n + lr] = x[r];
+  }
+  __syncthreads();
+  // Row-major inverse in block indices; the frame's upper triangle is 0.
+  double *o = inv + (int64_t)p * PB * PB;
+#pragma unroll
+  for (int d = 0; d < 16; ++d) {
+    const int r = wave + 4 * d, rf = fr(r), cf = fr(lane);
+    o[r * PB + lane] = rf >= cf ? W[rf][cf] : 0.0;
+  }
 }
 
 template <typename T, bool LOWER>
-__global__ __launch_bounds__(64) void trsm_inv_prep(int m, const T *__restrict__ a,
-                                                    int64_t rs_a, int64_t cs_a,
-                                                    int diag_mode,
-                                                    double *__restrict__ inv,
-                                                    int *__restrict__ flags, int nflags) {
-  __shared__ double LT[PB][PB + 1];
+__global__ __launch_bounds__(kInvThreads) void trsm_inv_prep(int m, const T *__restrict__ a,
+                                                             int64_t rs_a, int64_t cs_a,
+                                                             int diag_mode,
+                                                             double *__restrict__ inv,
+                                                             int *__restrict__ flags, int nflags) {
+  __shared__ double W[PB][PB + 1];
   __shared__ double rd[PB];
   const int p = blockIdx.x;
-  for (int i = p * PB + (int)threadIdx.x; i < nflags; i += gridDim.x * PB) flags[i] = 0;
-  inv_prep_block<T, LOWER>(p, m, a, rs_a, cs_a, diag_mode, inv, LT, rd);
+  for (int i = p * kInvThreads + (int)threadIdx.x; i < nflags; i += gridDim.x * kInvThreads)
+    flags[i] = 0;
+  inv_prep_block<T, LOWER>(p, m, a, rs_a, cs_a, diag_mode, inv, W, rd);
 }
 
 // Spin until *f == epoch (or the abort flag is up; after max_spins polls the
@@ -624,14 +704,14 @@ static_assert(kGroupTrsmLists == 8, "two inverse lists and three panel classes")
 
 // One launch per triangle orientation (both in one kernel spill).
 template <typename T, bool LOWER>
-__global__ __launch_bounds__(64) void grouped_inv_prep(const TrsmGroupProb<T> *__restrict__ probs,
-                                                       const TrsmGroupItem *__restrict__ items,
-                                                       double *inv) {
-  __shared__ double LT[PB][PB + 1];
+__global__ __launch_bounds__(kInvThreads) void grouped_inv_prep(const TrsmGroupProb<T> *__restrict__ probs,
+                                                                const TrsmGroupItem *__restrict__ items,
+                                                                double *inv) {
+  __shared__ double W[PB][PB + 1];
   __shared__ double rd[PB];
   const TrsmGroupItem w = items[blockIdx.x];
   const TrsmGroupProb<T> &d = probs[w.prob];
-  inv_prep_block<T, LOWER>(w.idx, d.m, d.a, d.rs_a, d.cs_a, d.unit, inv + d.inv, LT, rd);
+  inv_prep_block<T, LOWER>(w.idx, d.m, d.a, d.rs_a, d.cs_a, d.unit, inv + d.inv, W, rd);
 }
 
 // Two workgroups per CU where LDS allows (the 32-column lower body needs the
@@ -740,10 +820,10 @@ int prep(bool lower, int m, const T *a, int64_t rs_a, int64_t cs_a, int diag_mod
          double *inv, int *flags, int nflags, hipStream_t s) {
   const int blocks = (m + NB - 1) / NB;
   if (lower)
-    hipLaunchKernelGGL((trsm_inv_prep<T, true>), dim3(blocks), dim3(NB), 0, s, m, a, rs_a,
+    hipLaunchKernelGGL((trsm_inv_prep<T, true>), dim3(blocks), dim3(kInvThreads), 0, s, m, a, rs_a,
                        cs_a, diag_mode, inv, flags, nflags);
   else
-    hipLaunchKernelGGL((trsm_inv_prep<T, false>), dim3(blocks), dim3(NB), 0, s, m, a, rs_a,
+    hipLaunchKernelGGL((trsm_inv_prep<T, false>), dim3(blocks), dim3(kInvThreads), 0, s, m, a, rs_a,
                        cs_a, diag_mode, inv, flags, nflags);
   return hipGetLastError() == hipSuccess ? MUBLIS_OK : MUBLIS_EHIP;
 }
@@ -1144,10 +1224,10 @@ int gfx950_trsm_grouped(const TrsmGroupProb<T> *probs, const TrsmGroupItem *item
   it[0] = items;
   for (int c = 1; c < kGroupTrsmLists; ++c) it[c] = it[c - 1] + counts[c - 1];
   if (counts[0] > 0)
-    hipLaunchKernelGGL((grouped_inv_prep<T, true>), dim3(counts[0]), dim3(64), 0, s, probs,
+    hipLaunchKernelGGL((grouped_inv_prep<T, true>), dim3(counts[0]), dim3(kInvThreads), 0, s, probs,
                        it[0], inv);
   if (counts[1] > 0)
-    hipLaunchKernelGGL((grouped_inv_prep<T, false>), dim3(counts[1]), dim3(64), 0, su, probs,
+    hipLaunchKernelGGL((grouped_inv_prep<T, false>), dim3(counts[1]), dim3(kInvThreads), 0, su, probs,
                        it[1], inv);
   if (counts[2] > 0)
     hipLaunchKernelGGL((grouped_panels<T, true, kGroupTrsmWide, kGroupTrsmWideCols>),
