@@ -597,12 +597,32 @@ int launch_items(const GroupProb *probs, const GroupItem *items, const int *ncls
 // a batch that fills the GPU -- at least kTuneTiles tiles at the 32-tile
 // shape -- after a warm-up replay, as the best of `reps` replays: throughput
 // inside a large batch, which is what the grouped workloads run, not the
-// latency of a few problems.  A choice within kTieMargin of the heuristic's
-// (for schedules, variant 0) takes the heuristic's, so repeated tuning
-// settles on the same table wherever the timings cannot tell the choices
-// apart.
+// latency of a few problems.  Every candidate within kTieMargin of the
+// fastest is a near-tie; among near-ties the heuristic's choice (for
+// schedules, variant 0) wins, else the lowest-numbered one, so repeated
+// tuning settles on the same table wherever the timings cannot tell the
+// choices apart (run-to-run noise of a batch is a few percent).
 constexpr int kTuneTiles = 6144;
 constexpr double kTieMargin = 0.03;
+
+// Index chosen among n timings: the preferred candidate if it is within the
+// margin of the fastest, else the lowest-numbered candidate that is.  *tie
+// is set when more than one candidate is within the margin.
+inline int tune_pick(const double *ms, int n, int preferred, bool *tie) {
+  int best = 0;
+  for (int i = 1; i < n; ++i)
+    if (ms[i] < ms[best]) best = i;
+  const double lim = ms[best] * (1 + kTieMargin);
+  int pick = -1, within = 0;
+  for (int i = 0; i < n; ++i)
+    if (ms[i] <= lim) {
+      ++within;
+      if (pick < 0) pick = i;
+    }
+  if (ms[preferred] <= lim) pick = preferred;
+  *tie = within > 1;
+  return pick;
+}
 
 struct Tuner {
   float *buf = nullptr;
@@ -702,24 +722,30 @@ struct Tuner {
     }
     return place(ops);
   }
-  // best of `reps` replays of a plan of ops created with flags
-  int time(const std::vector<mublis_sbatch_op_t> &ops, int flags, double *best) {
-    mublis_grouped_plan_t plan = nullptr;
-    int st = mublis_sgrouped_plan_create(ops.data(), (int)ops.size(), flags, &plan);
-    if (st) return st;
-    st = mublis_grouped_plan_execute(plan, s);  // warm-up
-    *best = 1e30;
-    for (int r = 0; r < reps && !st; ++r) {
-      float ms = 0;
-      if (hipEventRecord(e0, s) != hipSuccess) st = MUBLIS_EHIP;
-      if (!st) st = mublis_grouped_plan_execute(plan, s);
-      if (!st && (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
-                  hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
-        st = MUBLIS_EHIP;
-      if (!st) *best = std::min(*best, (double)ms);
+  // Best of `reps` replays of each of n plans of the same ops, created with
+  // flags[i], into ms[i].  The replays go round-robin over the candidates
+  // (after one warm-up replay each), so a drift of the clocks over the run
+  // does not favour the candidates timed first or last.
+  int time(const std::vector<mublis_sbatch_op_t> &ops, const int *flags, int n, double *ms) {
+    std::vector<mublis_grouped_plan_t> plans(n, nullptr);
+    int st = MUBLIS_OK;
+    for (int i = 0; i < n && !st; ++i) {
+      st = mublis_sgrouped_plan_create(ops.data(), (int)ops.size(), flags[i], &plans[i]);
+      if (!st) st = mublis_grouped_plan_execute(plans[i], s);  // warm-up
+      ms[i] = 1e30;
     }
+    for (int r = 0; r < reps && !st; ++r)
+      for (int i = 0; i < n && !st; ++i) {
+        float t = 0;
+        if (hipEventRecord(e0, s) != hipSuccess) st = MUBLIS_EHIP;
+        if (!st) st = mublis_grouped_plan_execute(plans[i], s);
+        if (!st && (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                    hipEventElapsedTime(&t, e0, e1) != hipSuccess))
+          st = MUBLIS_EHIP;
+        if (!st) ms[i] = std::min(ms[i], (double)t);
+      }
     if (hipStreamSynchronize(s) != hipSuccess && !st) st = MUBLIS_EHIP;
-    mublis_grouped_plan_destroy(plan);
+    for (auto p : plans) mublis_grouped_plan_destroy(p);
     return st;
   }
 };
@@ -885,8 +911,8 @@ int mublis_grouped_autotune(const char *path, int lo, int hi, int per_bucket, in
   fprintf(f, "# sched class variant | gemm lm ln lk tile | syrk ln lk tile; buckets are "
              "floor(log2) of the sizes\n");
   fprintf(f, "# each candidate: a batch of >= %d tiles (>= %d problems), best of %d replays; "
-             "a choice within %.0f%% of the\n# heuristic's (schedule variant 0) takes the "
-             "heuristic's and is marked ~\n",
+             "candidates within %.0f%% of the\n# fastest are near-ties (marked ~): the "
+             "heuristic's (schedule variant 0) if among them, else the first\n",
           kTuneTiles, per_bucket, reps, 100 * kTieMargin);
   // 1. the schedule of each tile class, on a batch of problems of that
   //    class's typical sizes with the tile shape forced
@@ -896,14 +922,13 @@ int mublis_grouped_autotune(const char *path, int lo, int hi, int per_bucket, in
     std::vector<mublis_sbatch_op_t> ops;
     st = t.batch_range(ops, kSchedRange[c][0], kSchedRange[c][1], c, per_bucket);
     double ms[kSchedVariants];
-    for (int v = 0; v < kSchedVariants && !st; ++v)
-      st = t.time(ops, MUBLIS_GROUPED_FORCE_TILE(c) | MUBLIS_GROUPED_FORCE_SCHED(v), &ms[v]);
+    int fl[kSchedVariants];
+    for (int v = 0; v < kSchedVariants; ++v)
+      fl[v] = MUBLIS_GROUPED_FORCE_TILE(c) | MUBLIS_GROUPED_FORCE_SCHED(v);
+    if (!st) st = t.time(ops, fl, kSchedVariants, ms);
     if (st) break;
-    int best = 0;
-    for (int v = 1; v < kSchedVariants; ++v)
-      if (ms[v] < ms[best]) best = v;
-    const bool tie = best != 0 && ms[0] <= ms[best] * (1 + kTieMargin);
-    sched[c] = tie ? 0 : best;
+    bool tie = false;
+    sched[c] = tune_pick(ms, kSchedVariants, 0, &tie);
     fprintf(f, "sched %d %d%s\n", c, sched[c], tie ? " ~" : "");
   }
   // 2. the tile shape of each GEMM bucket, then of each SYRK bucket, with
@@ -920,16 +945,14 @@ int mublis_grouped_autotune(const char *path, int lo, int hi, int per_bucket, in
           std::vector<mublis_sbatch_op_t> ops;
           st = t.batch_bucket(ops, sym != 0, lm, ln, lk, per_bucket);
           double ms[3];
-          for (int tile = 0; tile < 3 && !st; ++tile)
-            st = t.time(ops, MUBLIS_GROUPED_FORCE_TILE(tile), &ms[tile]);
+          const int fl[3] = {MUBLIS_GROUPED_FORCE_TILE(0), MUBLIS_GROUPED_FORCE_TILE(1),
+                             MUBLIS_GROUPED_FORCE_TILE(2)};
+          if (!st) st = t.time(ops, fl, 3, ms);
           if (st) break;
           const int h = sym ? heuristic_tile(1 << ln, 1 << ln, 1 << lk)
                             : heuristic_tile(1 << lm, 1 << ln, 1 << lk);
-          int best = 0;
-          for (int tile = 1; tile < 3; ++tile)
-            if (ms[tile] < ms[best]) best = tile;
-          const bool tie = best != h && ms[h] <= ms[best] * (1 + kTieMargin);
-          const int pick = tie ? h : best;
+          bool tie = false;
+          const int pick = tune_pick(ms, 3, h, &tie);
           if (sym) fprintf(f, "syrk %d %d %d%s\n", ln, lk, pick, tie ? " ~" : "");
           else fprintf(f, "gemm %d %d %d %d%s\n", lm, ln, lk, pick, tie ? " ~" : "");
         }
