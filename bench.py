@@ -566,42 +566,26 @@ def fifo_latencies(arrivals, service):
     return out
 
 
-def serve_run(S, srv, reqs, arrivals, clients=8):
-    """Submit reqs from `clients` threads at the given arrival offsets
-    (seconds), wait for all; returns (stats, max arrival lag in ms)."""
-    import threading
+def serve_run(S, srv, reqs, arrivals):
+    """Submit reqs at the given arrival offsets (seconds) and wait for all;
+    returns (stats, largest submission delay in ms).  The arrivals are
+    issued by the library's native replay loop (mublis_server_replay):
+    Python client threads delayed arrivals by 5-30 ms on some runs (the
+    interpreter's threads not scheduled on time), which queued requests
+    and moved p99 from under 1 ms to 7-20 ms with the same library."""
     srv.reset_stats()
-    tickets, lag = [None] * len(reqs), [0.0] * len(reqs)
-    kw = [{k: v for k, v in r.items() if k not in ("op", "flop")} for r in reqs]
-    start = time.perf_counter() + 0.02
-
-    def client(c):
-        for i in range(c, len(reqs), clients):
-            t_arr = start + arrivals[i]
-            d = t_arr - time.perf_counter()
-            if d > 0:
-                time.sleep(d)
-            tickets[i] = srv.submit(reqs[i]["op"], **kw[i])
-            lag[i] = time.perf_counter() - t_arr
-    old = sys.getswitchinterval()
-    sys.setswitchinterval(1e-4)
-    try:
-        ts = [threading.Thread(target=client, args=(c,)) for c in range(clients)]
-        for t in ts:
-            t.start()
-        for t in ts:
-            t.join()
-        for t in tickets:
-            st, _ = srv.wait(t)
-            assert st == 0, st
-    finally:
-        sys.setswitchinterval(old)
-    return srv.stats(), 1e3 * max(lag)
+    cr = [S.Server.request(r["op"], **{k: v for k, v in r.items() if k not in ("op", "flop")})
+          for r in reqs]
+    tickets, lag = srv.replay(cr, arrivals)
+    for t in tickets:
+        st, _ = srv.wait(t)
+        assert st == 0, st
+    return srv.stats(), lag
 
 
 def run_serving(args, torch, M, lib, extra):
-    """Serving workload: Poisson arrivals at a fixed rate from 8 client
-    threads, 60% SGEMM / 20% SSYRK+SSYR2K / 20% DTRSM (sizes in
+    """Serving workload: Poisson arrivals at a fixed rate (replayed by the
+    library's native loop), 60% SGEMM / 20% SSYRK+SSYR2K / 20% DTRSM (sizes in
     mublis_amd.workloads.serving_requests), operands device-resident by
     handle in a 64 GB budget; against the same requests issued as serial
     synchronous cblas_* calls."""

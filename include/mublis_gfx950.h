@@ -455,6 +455,18 @@ int mublis_matrix_destroy(mublis_server_t srv, mublis_matrix_t h);
  * requests (shapes beyond the matrices, wrong precision, C aliasing an
  * input) are rejected here with MUBLIS_EINVAL. */
 int mublis_submit(mublis_server_t srv, const mublis_request_t *req, uint64_t *ticket);
+/* Replay a request trace from the calling thread: request i is submitted
+ * (as by mublis_submit, ticket into tickets[i]) arrival_s[i] seconds after
+ * the call, sleeping between arrivals; returns once the last is submitted.
+ * Stops at the first rejected request and returns its status; *submitted
+ * counts the requests queued (their tickets are valid), *max_lag_ms the
+ * largest delay of a submission behind its arrival time.  No counterpart in
+ * the reference: a load generator for capacity and latency measurement whose
+ * arrivals do not depend on the caller's own threads being scheduled on
+ * time (an interpreter's client threads delayed arrivals by up to 30 ms). */
+int mublis_server_replay(mublis_server_t srv, const mublis_request_t *reqs,
+                         const double *arrival_s, int n, uint64_t *tickets, int *submitted,
+                         double *max_lag_ms);
 /* Block until the request completes; returns its status and (optionally)
  * its latency from submission to completion.  Each ticket is waited once. */
 int mublis_wait(mublis_server_t srv, uint64_t ticket, double *latency_ms);

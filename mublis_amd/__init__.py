@@ -145,6 +145,7 @@ def _declare(lib):
                        ("mublis_matrix_read", [_VP, _VP, _VP, _I64]),
                        ("mublis_matrix_destroy", [_VP, _VP]),
                        ("mublis_submit", [_VP, _VP, _VP]),
+                       ("mublis_server_replay", [_VP, _VP, _VP, _I, _VP, _VP, _VP]),
                        ("mublis_wait", [_VP, ctypes.c_uint64, _VP]),
                        ("mublis_server_stats", [_VP, _VP]),
                        ("mublis_server_reset_stats", [_VP])):

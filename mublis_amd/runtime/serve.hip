@@ -725,6 +725,33 @@ int mublis_submit(mublis_server_t s, const mublis_request_t *req, uint64_t *tick
   return MUBLIS_OK;
 }
 
+int mublis_server_replay(mublis_server_t s, const mublis_request_t *reqs,
+                         const double *arrival_s, int n, uint64_t *tickets, int *submitted,
+                         double *max_lag_ms) {
+  if (submitted) *submitted = 0;
+  if (max_lag_ms) *max_lag_ms = 0;
+  if (!s || n < 0 || (n > 0 && (!reqs || !arrival_s || !tickets))) return MUBLIS_EINVAL;
+  using SClock = std::chrono::steady_clock;
+  const SClock::time_point start = SClock::now();
+  double lag = 0;
+  for (int i = 0; i < n; ++i) {
+    const SClock::time_point t =
+        start + std::chrono::duration_cast<SClock::duration>(
+                    std::chrono::duration<double>(std::max(0.0, arrival_s[i])));
+    // sleep to within 50 us of the arrival (the scheduler's wake-up
+    // granularity), then yield until it
+    const SClock::time_point near = t - std::chrono::microseconds(50);
+    if (SClock::now() < near) std::this_thread::sleep_until(near);
+    while (SClock::now() < t) std::this_thread::yield();
+    const int st = mublis_submit(s, &reqs[i], &tickets[i]);
+    if (st) return st;
+    if (submitted) *submitted = i + 1;
+    lag = std::max(lag, std::chrono::duration<double, std::milli>(SClock::now() - t).count());
+  }
+  if (max_lag_ms) *max_lag_ms = lag;
+  return MUBLIS_OK;
+}
+
 int mublis_wait(mublis_server_t s, uint64_t ticket, double *latency_ms) {
   if (!s) return MUBLIS_EINVAL;
   std::unique_lock<std::mutex> lk(s->mu);

@@ -97,13 +97,22 @@ class Server:
             m.write(data)
         return m
 
-    def submit(self, op, m=0, n=0, k=0, alpha=1.0, beta=0.0, a=None, b=None, c=None,
-               trans_a=NO_TRANS, trans_b=NO_TRANS, side=LEFT, uplo=LOWER, diag=NON_UNIT):
+    @staticmethod
+    def request(op, m=0, n=0, k=0, alpha=1.0, beta=0.0, a=None, b=None, c=None,
+                trans_a=NO_TRANS, trans_b=NO_TRANS, side=LEFT, uplo=LOWER, diag=NON_UNIT):
+        """The mublis_request_t of a request, for submit_request."""
+        return Request(op, trans_a, trans_b, side, uplo, diag, m, n, k, alpha, beta,
+                       a.handle if a is not None else None,
+                       b.handle if b is not None else None,
+                       c.handle if c is not None else None)
+
+    def submit(self, op, **kw):
         """Queue a request; returns its ticket.  Raises on a rejected request."""
-        req = Request(op, trans_a, trans_b, side, uplo, diag, m, n, k, alpha, beta,
-                      a.handle if a is not None else None,
-                      b.handle if b is not None else None,
-                      c.handle if c is not None else None)
+        return self.submit_request(self.request(op, **kw))
+
+    def submit_request(self, req):
+        """Queue a request built by request(); returns its ticket.  Raises on a
+        rejected request."""
         t = ctypes.c_uint64()
         check(self.lib.mublis_submit(self.handle, ctypes.byref(req), ctypes.byref(t)),
               "mublis_submit")
@@ -119,6 +128,26 @@ class Server:
         t = ctypes.c_uint64()
         st = self.lib.mublis_submit(self.handle, ctypes.byref(req), ctypes.byref(t))
         return st, t.value
+
+    def replay(self, requests, arrivals):
+        """Submit requests (built by request()) at arrival offsets in seconds
+        from a native loop (mublis_server_replay); returns (tickets, largest
+        submission delay in ms).  Raises on a rejected request, after waiting
+        for the ones already queued."""
+        n = len(requests)
+        arr = (Request * n)(*requests)
+        at = (ctypes.c_double * n)(*[float(x) for x in arrivals])
+        tk = (ctypes.c_uint64 * n)()
+        done, lag = ctypes.c_int(), ctypes.c_double()
+        st = self.lib.mublis_server_replay(self.handle, ctypes.cast(arr, ctypes.c_void_p),
+                                           ctypes.cast(at, ctypes.c_void_p), n,
+                                           ctypes.cast(tk, ctypes.c_void_p), ctypes.byref(done),
+                                           ctypes.byref(lag))
+        if st:
+            for i in range(done.value):
+                self.wait(tk[i])
+            check(st, "mublis_server_replay")
+        return list(tk), lag.value
 
     def wait(self, ticket):
         """Block until the request completes; returns (status, latency_ms)."""

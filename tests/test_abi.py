@@ -72,6 +72,20 @@ def test_empty_problems_return_without_a_device(lib):
     assert lib.mublis_strsm(0, 1, 0, 0, 5, 0, 1.0, None, 1, 1, None, 1, 1) == 0
 
 
+def test_replay_rejects_bad_arguments_without_a_device(lib):
+    """mublis_server_replay checks its arguments before touching a server:
+    no server, a negative count, or missing arrays return MUBLIS_EINVAL and
+    report nothing submitted."""
+    from mublis_amd import EINVAL
+    done, lag = ctypes.c_int(7), ctypes.c_double(3.0)
+    assert lib.mublis_server_replay(None, None, None, 0, None, ctypes.byref(done),
+                                    ctypes.byref(lag)) == EINVAL
+    assert done.value == 0 and lag.value == 0.0
+    fake = ctypes.c_void_p(1)  # never dereferenced: the count is checked first
+    assert lib.mublis_server_replay(fake, None, None, -1, None, None, None) == EINVAL
+    assert lib.mublis_server_replay(fake, None, None, 2, None, None, None) == EINVAL
+
+
 @pytest.mark.skipif(os.path.exists("/dev/kfd"), reason="needs a host without a GPU")
 def test_no_device_reports_backend_failure_through_xerbla():
     """Without a gfx950 device every call fails with status 1 and CBLAS

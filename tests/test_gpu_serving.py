@@ -339,6 +339,50 @@ def test_waiting_requests_share_one_launch(lib):
     assert st["grouped_launches"] <= 4, st
 
 
+def test_replay_equals_alone(lib):
+    """Requests replayed at Poisson arrival offsets by the native replay loop
+    (mublis_server_replay) give the bits of the same requests run alone; a
+    rejected request stops the replay with its status after the requests
+    before it were queued."""
+    specs = make_mix(seed=13, count=40)
+    rng = np.random.default_rng(5)
+    arrivals = np.cumsum(rng.exponential(2e-4, len(specs)))
+    with S.Server(hbm_budget=2 << 30) as srv:
+        mats = [place(srv, s) for s in specs]
+        alone = run_alone(srv, specs, mats)
+        for s, ms in zip(specs, mats):
+            reset_inputs(s, ms)
+        reqs = [S.Server.request(s["op"], a=ms.get("a"), b=ms.get("b"), c=ms.get("c"),
+                                 **s["args"]) for s, ms in zip(specs, mats)]
+        tickets, lag = srv.replay(reqs, arrivals)
+        assert len(tickets) == len(specs) and lag >= 0
+        for t in tickets:
+            assert srv.wait(t)[0] == 0
+        for i, (s, ms, x) in enumerate(zip(specs, mats, alone)):
+            y = ms[s["out"]].read()
+            assert np.array_equal(x, y, equal_nan=True), (i, S.OP_NAMES[s["op"]], s["args"])
+        # the third request is invalid (k beyond A): two are queued, then EINVAL
+        a = srv.matrix(S.F32, 64, 32)
+        c = [srv.matrix(S.F32, 64, 64) for _ in range(3)]
+        bad = [S.Server.request(S.SSYRK, n=64, k=32, a=a, c=c[0], uplo=S.LOWER),
+               S.Server.request(S.SSYRK, n=64, k=32, a=a, c=c[1], uplo=S.LOWER),
+               S.Server.request(S.SGEMM, m=64, n=64, k=48, a=a, b=a, c=c[2])]
+        n = len(bad)
+        arr = (S.Request * n)(*bad)
+        at = (ctypes.c_double * n)(0.0, 0.0, 0.0)
+        tk = (ctypes.c_uint64 * n)()
+        done, lg = ctypes.c_int(), ctypes.c_double()
+        st = lib.mublis_server_replay(srv.handle, ctypes.cast(arr, ctypes.c_void_p),
+                                      ctypes.cast(at, ctypes.c_void_p), n,
+                                      ctypes.cast(tk, ctypes.c_void_p), ctypes.byref(done),
+                                      ctypes.byref(lg))
+        assert st == M.EINVAL and done.value == 2
+        for i in range(2):
+            assert srv.wait(tk[i])[0] == 0
+        for x in [a] + c:
+            x.destroy()
+
+
 def test_budget_admission_and_rejections(lib):
     with S.Server(hbm_budget=64 << 20) as srv:
         big = 1536  # 9 MiB per FP32 matrix; 56 MiB are left for matrices
