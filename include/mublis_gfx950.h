@@ -458,6 +458,7 @@ int mublis_submit(mublis_server_t srv, const mublis_request_t *req, uint64_t *ti
 /* Replay a request trace from the calling thread: request i is submitted
  * (as by mublis_submit, ticket into tickets[i]) arrival_s[i] seconds after
  * the call, sleeping between arrivals; returns once the last is submitted.
+ * Offsets must lie in [0, 86400] (else MUBLIS_EINVAL before any submission).
  * Stops at the first rejected request and returns its status; *submitted
  * counts the requests queued (their tickets are valid), *max_lag_ms the
  * largest delay of a submission behind its arrival time.  No counterpart in

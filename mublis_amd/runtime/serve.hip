@@ -731,13 +731,17 @@ int mublis_server_replay(mublis_server_t s, const mublis_request_t *reqs,
   if (submitted) *submitted = 0;
   if (max_lag_ms) *max_lag_ms = 0;
   if (!s || n < 0 || (n > 0 && (!reqs || !arrival_s || !tickets))) return MUBLIS_EINVAL;
+  // offsets must be finite, from 0 to a day (the clock arithmetic below
+  // would overflow on larger ones)
+  for (int i = 0; i < n; ++i)
+    if (!(arrival_s[i] >= 0.0 && arrival_s[i] <= 86400.0)) return MUBLIS_EINVAL;
   using SClock = std::chrono::steady_clock;
   const SClock::time_point start = SClock::now();
   double lag = 0;
   for (int i = 0; i < n; ++i) {
     const SClock::time_point t =
         start + std::chrono::duration_cast<SClock::duration>(
-                    std::chrono::duration<double>(std::max(0.0, arrival_s[i])));
+                    std::chrono::duration<double>(arrival_s[i]));
     // sleep to within 50 us of the arrival (the scheduler's wake-up
     // granularity), then yield until it
     const SClock::time_point near = t - std::chrono::microseconds(50);

@@ -84,6 +84,12 @@ def test_replay_rejects_bad_arguments_without_a_device(lib):
     fake = ctypes.c_void_p(1)  # never dereferenced: the count is checked first
     assert lib.mublis_server_replay(fake, None, None, -1, None, None, None) == EINVAL
     assert lib.mublis_server_replay(fake, None, None, 2, None, None, None) == EINVAL
+    # offsets outside [0, 86400] s are refused before the server is touched
+    for bad in (-1.0, float("nan"), float("inf"), 1e9):
+        at = (ctypes.c_double * 1)(bad)
+        tk = (ctypes.c_uint64 * 1)()
+        req = ctypes.create_string_buffer(256)
+        assert lib.mublis_server_replay(fake, req, at, 1, tk, None, None) == EINVAL, bad
 
 
 @pytest.mark.skipif(os.path.exists("/dev/kfd"), reason="needs a host without a GPU")
