@@ -53,7 +53,7 @@ using namespace gfx950;
 // depth never changes the k order of the MFMAs.
 template <typename T, int TILE>
 bool use_shallow(const GemmArgs<T> &p, int64_t wgs) {
-  if (p.lpt) return false;
+  if (p.lpt || TILE == 32) return false;
   if (TILE == 64) return p.tri != 3 && wgs > 1024;
   return sizeof(T) == 4 && wgs > 512;
 }
@@ -162,7 +162,7 @@ hipError_t launch(GemmArgs<T> p, bool akc, bool bnc, hipStream_t s, int nbatch =
     p.lpt_r1 = nbatch == 1 ? 32 * KernelShape<T, TILE, false>::WG_PER_CU : 0;
   }
   const dim3 grid(wgs, nbatch);
-  if constexpr (TILE == 64 || sizeof(T) == 4) {
+  if constexpr ((TILE == 64 || sizeof(T) == 4) && TILE != 32) {
     if (use_shallow<T, TILE>(p, (int64_t)wgs * nbatch)) {
       launch_layout<T, TILE, true>(p, akc, bnc, grid, s);
       return hipGetLastError();
@@ -278,7 +278,19 @@ int gemm_dev(int m, int n, int k, T alpha, const T *a, int64_t rs_a, int64_t cs_
       p.tri = 3;
       if (e == hipSuccess) e = launch<T, 64>(p, akc, bnc, s);
     } else {
-      e = n128 >= need ? launch<T, 128>(p, akc, bnc, s) : launch<T, 64>(p, akc, bnc, s);
+      // 64-tile grids of at most 1.5 workgroups per CU take 32-tiles (4 per
+      // CU, one to 1.5 rounds), the same MFMA steps per element.  Measured
+      // on MI355X (TFLOP/s, 64- -> 32-tiles): SGEMM 1024 x 1024 x 4096 50.0
+      // -> 56.4, 768 x 1500 x 2000 33.6 -> 40.4, 3000 x 500 x 3000 46.0 ->
+      // 54.3; DGEMM the same shapes 33.6 -> 37.6, 23.2 -> 29.7, 31.2 ->
+      // 36.0; STRSM 8192 x 1024 1.21 -> 1.08 ms (its 1024-row leaves and
+      // updates), 4096 x 512 0.47 -> 0.36 ms, DTRSM 8192 x 1024 1.92 -> 1.71
+      // ms.  With 400 or more 64-tiles (1280 x 1280 x 4096, 500 x 4000 x
+      // 2500: 1.6-2 rounds of 32-tiles) they were 4-11 % slower.
+      const int64_t t64 = (int64_t)((m + 63) / 64) * ((n + 63) / 64);
+      e = n128 >= need ? launch<T, 128>(p, akc, bnc, s)
+          : (!p.tri && t64 <= 384) ? launch<T, 32>(p, akc, bnc, s)
+                                   : launch<T, 64>(p, akc, bnc, s);
     }
     if (e != hipSuccess) st = MUBLIS_EHIP;
     if (st == MUBLIS_OK && a2 && !same2) {
