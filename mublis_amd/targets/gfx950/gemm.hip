@@ -286,11 +286,18 @@ int gemm_dev(int m, int n, int k, T alpha, const T *a, int64_t rs_a, int64_t cs_
       // 36.0; STRSM 8192 x 1024 1.21 -> 1.08 ms (its 1024-row leaves and
       // updates), 4096 x 512 0.47 -> 0.36 ms, DTRSM 8192 x 1024 1.92 -> 1.71
       // ms.  With 400 or more 64-tiles (1280 x 1280 x 4096, 500 x 4000 x
-      // 2500: 1.6-2 rounds of 32-tiles) they were 4-11 % slower.
+      // 2500: 1.6-2 rounds of 32-tiles) they were 4-11 % slower.  The same
+      // for a triangle of at most 384 64-tiles (SYRK/SYR2K n up to 1700),
+      // but FP32 triangles only from K = 1025: DSYRK 512 / 2048 3.9 -> 5.3,
+      // 1024 / 4096 17.6 -> 21.5, 1536 / 1536 23.7 -> 28.1; SSYRK 1024 /
+      // 4096 27.8 -> 32.5, 1536 / 1536 32.6 -> 38.8, but 1024 / 1024 17.3 ->
+      // 15.9 and 1700 / 512 23.8 -> 21.5.
       const int64_t t64 = (int64_t)((m + 63) / 64) * ((n + 63) / 64);
+      const int64_t tri64 = (int64_t)((m + 63) / 64) * ((m + 63) / 64 + 1) / 2;
+      const bool small32 = p.tri ? tri64 <= 384 && (sizeof(T) == 8 || k > 1024) : t64 <= 384;
       e = n128 >= need ? launch<T, 128>(p, akc, bnc, s)
-          : (!p.tri && t64 <= 384) ? launch<T, 32>(p, akc, bnc, s)
-                                   : launch<T, 64>(p, akc, bnc, s);
+          : small32    ? launch<T, 32>(p, akc, bnc, s)
+                       : launch<T, 64>(p, akc, bnc, s);
     }
     if (e != hipSuccess) st = MUBLIS_EHIP;
     if (st == MUBLIS_OK && a2 && !same2) {
