@@ -416,6 +416,17 @@ __device__ inline void gemm_tile(const GemmArgs<T> &p, int tm, int tn, T *lds) {
   const int lane = tid & 63, wave = tid >> 6;
   const int wm = (wave & 1) * WTM, wn = (wave >> 1) * WTN;
   const int lr = lane & 15, lk = lane >> 4;
+  // SMALL: a wave whose whole region of the tile stores nothing (past the
+  // edge of C, or on the unstored side of the ji triangle) skips its
+  // fragment reads and MFMAs; it still loads and stages its share of every
+  // slice.  The elements that are stored see the same MFMA steps either way.
+  // Measured on MI355X, the batched replay of 4096 SGEMM + 1024 SSYRK: 1.093
+  // -> 1.072 ms, the same bits; one-problem launches of SYRK and ragged GEMMs
+  // showed no gain, so they keep the unconditional loop.
+  const int wr0 = m0 + __builtin_amdgcn_readfirstlane(wm);
+  const int wc0 = n0 + __builtin_amdgcn_readfirstlane(wn);
+  const bool live = !SMALL || (wr0 < p.m && wc0 < p.n && !(p.ji == 1 && wc0 > wr0 + WTM - 1) &&
+                               !(p.ji == 2 && wr0 > wc0 + WTN - 1));
 
   acc_t acc[TIM][TIN];
 #pragma unroll
@@ -511,33 +522,35 @@ __device__ inline void gemm_tile(const GemmArgs<T> &p, int tm, int tn, T *lds) {
       // others run their loads and LDS stores (measured: FP64 NN +2.6 %, FP32
       // +1.3 %; FP64 with op(A) = A^T and column-major B -0.5 % at 2 waves
       // per SIMD, +3.1 % at 4).
-      __builtin_amdgcn_s_setprio(1);
+      if (live) {
+        __builtin_amdgcn_s_setprio(1);
 #pragma unroll
-      for (int kk = 0; kk < BK / 4; ++kk) {
-        T af[TIM], bf[TIN];
-        const int kr = kk * 4 + lk, row = kr * S;
-        const int swa = A_KC ? MF::swz(kr) : 0, swb = B_NC ? 0 : MF::swz(kr);
-        // (x + lr) ^ s == (x ^ (s & ~15)) + (lr ^ (s & 15)) for x a multiple
-        // of 16 and lr < 16; s & ~15 depends on kk alone (lk < 4 sets only
-        // bits 0-1 of kr), so each tile's part is an immediate offset.  This
-        // form saves address registers (it took the FP64 8-wave kernel with
-        // op(A) = A^T out of spilling).  For B when only B is swizzled
-        // (column-major A and B) the plain form measured faster.
-        const int hia = A_KC ? MF::swz(4 * kk) & ~15 : 0;
-        const int hib = B_NC ? 0 : MF::swz(4 * kk) & ~15;
-        const int rowa = row + (lr ^ (swa & 15)), rowb = row + (lr ^ (swb & 15));
-        constexpr bool SPLIT_B = A_KC || B_NC;
+        for (int kk = 0; kk < BK / 4; ++kk) {
+          T af[TIM], bf[TIN];
+          const int kr = kk * 4 + lk, row = kr * S;
+          const int swa = A_KC ? MF::swz(kr) : 0, swb = B_NC ? 0 : MF::swz(kr);
+          // (x + lr) ^ s == (x ^ (s & ~15)) + (lr ^ (s & 15)) for x a multiple
+          // of 16 and lr < 16; s & ~15 depends on kk alone (lk < 4 sets only
+          // bits 0-1 of kr), so each tile's part is an immediate offset.  This
+          // form saves address registers (it took the FP64 8-wave kernel with
+          // op(A) = A^T out of spilling).  For B when only B is swizzled
+          // (column-major A and B) the plain form measured faster.
+          const int hia = A_KC ? MF::swz(4 * kk) & ~15 : 0;
+          const int hib = B_NC ? 0 : MF::swz(4 * kk) & ~15;
+          const int rowa = row + (lr ^ (swa & 15)), rowb = row + (lr ^ (swb & 15));
+          constexpr bool SPLIT_B = A_KC || B_NC;
 #pragma unroll
-        for (int i = 0; i < TIM; ++i) af[i] = as[rowa + ((wm + i * 16) ^ hia)];
+          for (int i = 0; i < TIM; ++i) af[i] = as[rowa + ((wm + i * 16) ^ hia)];
 #pragma unroll
-        for (int j = 0; j < TIN; ++j)
-          bf[j] = SPLIT_B ? bs[rowb + ((wn + j * 16) ^ hib)] : bs[row + ((wn + j * 16 + lr) ^ swb)];
+          for (int j = 0; j < TIN; ++j)
+            bf[j] = SPLIT_B ? bs[rowb + ((wn + j * 16) ^ hib)] : bs[row + ((wn + j * 16 + lr) ^ swb)];
 #pragma unroll
-        for (int i = 0; i < TIM; ++i)
+          for (int i = 0; i < TIM; ++i)
 #pragma unroll
-          for (int j = 0; j < TIN; ++j) acc[i][j] = MF::step(af[i], bf[j], acc[i][j]);
+            for (int j = 0; j < TIN; ++j) acc[i][j] = MF::step(af[i], bf[j], acc[i][j]);
+        }
+        __builtin_amdgcn_s_setprio(0);
       }
-      __builtin_amdgcn_s_setprio(0);
       if (more) {
         ra.template store<A_KC>(As[cur ^ 1], tid);
         rb.template store<!B_NC>(Bs[cur ^ 1], tid);
