@@ -437,8 +437,75 @@ __device__ inline void gemm_tile(const GemmArgs<T> &p, int tm, int tn, T *lds) {
   typename Ops::SA ra;
   typename Ops::SB rb;
 
+  // Elements of C this tile stores.
+  auto c_in = [&](int row, int col) {
+    return row < p.m && col < p.n && !(p.ji == 1 && col > row) && !(p.ji == 2 && col < row);
+  };
+  // SMALL with a column-major C: when a thread's share of the C tile is at
+  // most 8 values (the epilogue's layout below), they are read as the last
+  // K slice begins -- its operand registers are free by then -- so the read
+  // latency runs under that slice's MFMAs instead of after them.
+  constexpr int CQ = BM * BN / NTH;
+  constexpr bool CPRE = SMALL && CQ <= 8;
+  T cpre[CPRE ? CQ : 1];
+  bool have_c = false;
+  if constexpr (SMALL) {
+    // One K segment (SYR2K runs two, each over its own operands); the
+    // caller has synchronised before the tile, so the first segment stages
+    // its first slice without a barrier of its own.  PRE: the last segment.
+    auto segment = [&](auto PRE, int seg) {
+      const T *a = seg ? p.a2 : p.a;
+      const T *b = seg ? p.b2 : p.b;
+      const int64_t lda = seg ? (A_KC ? p.rs_a2 : p.cs_a2) : (A_KC ? p.rs_a : p.cs_a);
+      const int64_t ldb = seg ? (B_NC ? p.rs_b2 : p.cs_b2) : (B_NC ? p.rs_b : p.cs_b);
+      const bool va = seg ? p.vec_a2 : p.vec_a;
+      const bool vb = seg ? p.vec_b2 : p.vec_b;
+      int kbeg = 0, kend = p.k;
+      if (p.tri_a == 1) kend = min(kend, m0 + BM);
+      if (p.tri_a == 2) kbeg = max(kbeg, m0);
+      if (p.tri_b == 1) kbeg = max(kbeg, n0);
+      if (p.tri_b == 2) kend = min(kend, n0 + BN);
+      const int kt0 = kbeg / BK;
+      const int ktiles = kend > kbeg ? (kend + BK - 1) / BK - kt0 : 0;
+      if (ktiles <= 0) return;
+      load_slices<T, TILE, A_KC, B_NC, BK, NTH, true>(ra, rb, a, lda, b, ldb, m0, n0, kt0 * BK,
+                                                      p.m, p.n, p.k, va, vb, tid);
+      if (seg > 0) __syncthreads();  // the first segment's readers are done with buffer 0
+      ra.template store<A_KC>(As[0], tid);
+      rb.template store<!B_NC>(Bs[0], tid);
+      __syncthreads();
 #pragma unroll 1
-  for (int seg = 0; seg < 2; ++seg) {
+      for (int kt = 0; kt + 1 < ktiles; ++kt) {
+        const int cur = kt & 1;
+        load_slices<T, TILE, A_KC, B_NC, BK, NTH, true>(ra, rb, a, lda, b, ldb, m0, n0,
+                                                        (kt0 + kt + 1) * BK, p.m, p.n, p.k, va,
+                                                        vb, tid);
+        if (live) mfma_slice<T, TILE, A_KC, B_NC, BK, NTH>(acc, As[cur], Bs[cur], wm, wn, lr, lk);
+        ra.template store<A_KC>(As[cur ^ 1], tid);
+        rb.template store<!B_NC>(Bs[cur ^ 1], tid);
+        __syncthreads();
+      }
+      if constexpr (decltype(PRE)::value && CPRE) {
+        if (p.rs_c == 1 && p.beta != T(0)) {
+#pragma unroll
+          for (int q = 0; q < CQ; ++q) {
+            const int e = tid + q * NTH, row = m0 + e % BM, col = n0 + e / BM;
+            cpre[q] = c_in(row, col) ? gmem(p.c)[(int64_t)row + (int64_t)col * p.cs_c] : T(0);
+          }
+          have_c = true;
+        }
+      }
+      if (live)
+        mfma_slice<T, TILE, A_KC, B_NC, BK, NTH>(acc, As[(ktiles - 1) & 1], Bs[(ktiles - 1) & 1],
+                                                 wm, wn, lr, lk);
+      __syncthreads();
+    };
+    if (p.a2) segment(std::false_type{}, 0);
+    segment(std::true_type{}, p.a2 ? 1 : 0);
+  }
+
+#pragma unroll 1
+  for (int seg = 0; seg < (SMALL ? 0 : 2); ++seg) {
     const T *a = seg ? p.a2 : p.a;
     const T *b = seg ? p.b2 : p.b;
     if (!a) break;
@@ -499,11 +566,9 @@ __device__ inline void gemm_tile(const GemmArgs<T> &p, int tm, int tn, T *lds) {
       }
       continue;
     }
-    load_slices<T, TILE, A_KC, B_NC, BK, NTH, SMALL>(ra, rb, a, lda, b, ldb, m0, n0, kt0 * BK, p.m, p.n,
-                                     p.k, va, vb, tid);
-    // previous segment's readers are done with buffer 0 (SMALL: the caller
-    // synchronised before the first)
-    if (!SMALL || seg > 0) __syncthreads();
+    load_slices<T, TILE, A_KC, B_NC, BK, NTH>(ra, rb, a, lda, b, ldb, m0, n0, kt0 * BK, p.m, p.n,
+                                              p.k, va, vb, tid);
+    __syncthreads();  // previous segment's readers are done with buffer 0
     ra.template store<A_KC>(As[0], tid);
     rb.template store<!B_NC>(Bs[0], tid);
     __syncthreads();
@@ -513,8 +578,8 @@ __device__ inline void gemm_tile(const GemmArgs<T> &p, int tm, int tn, T *lds) {
       const int cur = kt & 1;
       const bool more = kt + 1 < ktiles;
       if (more)
-        load_slices<T, TILE, A_KC, B_NC, BK, NTH, SMALL>(ra, rb, a, lda, b, ldb, m0, n0,
-                                         (kt0 + kt + 1) * BK, p.m, p.n, p.k, va, vb, tid);
+        load_slices<T, TILE, A_KC, B_NC, BK, NTH>(ra, rb, a, lda, b, ldb, m0, n0,
+                                                  (kt0 + kt + 1) * BK, p.m, p.n, p.k, va, vb, tid);
       const T *as = As[cur];
       const T *bs = Bs[cur];
       // A wave raises its priority for its MFMA block, so that of the waves
@@ -587,10 +652,9 @@ __device__ inline void gemm_tile(const GemmArgs<T> &p, int tm, int tn, T *lds) {
             ct[(wn + j * 16 + lr) * PT + wm + i * 16 + MF::row(lk, r)] = acc[i][j][r];
       __syncthreads();
       const T alpha = p.alpha, beta = p.beta;
-      auto ok = [&](int row, int col) {
-        return row < p.m && col < p.n && !(p.ji == 1 && col > row) && !(p.ji == 2 && col < row);
-      };
+      auto ok = c_in;
       constexpr int CH = NQ < 8 ? NQ : 8;  // C values in flight per thread
+      static_assert(!CPRE || (CQ == NQ && CH == NQ), "read ahead: one round of C values");
 #pragma unroll
       for (int q0 = 0; q0 < NQ; q0 += CH) {
         T cv[CH];
@@ -598,6 +662,12 @@ __device__ inline void gemm_tile(const GemmArgs<T> &p, int tm, int tn, T *lds) {
 #pragma unroll
           for (int q = 0; q < CH; ++q) {
             const int e = tid + (q0 + q) * NTH, row = m0 + e % BM, col = n0 + e / BM;
+            if constexpr (CPRE) {
+              if (have_c) {
+                cv[q] = cpre[q];
+                continue;
+              }
+            }
             cv[q] = ok(row, col) ? gmem(p.c)[(int64_t)row + (int64_t)col * p.cs_c] : T(0);
           }
         }
