@@ -441,13 +441,36 @@ __device__ inline void gemm_tile(const GemmArgs<T> &p, int tm, int tn, T *lds) {
   auto c_in = [&](int row, int col) {
     return row < p.m && col < p.n && !(p.ji == 1 && col > row) && !(p.ji == 2 && col < row);
   };
-  // SMALL with a column-major C: when a thread's share of the C tile is at
-  // most 8 values (the epilogue's layout below), they are read as the last
-  // K slice begins -- its operand registers are free by then -- so the read
+  // SMALL with a column-major C: the epilogue below gives each thread runs
+  // of CV consecutive rows of one column (16 bytes): run u of the thread is
+  // q = tid + u * NTH, rows m0 + CV * (q % (BM / CV)) ..., column n0 + q /
+  // (BM / CV), so the lanes of a wave cover whole columns.  A run that C
+  // stores whole moves as one vector; any other (the edge of C, the diagonal
+  // of a SYRK tile) element by element, only where c_in holds.
+  constexpr int CV = 16 / sizeof(T), CR = BM / CV, CU = BM * BN / (CV * NTH);
+  static_assert(CU * CV * NTH == BM * BN, "whole runs per thread");
+  typedef typename MF::vec_t cvec_t;
+  auto run_whole = [&](int row, int col) {
+    return row + CV - 1 < p.m && col < p.n && !(p.ji == 1 && col > row) &&
+           !(p.ji == 2 && col < row + CV - 1);
+  };
+  auto load_run = [&](int row, int col, T(&v)[CV]) {
+    const T *cp = p.c + (int64_t)row + (int64_t)col * p.cs_c;
+    if (run_whole(row, col)) {
+      cvec_t t;
+      __builtin_memcpy(&t, gmem(cp), sizeof t);
+#pragma unroll
+      for (int e = 0; e < CV; ++e) v[e] = t[e];
+    } else {
+#pragma unroll
+      for (int e = 0; e < CV; ++e) v[e] = c_in(row + e, col) ? gmem(cp)[e] : T(0);
+    }
+  };
+  // When a thread's share is at most 8 values they are read as the last K
+  // slice begins -- its operand registers are free by then -- so the read
   // latency runs under that slice's MFMAs instead of after them.
-  constexpr int CQ = BM * BN / NTH;
-  constexpr bool CPRE = SMALL && CQ <= 8;
-  T cpre[CPRE ? CQ : 1];
+  constexpr bool CPRE = SMALL && CU * CV <= 8;
+  T cpre[CPRE ? CU : 1][CV];
   bool have_c = false;
   if constexpr (SMALL) {
     // One K segment (SYR2K runs two, each over its own operands); the
@@ -488,9 +511,9 @@ __device__ inline void gemm_tile(const GemmArgs<T> &p, int tm, int tn, T *lds) {
       if constexpr (decltype(PRE)::value && CPRE) {
         if (p.rs_c == 1 && p.beta != T(0)) {
 #pragma unroll
-          for (int q = 0; q < CQ; ++q) {
-            const int e = tid + q * NTH, row = m0 + e % BM, col = n0 + e / BM;
-            cpre[q] = c_in(row, col) ? gmem(p.c)[(int64_t)row + (int64_t)col * p.cs_c] : T(0);
+          for (int u = 0; u < CU; ++u) {
+            const int q = tid + u * NTH;
+            load_run(m0 + CV * (q % CR), n0 + q / CR, cpre[u]);
           }
           have_c = true;
         }
@@ -636,12 +659,12 @@ __device__ inline void gemm_tile(const GemmArgs<T> &p, int tm, int tn, T *lds) {
     if (p.rs_c == 1) {
       // The K loop ended with a barrier (and without one the caller's
       // barrier stands): LDS is free.  Column c of the tile is LDS row c
-      // (pitch BM + 4: a lane's 4 rows stay one aligned 16-byte store).
-      // Thread tid handles elements e = tid + q * NTH (row e % BM, column
-      // e / BM), so consecutive lanes take consecutive rows of one column.
-      constexpr int PT = BM + 4, NQ = BM * BN / NTH;
+      // (pitch BM + 4: a lane's 4 rows stay one aligned 16-byte store, and
+      // a run of CV rows one aligned 16-byte read).  C moves in the runs
+      // described above.
+      constexpr int PT = BM + 4;
       static_assert(BN * PT <= LdsSize<T, TILE, BK>::value, "staged tile fits the LDS");
-      static_assert(NQ * NTH == BM * BN, "whole element rounds");
+      static_assert((PT * sizeof(T)) % 16 == 0, "runs stay 16-byte aligned in LDS");
       T *ct = lds;
 #pragma unroll
       for (int i = 0; i < TIM; ++i)
@@ -652,35 +675,46 @@ __device__ inline void gemm_tile(const GemmArgs<T> &p, int tm, int tn, T *lds) {
             ct[(wn + j * 16 + lr) * PT + wm + i * 16 + MF::row(lk, r)] = acc[i][j][r];
       __syncthreads();
       const T alpha = p.alpha, beta = p.beta;
-      auto ok = c_in;
-      constexpr int CH = NQ < 8 ? NQ : 8;  // C values in flight per thread
-      static_assert(!CPRE || (CQ == NQ && CH == NQ), "read ahead: one round of C values");
+      constexpr int CH = CU < 2 ? CU : 2;  // runs of C in flight per thread
 #pragma unroll
-      for (int q0 = 0; q0 < NQ; q0 += CH) {
-        T cv[CH];
+      for (int u0 = 0; u0 < CU; u0 += CH) {
+        T cv[CH][CV];
         if (beta != T(0)) {
 #pragma unroll
-          for (int q = 0; q < CH; ++q) {
-            const int e = tid + (q0 + q) * NTH, row = m0 + e % BM, col = n0 + e / BM;
+          for (int u = 0; u < CH; ++u) {
             if constexpr (CPRE) {
               if (have_c) {
-                cv[q] = cpre[q];
+#pragma unroll
+                for (int e = 0; e < CV; ++e) cv[u][e] = cpre[u0 + u][e];
                 continue;
               }
             }
-            cv[q] = ok(row, col) ? gmem(p.c)[(int64_t)row + (int64_t)col * p.cs_c] : T(0);
+            const int q = tid + (u0 + u) * NTH;
+            load_run(m0 + CV * (q % CR), n0 + q / CR, cv[u]);
           }
         }
 #pragma unroll
-        for (int q = 0; q < CH; ++q) {
-          const int e = tid + (q0 + q) * NTH, rl = e % BM, cl = e / BM;
+        for (int u = 0; u < CH; ++u) {
+          const int q = tid + (u0 + u) * NTH, rl = CV * (q % CR), cl = q / CR;
           const int row = m0 + rl, col = n0 + cl;
-          if (!ok(row, col)) continue;
-          auto *cp = gmem(p.c) + (int64_t)row + (int64_t)col * p.cs_c;
-          const T prod = alpha * ct[cl * PT + rl];
-          if (beta == T(0)) *cp = prod;
-          else if (beta == T(1)) *cp = cv[q] + prod;
-          else *cp = prod + beta * cv[q];
+          const cvec_t acv = *reinterpret_cast<const cvec_t *>(ct + cl * PT + rl);
+          T out[CV];
+#pragma unroll
+          for (int e = 0; e < CV; ++e) {
+            const T prod = alpha * acv[e];
+            out[e] = beta == T(0) ? prod : beta == T(1) ? cv[u][e] + prod : prod + beta * cv[u][e];
+          }
+          T *cp = p.c + (int64_t)row + (int64_t)col * p.cs_c;
+          if (run_whole(row, col)) {
+            cvec_t t;
+#pragma unroll
+            for (int e = 0; e < CV; ++e) t[e] = out[e];
+            __builtin_memcpy(gmem(cp), &t, sizeof t);
+          } else {
+#pragma unroll
+            for (int e = 0; e < CV; ++e)
+              if (c_in(row + e, col)) gmem(cp)[e] = out[e];
+          }
         }
       }
       return;
