@@ -390,10 +390,10 @@ __device__ inline void mfma_slice(typename Mfma<T>::acc_t (&acc)[TIM][TIN], cons
 // SMALL is the grouped kernel's mode for small tiles, whose short K loops
 // leave the fixed costs of a tile a large share of its time; it changes only
 // how data moves, never the arithmetic of an element:
-//  * a column-major C (rs_c == 1) is written through LDS: the MFMA output
-//    layout gives each lane 4 rows of one column, so a wave's store touches
-//    16 columns, 16 bytes in each; staged through LDS each wave reads and
-//    writes runs of consecutive rows of one column instead;
+//  * a column-major C (rs_c == 1) moves in 16-byte runs: the FP32 MFMA
+//    output layout gives each lane 4 consecutive rows of one column, and the
+//    item's C values are read as its last K slice begins;
+//  * a wave whose region of the tile stores nothing skips its MFMAs;
 //  * slices of edge tiles are loaded by whole vectors where they can
 //    (Slice::load's EV);
 //  * the caller has synchronised before the tile, so the first segment
@@ -441,14 +441,16 @@ __device__ inline void gemm_tile(const GemmArgs<T> &p, int tm, int tn, T *lds) {
   auto c_in = [&](int row, int col) {
     return row < p.m && col < p.n && !(p.ji == 1 && col > row) && !(p.ji == 2 && col < row);
   };
-  // SMALL with a column-major C: the epilogue below gives each thread runs
-  // of CV consecutive rows of one column (16 bytes): run u of the thread is
-  // q = tid + u * NTH, rows m0 + CV * (q % (BM / CV)) ..., column n0 + q /
-  // (BM / CV), so the lanes of a wave cover whole columns.  A run that C
-  // stores whole moves as one vector; any other (the edge of C, the diagonal
-  // of a SYRK tile) element by element, only where c_in holds.
-  constexpr int CV = 16 / sizeof(T), CR = BM / CV, CU = BM * BN / (CV * NTH);
-  static_assert(CU * CV * NTH == BM * BN, "whole runs per thread");
+  // SMALL with a column-major C: the FP32 MFMA leaves each lane 4
+  // consecutive rows of one column of each of its 16 x 16 tiles, a 16-byte
+  // run of C.  Run u = (i, j) = (u / TIN, u % TIN) of a lane covers rows
+  // run_row(u) .. + 3 of column run_col(u).  A run that C stores whole moves
+  // as one vector; any other (the edge of C, the diagonal of a SYRK tile)
+  // element by element, only where c_in holds.
+  constexpr int CV = 16 / sizeof(T), CU = TIM * TIN;
+  static_assert(!SMALL || (sizeof(T) == 4 && CV == 4), "FP32: a lane's 4 rows are one run");
+  auto run_row = [&](int u) { return m0 + wm + 16 * (u / TIN) + 4 * lk; };
+  auto run_col = [&](int u) { return n0 + wn + 16 * (u % TIN) + lr; };
   typedef typename MF::vec_t cvec_t;
   auto run_whole = [&](int row, int col) {
     return row + CV - 1 < p.m && col < p.n && !(p.ji == 1 && col > row) &&
@@ -511,10 +513,7 @@ __device__ inline void gemm_tile(const GemmArgs<T> &p, int tm, int tn, T *lds) {
       if constexpr (decltype(PRE)::value && CPRE) {
         if (p.rs_c == 1 && p.beta != T(0)) {
 #pragma unroll
-          for (int u = 0; u < CU; ++u) {
-            const int q = tid + u * NTH;
-            load_run(m0 + CV * (q % CR), n0 + q / CR, cpre[u]);
-          }
+          for (int u = 0; u < CU; ++u) load_run(run_row(u), run_col(u), cpre[u]);
           have_c = true;
         }
       }
@@ -657,23 +656,7 @@ __device__ inline void gemm_tile(const GemmArgs<T> &p, int tm, int tn, T *lds) {
   // memory latency per row of tiles instead of one per element).
   if constexpr (SMALL) {
     if (p.rs_c == 1) {
-      // The K loop ended with a barrier (and without one the caller's
-      // barrier stands): LDS is free.  Column c of the tile is LDS row c
-      // (pitch BM + 4: a lane's 4 rows stay one aligned 16-byte store, and
-      // a run of CV rows one aligned 16-byte read).  C moves in the runs
-      // described above.
-      constexpr int PT = BM + 4;
-      static_assert(BN * PT <= LdsSize<T, TILE, BK>::value, "staged tile fits the LDS");
-      static_assert((PT * sizeof(T)) % 16 == 0, "runs stay 16-byte aligned in LDS");
-      T *ct = lds;
-#pragma unroll
-      for (int i = 0; i < TIM; ++i)
-#pragma unroll
-        for (int j = 0; j < TIN; ++j)
-#pragma unroll
-          for (int r = 0; r < 4; ++r)
-            ct[(wn + j * 16 + lr) * PT + wm + i * 16 + MF::row(lk, r)] = acc[i][j][r];
-      __syncthreads();
+      // Straight from the accumulators, in the runs described above.
       const T alpha = p.alpha, beta = p.beta;
       constexpr int CH = CU < 2 ? CU : 2;  // runs of C in flight per thread
 #pragma unroll
@@ -689,19 +672,17 @@ __device__ inline void gemm_tile(const GemmArgs<T> &p, int tm, int tn, T *lds) {
                 continue;
               }
             }
-            const int q = tid + (u0 + u) * NTH;
-            load_run(m0 + CV * (q % CR), n0 + q / CR, cv[u]);
+            load_run(run_row(u0 + u), run_col(u0 + u), cv[u]);
           }
         }
 #pragma unroll
         for (int u = 0; u < CH; ++u) {
-          const int q = tid + (u0 + u) * NTH, rl = CV * (q % CR), cl = q / CR;
-          const int row = m0 + rl, col = n0 + cl;
-          const cvec_t acv = *reinterpret_cast<const cvec_t *>(ct + cl * PT + rl);
+          const int i = (u0 + u) / TIN, j = (u0 + u) % TIN;
+          const int row = run_row(u0 + u), col = run_col(u0 + u);
           T out[CV];
 #pragma unroll
           for (int e = 0; e < CV; ++e) {
-            const T prod = alpha * acv[e];
+            const T prod = alpha * acc[i][j][e];
             out[e] = beta == T(0) ? prod : beta == T(1) ? cv[u][e] + prod : prod + beta * cv[u][e];
           }
           T *cp = p.c + (int64_t)row + (int64_t)col * p.cs_c;
