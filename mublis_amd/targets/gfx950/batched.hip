@@ -119,8 +119,12 @@ __global__ __launch_bounds__(NTH, WG_PER_CU * NTH / 256) void grouped_kernel(
     const GroupItem wn = items[nx < nitems ? nx : it];
     const GroupProb g = probs[w.prob];
     // every item of a launch has the launch's tile shape (kind >> 2)
+    // No barrier between items: a tile's last K slice ends with one, after
+    // which neither epilogue touches lds, and a 128-tile's first slice
+    // waits at a barrier of its own before it restages lds.  A wave that
+    // finishes its part of the epilogue starts loading the next item's
+    // first slice while the others still store.
     run_tile<MAXT, BK, NTH>(g, w.tm, w.tn, w.kind & 3, lds);
-    __syncthreads();  // the next item restages lds (gemm_tile's SMALL mode relies on it)
     if (nx >= nitems) break;
     it = nx;
     w = wn;

@@ -382,8 +382,9 @@ __device__ inline void mfma_slice(typename Mfma<T>::acc_t (&acc)[TIM][TIN], cons
 
 // One TILE x TILE output tile (tm, tn) of the product described by p,
 // computed by the calling workgroup of NTH threads with `lds` (LdsSize
-// elements) as staging memory.  The caller synchronises before reusing lds.
-// BK, the K depth of one staged slice, sets only how K is cut into slices:
+// elements) as staging memory.  The caller synchronises before reusing lds
+// (SMALL: the tile's last K slice ends with a barrier and nothing after it
+// touches lds, so a tile that follows needs no barrier of its own).  BK, the K depth of one staged slice, sets only how K is cut into slices:
 // every depth runs the same MFMA steps in the same k order (a slice past k
 // adds exact zeros), so the result does not depend on it.
 //
@@ -396,8 +397,8 @@ __device__ inline void mfma_slice(typename Mfma<T>::acc_t (&acc)[TIM][TIN], cons
 //  * a wave whose region of the tile stores nothing skips its MFMAs;
 //  * slices of edge tiles are loaded by whole vectors where they can
 //    (Slice::load's EV);
-//  * the caller has synchronised before the tile, so the first segment
-//    stages its first slice without a barrier of its own.
+//  * the first segment stages its first slice without a barrier of its
+//    own: the previous tile's last K slice ended with one.
 template <typename T, int TILE, bool A_KC, bool B_NC, int BK = Mfma<T>::BK, int NTH = NT,
           bool SMALL = false>
 __device__ inline void gemm_tile(const GemmArgs<T> &p, int tm, int tn, T *lds) {
@@ -475,9 +476,10 @@ __device__ inline void gemm_tile(const GemmArgs<T> &p, int tm, int tn, T *lds) {
   T cpre[CPRE ? CU : 1][CV];
   bool have_c = false;
   if constexpr (SMALL) {
-    // One K segment (SYR2K runs two, each over its own operands); the
-    // caller has synchronised before the tile, so the first segment stages
-    // its first slice without a barrier of its own.  PRE: the last segment.
+    // One K segment (SYR2K runs two, each over its own operands); the first
+    // segment stages its first slice without a barrier of its own (every
+    // reader of lds passed the barrier that ends the previous tile's last
+    // slice, and the epilogue does not use lds).  PRE: the last segment.
     auto segment = [&](auto PRE, int seg) {
       const T *a = seg ? p.a2 : p.a;
       const T *b = seg ? p.b2 : p.b;
