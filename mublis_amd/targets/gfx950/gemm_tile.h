@@ -351,15 +351,19 @@ struct LdsSize {
 };
 
 // The MFMA steps of one staged slice (the K loop of gemm_tile below, as a
-// function for the loop that loads two slices ahead).
+// function for the loop that loads two slices ahead).  Only the first nk of
+// its BK / 4 steps run; the caller skips only steps whose k rows all lie
+// past the end of K (exact zeros in both operands) and accounts for them.
 template <typename T, int TILE, bool A_KC, bool B_NC, int BK, int NTH, int TIM, int TIN>
 __device__ inline void mfma_slice(typename Mfma<T>::acc_t (&acc)[TIM][TIN], const T *as,
-                                  const T *bs, int wm, int wn, int lr, int lk) {
+                                  const T *bs, int wm, int wn, int lr, int lk,
+                                  int nk = BK / 4) {
   typedef Mfma<T> MF;
   constexpr int S = Tile<TILE>::S;
   __builtin_amdgcn_s_setprio(1);
 #pragma unroll
   for (int kk = 0; kk < BK / 4; ++kk) {
+    if (kk >= nk) break;
     T af[TIM], bf[TIN];
     const int kr = kk * 4 + lk, row = kr * S;
     const int swa = A_KC ? MF::swz(kr) : 0, swb = B_NC ? 0 : MF::swz(kr);
@@ -380,13 +384,34 @@ __device__ inline void mfma_slice(typename Mfma<T>::acc_t (&acc)[TIM][TIN], cons
   __builtin_amdgcn_s_setprio(0);
 }
 
+// Adds +0 to every accumulator: what a 4-deep MFMA step of exact zeros does.
+template <typename T, int TIM, int TIN>
+__device__ inline void zero_pad_fix(typename Mfma<T>::acc_t (&acc)[TIM][TIN]) {
+#pragma unroll
+  for (int i = 0; i < TIM; ++i)
+#pragma unroll
+    for (int j = 0; j < TIN; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) acc[i][j][r] += T(0);
+}
+
 // One TILE x TILE output tile (tm, tn) of the product described by p,
 // computed by the calling workgroup of NTH threads with `lds` (LdsSize
 // elements) as staging memory.  The caller synchronises before reusing lds
 // (SMALL: the tile's last K slice ends with a barrier and nothing after it
-// touches lds, so a tile that follows needs no barrier of its own).  BK, the K depth of one staged slice, sets only how K is cut into slices:
-// every depth runs the same MFMA steps in the same k order (a slice past k
-// adds exact zeros), so the result does not depend on it.
+// touches lds, so a tile that follows needs no barrier of its own).
+//
+// BK, the K depth of one staged slice, sets only how K is cut into slices:
+// every depth runs the same MFMA steps in the same k order, and the steps of
+// a slice past k add exact zeros.  Such a zero step leaves every
+// accumulator as it was (denormals included, measured on gfx950) except -0,
+// which becomes +0 (an accumulator reaches -0 only when the products of an
+// element underflow).  So that this does not depend on the depth either,
+// every K segment ends as if zero steps ran up to a multiple of the full
+// depth Mfma<T>::BK: a shallower slice depth, or SMALL's last slice, which
+// skips its zero steps, adds +0 to the accumulators once instead
+// (zero_pad_fix).  Segments cut short by a triangular operand (tri_a,
+// tri_b) run the same slices at every depth they are launched with.
 //
 // SMALL is the grouped kernel's mode for small tiles, whose short K loops
 // leave the fixed costs of a tile a large share of its time; it changes only
@@ -519,9 +544,16 @@ __device__ inline void gemm_tile(const GemmArgs<T> &p, int tm, int tn, T *lds) {
           have_c = true;
         }
       }
-      if (live)
+      // The last slice runs only the 4-deep steps that reach into K (the
+      // rest have exact zeros in both operands), then stands in for the
+      // zero steps up to a multiple of the full depth (see above).
+      const int klast = (kt0 + ktiles - 1) * BK;
+      const int nk = min(BK / 4, (p.k - klast + 3) / 4);
+      if (live) {
         mfma_slice<T, TILE, A_KC, B_NC, BK, NTH>(acc, As[(ktiles - 1) & 1], Bs[(ktiles - 1) & 1],
-                                                 wm, wn, lr, lk);
+                                                 wm, wn, lr, lk, nk);
+        if (klast + 4 * nk < (p.k + MF::BK - 1) / MF::BK * MF::BK) zero_pad_fix<T>(acc);
+      }
       __syncthreads();
     };
     if (p.a2) segment(std::false_type{}, 0);
@@ -590,6 +622,10 @@ __device__ inline void gemm_tile(const GemmArgs<T> &p, int tm, int tn, T *lds) {
       }
       continue;
     }
+    // a shallow depth stands in for the zero steps up to a multiple of the
+    // full depth once the segment's slices are done (see above)
+    const bool pad_fix = BK < MF::BK && !p.tri_a && !p.tri_b &&
+                         (p.k + BK - 1) / BK * BK < (p.k + MF::BK - 1) / MF::BK * MF::BK;
     load_slices<T, TILE, A_KC, B_NC, BK, NTH>(ra, rb, a, lda, b, ldb, m0, n0, kt0 * BK, p.m, p.n,
                                               p.k, va, vb, tid);
     __syncthreads();  // previous segment's readers are done with buffer 0
@@ -645,6 +681,9 @@ __device__ inline void gemm_tile(const GemmArgs<T> &p, int tm, int tn, T *lds) {
         rb.template store<!B_NC>(Bs[cur ^ 1], tid);
       }
       __syncthreads();
+    }
+    if constexpr (BK < MF::BK) {
+      if (pad_fix) zero_pad_fix<T>(acc);
     }
   }
 

@@ -130,6 +130,46 @@ def test_grouped_syr2k_equals_single_calls(lib):
         assert np.linalg.norm(got - ref) <= 2 * (k + 2) * eps * scale, (n, k)
 
 
+def test_signed_zero_of_underflowing_products(lib):
+    """Where every product of an element underflows the accumulator is -0,
+    and an MFMA step that adds products of K's zero padding turns it into
+    +0.  Every path gives the result of padding K to a multiple of the full
+    slice depth (32 in FP32), so an element is +0 exactly when K is not a
+    multiple of 32: single calls of every tile shape and depth (32-, 64-,
+    64 half-depth, 128 half-depth) and the grouped kernel, whose last slice
+    skips its steps of padding alone.  Compared bit for bit, signs of zero
+    included."""
+    torch = pytest.importorskip("torch")
+    g = torch.Generator(device="cuda").manual_seed(5)
+    ops, specs = [], []
+    for (m, n) in ((40, 40), (200, 150), (2100, 2048), (3072, 3072)):
+        for k in (16, 20, 28, 30, 32, 36, 64):
+            A = -(torch.rand((k, m), device="cuda", generator=g) * 0.5 + 0.5) * 1e-30
+            B = (torch.rand((n, k), device="cuda", generator=g) * 0.5 + 0.5) * 1e-30
+            C = torch.ones((n, m), device="cuda")
+            ops.append(M.SbatchOp(M.BATCH_GEMM, 0, 0, 0, m, n, k, 1.0, 0.0, A.data_ptr(), m,
+                                  B.data_ptr(), k, C.data_ptr(), m))
+            specs.append(dict(op="gemm", ta=0, tb=0, m=m, n=n, k=k, A=A, B=B, C=C, lda=m, ldb=k))
+    initial = [s["C"].clone() for s in specs]
+    want = single_calls(lib, torch, specs, initial, 1.0, 0.0)
+    for s, w in zip(specs, want):
+        k = s["k"]
+        sign = 0 if k % 32 else -(2 ** 31)
+        bits = w.view(torch.int32)
+        assert bool((bits == sign).all()), (s["m"], s["n"], k, "single call")
+    for s, c0 in zip(specs, initial):
+        s["C"].copy_(c0)
+    plan = M.GroupedPlan(ops)
+    try:
+        plan.execute()
+        torch.cuda.synchronize()
+    finally:
+        plan.close()
+    bad = [(s["m"], s["n"], s["k"]) for s, w in zip(specs, want)
+           if not torch.equal(s["C"].view(torch.int32), w.view(torch.int32))]
+    assert not bad, bad[:5]
+
+
 def test_forced_tiles_and_replay(lib):
     """Every tile shape under every launch schedule gives the bits of the
     single calls, and a second replay accumulates again (beta = 1)."""
