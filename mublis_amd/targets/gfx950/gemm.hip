@@ -54,7 +54,7 @@ using namespace gfx950;
 template <typename T, int TILE>
 bool use_shallow(const GemmArgs<T> &p, int64_t wgs) {
   if (p.lpt || TILE == 32) return false;
-  if (TILE == 64) return p.tri != 3 && wgs > 1024;
+  if (TILE == 64) return wgs > 1024;
   return sizeof(T) == 4 && wgs > 512;
 }
 //
@@ -101,7 +101,7 @@ __global__ __launch_bounds__((KernelShape<T, TILE, SHALLOW>::NTH),
       return;
   } else {
     map_tile(blockIdx.x, p.ntiles, p.tiles_m, p.tiles_n, p.tri, p.ji, tm, tn);
-    if (tm >= p.tiles_m || tn >= p.tiles_n) return;  // mode 3 past an odd last tile
+    if (tm >= p.tiles_m || tn >= p.tiles_n) return;  // mode 3 past a short last block
   }
   gemm_tile<T, TILE, A_KC, B_NC, BK, KernelShape<T, TILE, SHALLOW>::NTH>(p, tm, tn, lds);
 }
@@ -148,7 +148,7 @@ hipError_t launch(GemmArgs<T> p, bool akc, bool bnc, hipStream_t s, int nbatch =
   const int t = p.tiles_m;
   p.ntiles = p.tri == 1 ? t * (t + 1) / 2
            : p.tri == 2 ? t * (t - 1) / 2
-           : p.tri == 3 ? 3 * ((t + 1) / 2)
+           : p.tri == 3 ? 10 * ((t + 3) / 4)
                         : p.tiles_m * p.tiles_n;
   if (p.ntiles == 0) return hipSuccess;
   int wgs = p.ntiles;
@@ -271,12 +271,13 @@ int gemm_dev(int m, int n, int k, T alpha, const T *a, int64_t rs_a, int64_t cs_
     if (p.tri && n128 >= need) {
       // The triangle's diagonal 128-tiles would compute twice what they
       // store and push the tile count past a round of 512: 128-tiles
-      // strictly inside the triangle, then the diagonal blocks as three
-      // 64-tiles each.
+      // strictly inside the triangle, then the diagonal blocks as ten
+      // 32-tiles each (three 64-tiles each left most CUs idle: 192
+      // workgroups at n = 8192).
       p.tri = 2;
       e = launch<T, 128>(p, akc, bnc, s);
       p.tri = 3;
-      if (e == hipSuccess) e = launch<T, 64>(p, akc, bnc, s);
+      if (e == hipSuccess) e = launch<T, 32>(p, akc, bnc, s);
     } else {
       // 64-tile grids of at most 1.5 workgroups per CU take 32-tiles (4 per
       // CU, one to 1.5 rounds), the same MFMA steps per element.  Measured

@@ -75,8 +75,8 @@ struct GemmArgs {
   int tiles_m, tiles_n;
   // Grid over the ji triangle (SYRK/SYR2K): 0 every tile; 1 the tiles
   // touching the triangle; 2 only the tiles strictly inside it; 3 the tiles
-  // of the diagonal blocks of twice the tile size that touch the triangle
-  // (three per block), which complete mode 2.
+  // of the diagonal blocks of four times the tile size on the triangle's
+  // side (ten per block), which complete mode 2 of 128-tiles with 32-tiles.
   int tri;
   int ntiles;
   int vec_a, vec_b, vec_a2, vec_b2;  // 16-byte loads are aligned
@@ -198,10 +198,11 @@ __host__ __device__ inline void map_tile(int w, int ntiles, int tiles_m, int til
   const int xcd = w % 8, local = w / 8;
   const int id = (xcd < r ? xcd * (q + 1) : r * (q + 1) + (xcd - r) * q) + local;
   if (tri == 3) {
-    // diagonal block d of twice the tile: (2d, 2d), the off-diagonal tile
-    // on the ji side, (2d + 1, 2d + 1)
-    const int d = id / 3, r = id - 3 * d;
-    const int a = 2 * d + (r >= 1), b = 2 * d + (r == 2);
+    // diagonal block d of four times the tile: its 10 tiles on the ji side,
+    // row by row
+    const int d = id / 10, r = id - 10 * d;
+    const int row = (r >= 1) + (r >= 3) + (r >= 6), col = r - row * (row + 1) / 2;
+    const int a = 4 * d + row, b = 4 * d + col;
     if (ji == 2) { tm = b; tn = a; } else { tm = a; tn = b; }
     return;
   }

@@ -2,7 +2,7 @@
 // compiled from the same header the kernels use:
 //  * map_tile, full grid and triangle modes 1..3: every workgroup lands on a
 //    distinct tile inside the grid, and modes 2 + 3 together cover exactly
-//    the 64-tiles touching the ji triangle, each once;
+//    the 32-tiles touching the ji triangle, each once;
 //  * map_tile_lpt, lines along rows or columns, long end high or low: a
 //    bijection onto the tiles (padding workgroups excepted), every XCD
 //    (workgroup id mod 8) receives the same total K up to one dealt unit,
@@ -62,32 +62,31 @@ static void triangle(int t, int ji) {
   CHECK((int)seen.size() == nt, "tri %d covers %zu", t, seen.size());
 }
 
-// Modes 2 (128-tiles strictly inside) + 3 (diagonal 128-blocks as 64-tiles)
-// against the 64-tiles of an n x n triangle.
+// Modes 2 (128-tiles strictly inside) + 3 (diagonal 128-blocks as 32-tiles)
+// against the 32-tiles of an n x n triangle.
 static void split_triangle(int n, int ji) {
-  const int t128 = (n + 127) / 128, t64 = (n + 63) / 64;
-  std::vector<int> hits(t64 * t64, 0);
+  const int t128 = (n + 127) / 128, t32 = (n + 31) / 32;
+  std::vector<int> hits(t32 * t32, 0);
   const int n2 = t128 * (t128 - 1) / 2;
   for (int w = 0; w < n2; ++w) {
     int a, b;
     map_tile(w, n2, t128, t128, 2, ji, a, b);
-    CHECK(a < t128 && b < t128 && (ji == 1 ? b < a : a < b), "strict n %d -> %d,%d", n, a, b);
-    for (int i = 2 * a; i < 2 * a + 2 && i < t64; ++i)
-      for (int j = 2 * b; j < 2 * b + 2 && j < t64; ++j) hits[i * t64 + j]++;
+    for (int i = 4 * a; i < 4 * a + 4 && i < t32; ++i)
+      for (int j = 4 * b; j < 4 * b + 4 && j < t32; ++j) hits[i * t32 + j]++;
   }
-  const int n3 = 3 * ((t64 + 1) / 2);
+  const int n3 = 10 * ((t32 + 3) / 4);
   for (int w = 0; w < n3; ++w) {
     int a, b;
-    map_tile(w, n3, t64, t64, 3, ji, a, b);
-    if (a >= t64 || b >= t64) continue;  // the kernel's bounds exit
-    CHECK(a / 2 == b / 2, "diag n %d -> %d,%d off the diagonal block", n, a, b);
-    hits[a * t64 + b]++;
+    map_tile(w, n3, t32, t32, 3, ji, a, b);
+    if (a >= t32 || b >= t32) continue;  // the kernel's bounds exit
+    CHECK(a / 4 == b / 4, "diag n %d -> %d,%d off the diagonal block", n, a, b);
+    hits[a * t32 + b]++;
   }
-  for (int i = 0; i < t64; ++i)
-    for (int j = 0; j < t64; ++j) {
+  for (int i = 0; i < t32; ++i)
+    for (int j = 0; j < t32; ++j) {
       const bool want = ji == 1 ? j <= i : i <= j;
-      CHECK(hits[i * t64 + j] == (want ? 1 : 0), "split n %d ji %d tile %d,%d hit %d", n, ji,
-            i, j, hits[i * t64 + j]);
+      CHECK(hits[i * t32 + j] == (want ? 1 : 0), "split n %d ji %d tile %d,%d hit %d", n, ji,
+            i, j, hits[i * t32 + j]);
     }
 }
 

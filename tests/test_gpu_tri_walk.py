@@ -8,8 +8,8 @@
   Bar: ||B - B*||_F <= 2 s eps |alpha| ||T||_F ||B0||_F, s the triangle's
   order (the normwise GEMM bound of the other tests, with K = s).
 * SYRK / SYR2K at n >= 4096, launched as 128-tiles strictly inside the ji
-  triangle plus the diagonal blocks as three 64-tiles each (n = 4100: an odd
-  number of 64-tiles), against FP64 torch on the stored triangle; the other
+  triangle plus the diagonal blocks as ten 32-tiles each (n = 4100: a short
+  last diagonal block), against FP64 torch on the stored triangle; the other
   triangle must come back bit for bit.
   Bar: ||C - C*||_F <= 2 k eps (|alpha| ||A|| ||B|| + |beta| ||C0||)
   (twice the product term for SYR2K).

@@ -3,8 +3,8 @@
 tests/native/walk_check.cpp includes mublis_amd/targets/gfx950/gemm_tile.h,
 the header the kernels are built from, compiled for the host only:
 map_tile's full and triangle grids (including the SYRK split into 128-tiles
-strictly inside the triangle plus diagonal blocks as 64-tiles, which must
-cover every 64-tile of the triangle exactly once), and map_tile_lpt's
+strictly inside the triangle plus diagonal blocks as 32-tiles, which must
+cover every 32-tile of the triangle exactly once), and map_tile_lpt's
 long-first walk (a bijection onto the tiles; every XCD gets the same total
 K up to a bound that round-robin dealing would exceed; each XCD meets its
 tiles in non-increasing K).
