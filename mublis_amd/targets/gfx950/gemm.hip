@@ -421,8 +421,12 @@ int gfx950_gemm_batched(int m, int n, int k, T alpha, const T *a, int64_t rs_a, 
     p.bsb[q] = bsb[q];
     p.bsc[q] = bsc[q];
   }
+  // 32-tiles for batches of at most 384 64-tiles, as in gemm_dev (the TRSM
+  // inversion's doubling levels up to h = 256)
   const int64_t t128 = (int64_t)((m + 127) / 128) * ((n + 127) / 128) * count;
+  const int64_t t64 = (int64_t)((m + 63) / 64) * ((n + 63) / 64) * count;
   const hipError_t e = t128 >= 2048 ? launch<T, 128>(p, akc, bnc, s, count)
+                       : t64 <= 384 ? launch<T, 32>(p, akc, bnc, s, count)
                                     : launch<T, 64>(p, akc, bnc, s, count);
   return e == hipSuccess ? MUBLIS_OK : MUBLIS_EHIP;
 }
