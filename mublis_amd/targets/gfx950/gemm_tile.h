@@ -754,6 +754,45 @@ __device__ inline void gemm_tile(const GemmArgs<T> &p, int tm, int tn, T *lds) {
     rs_w = p.rs_c2;
     cs_w = p.cs_c2;
   }
+  if constexpr (sizeof(T) == 4) {
+    if (p.rs_c == 1 && rs_w == 1) {
+      // FP32, column-major C: a lane's 4 rows of a 16 x 16 tile are one
+      // 16-byte run, moved whole where C stores all of it (run_whole), as
+      // in the SMALL epilogue; the C values of a row of tiles are still all
+      // read before any is written.
+#pragma unroll
+      for (int i = 0; i < TIM; ++i) {
+        const int row = m0 + wm + i * 16 + 4 * lk;
+        T cv[TIN][CV];
+        if (beta != T(0)) {
+#pragma unroll
+          for (int j = 0; j < TIN; ++j) load_run(row, n0 + wn + j * 16 + lr, cv[j]);
+        }
+#pragma unroll
+        for (int j = 0; j < TIN; ++j) {
+          const int col = n0 + wn + j * 16 + lr;
+          T out[CV];
+#pragma unroll
+          for (int e = 0; e < CV; ++e) {
+            const T prod = alpha * acc[i][j][e];
+            out[e] = beta == T(0) ? prod : beta == T(1) ? cv[j][e] + prod : prod + beta * cv[j][e];
+          }
+          T *cp = cw + (int64_t)row + (int64_t)col * cs_w;
+          if (run_whole(row, col)) {
+            cvec_t t;
+#pragma unroll
+            for (int e = 0; e < CV; ++e) t[e] = out[e];
+            __builtin_memcpy(gmem(cp), &t, sizeof t);
+          } else {
+#pragma unroll
+            for (int e = 0; e < CV; ++e)
+              if (c_ok(row + e, col)) gmem(cp)[e] = out[e];
+          }
+        }
+      }
+      return;
+    }
+  }
 #pragma unroll
   for (int i = 0; i < TIM; ++i) {
     T cv[TIN][4];
